@@ -1,0 +1,194 @@
+/*
+ * ngx_backend.h -- the C ABI of the MI355X batched sensor-rendering backend.
+ *
+ * This is the drop-in boundary for the reference's batched-world path. The reference's plug-in point is
+ * the C++ interface detail::Backend (src/backend.h:36-85), built from a SceneInput (src/backend.h:19-34)
+ * and reached from BatchRenderer (src/batch.cpp:91-192). Each entry point below replaces one method of
+ * that interface; the citations say which. Plain C: pointers and sizes only, no exceptions across it.
+ *
+ * Conventions (the reference's, with the differences named):
+ *  - Errors: every call returns an ngx_status; NGX_INVALID_ARGUMENT is the reference's
+ *    std::invalid_argument (bad input, checked before anything changes, so a rejected call changes
+ *    nothing), NGX_DEVICE_ERROR its CudaError (src/include/neraulgraphics/renderer.h:32-39).
+ *    ngx_last_error() gives the message of the calling thread's last failure.
+ *  - Memory: NGX_MEM_HOST arrays are validated as the reference validates host arrays
+ *    (src/batch.cpp:91-192); NGX_MEM_DEVICE arrays (HIP device pointers) are trusted, as there.
+ *  - Streams: every launch goes on the hip stream given (NULL: the default stream) and nothing waits
+ *    for it, so a caller can capture a step into a HIP graph. ngx_synchronize() restores the
+ *    reference's "finished on return" contract (src/cuda_backend.cu:403-405, :542, :583).
+ *  - Results are in buffers the context owns, valid until the next call that writes them
+ *    (include/neraulgraphics/batch.h:122-126, renderer.h:85-87).
+ *  - A context is not thread-safe; the reference's Python edge serializes with a mutex per object
+ *    (python/bindings.cpp:242-251).
+ *
+ * Additions with no counterpart in the reference:
+ *  - per-world instance counts (instances_in_world): world w holds the first count[w] instances of the
+ *    template; the reference has the same list in every world (include/neraulgraphics/batch.h:7-9);
+ *  - ngx_pose_backward_depth / ngx_pose_backward_range: gradients of depth and range with respect to
+ *    body poses;
+ *  - ngx_debug_*: the top-level build's intermediates, for checking it against the reference algorithm.
+ */
+#ifndef NGX_BACKEND_H
+#define NGX_BACKEND_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ngx_ctx ngx_ctx;
+typedef int32_t ngx_status;
+
+enum {
+    NGX_OK = 0,
+    NGX_INVALID_ARGUMENT = 1,
+    NGX_DEVICE_ERROR = 2,
+    NGX_OUT_OF_MEMORY = 3,
+    NGX_BAD_STATE = 4,
+    NGX_UNSUPPORTED = 5
+};
+
+/* Memory (include/neraulgraphics/renderer.h:49). */
+typedef enum { NGX_MEM_HOST = 0, NGX_MEM_DEVICE = 1 } ngx_memory;
+
+/* Output bits (include/neraulgraphics/renderer.h:53-58). NGX_OUTPUT_RGB is not produced by this backend. */
+enum { NGX_OUTPUT_RGB = 1, NGX_OUTPUT_DEPTH = 2, NGX_OUTPUT_NORMAL = 4, NGX_OUTPUT_OBJECT_ID = 8 };
+
+/* Object IDs (include/neraulgraphics/scene.h:19-22). */
+enum { NGX_OBJECT_NONE = 0, NGX_OBJECT_SPHERE = 1, NGX_OBJECT_FLOOR = 33, NGX_OBJECT_INSTANCE = 34 };
+#define NGX_MAX_SPHERES 32
+
+/* The scene: src/backend.h:19-34 (SceneInput) with the meshes given raw, as BatchRenderer's constructor
+ * takes them (include/neraulgraphics/batch.h:164-167); the bottom-level trees are built here, once. */
+typedef struct {
+    int32_t width, height, worlds, cameras_per_world;
+
+    int32_t mesh_count;
+    const float *vertices;          /* every mesh's vertices, 3 floats each, mesh after mesh */
+    const int32_t *vertex_offsets;  /* mesh_count + 1: mesh m's vertices are vertex_offsets[m] .. [m+1]-1 */
+    const uint32_t *indices;        /* 3 per triangle, indexing the mesh's own vertices */
+    const int32_t *index_offsets;   /* mesh_count + 1 */
+    const float *normals;           /* per vertex like vertices, or NULL */
+    const uint8_t *mesh_has_normals;/* mesh_count flags (smooth shading), or NULL */
+
+    int32_t instances_per_world;    /* the template: what a full world holds */
+    const int32_t *instance_mesh;   /* instances_per_world mesh indices */
+    const float *instance_transforms; /* instances_per_world x 12: where every world starts */
+    const int32_t *instances_in_world;/* worlds counts in [0, instances_per_world], or NULL for all */
+
+    int32_t sphere_count;           /* <= NGX_MAX_SPHERES */
+    const float *spheres;           /* sphere_count x 4: centre x, y, z, radius */
+    float floor_y, floor_half_extent_x, floor_z_min, floor_z_max; /* half_extent_x 0: no floor */
+
+    int32_t device_id;              /* HIP device */
+} ngx_scene_desc;
+
+/* include/neraulgraphics/batch.h:126-136 */
+typedef struct {
+    int32_t memory;     /* ngx_memory: always NGX_MEM_DEVICE */
+    int32_t device_id;
+    int32_t views, width, height;
+    const uint8_t *rgb; /* always NULL */
+    const float *depth;
+    const float *normal;
+    const int32_t *object_id;
+} ngx_batch_frame;
+
+/* include/neraulgraphics/renderer.h:88-95 */
+typedef struct {
+    int32_t memory;
+    int32_t device_id;
+    int32_t count;
+    const float *distance;
+    const float *normal;
+    const int32_t *object_id;
+} ngx_ray_hits;
+
+/* make_cuda_backend (src/backend.h:88) with BatchRenderer's constructor checks (src/batch.cpp:47-75). */
+ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out);
+void ngx_destroy(ngx_ctx *ctx);
+
+/* Backend::set_instance_matrices (src/backend.h:62); host checks of src/batch.cpp:91-109.
+ * worlds x instances_per_world x 12 floats, world-major; entries beyond a world's count are ignored. */
+ngx_status ngx_set_instance_matrices(ngx_ctx *ctx, const float *matrices, ngx_memory memory, void *stream);
+
+/* Backend::bind_bodies (src/backend.h:63); checks of src/batch.cpp:111-129. Host arrays, per template
+ * instance: body index (-1..bodies_per_world-1), local pose (x, y, z, qx, qy, qz, qw), scale (x, y, z). */
+ngx_status ngx_bind_bodies(ngx_ctx *ctx, int32_t bodies_per_world, const int32_t *body_of_instance,
+                           const float *local_pose7, const float *scale3);
+
+/* Backend::set_body_poses (src/backend.h:65); checks of src/batch.cpp:131-144.
+ * worlds x bodies_per_world x 7 floats. Composed in FP64 as src/instances.cpp:137-164 does. */
+ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memory, void *stream);
+
+/* Backend::render_batch (src/backend.h:67-68); checks of src/batch.cpp:146-162. views x 12 camera-to-world
+ * floats (OpenGL convention). spp and frame only vary the RGB picture, which this backend does not draw. */
+ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, int32_t spp,
+                            uint32_t outputs, int32_t frame, void *stream, ngx_batch_frame *out);
+
+/* Backend::cast_rays_batch (src/backend.h:69-70); checks of src/batch.cpp:180-192.
+ * origins and directions: worlds x rays_per_world x 3 floats, both in `memory`. */
+ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *directions, int32_t rays_per_world,
+                               float max_distance, ngx_memory memory, void *stream, ngx_ray_hits *out);
+
+/* No counterpart in the reference: dL/d(body poses) of a depth render, through each pixel's fixed hit
+ * triangle. Device arrays: body_poses (worlds x B x 7, the poses to differentiate at; the top-level trees
+ * are rebuilt from them first), cams (views x 12), depth (views x H x W, the forward's output: it bounds
+ * the re-trace and marks misses), grad_depth (dL/d depth), grad_out (worlds x B x 7, overwritten). */
+ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov,
+                                   const float *depth, const float *grad_depth, float *grad_out, void *stream);
+
+/* No counterpart in the reference: the same for a ray query's distances. Device arrays. */
+ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const float *origins, const float *directions,
+                                   int32_t rays_per_world, float max_distance, const float *distance,
+                                   const float *grad_distance, float *grad_out, void *stream);
+
+/* Backend::copy_to_host (src/backend.h:54). Synchronous. */
+ngx_status ngx_copy_to_host(ngx_ctx *ctx, const void *src, void *dst, size_t bytes);
+
+/* Waits for everything queued on the context's last stream. */
+ngx_status ngx_synchronize(ngx_ctx *ctx);
+
+/* Backend::last_kernel_ms (src/backend.h:83): device time of the last render or ray query's kernel, in ms,
+ * measured with events; read after ngx_synchronize. */
+float ngx_last_kernel_ms(ngx_ctx *ctx);
+
+/* BatchRenderer's accessors (include/neraulgraphics/batch.h:214-221). */
+int32_t ngx_world_count(const ngx_ctx *ctx);
+int32_t ngx_view_count(const ngx_ctx *ctx);
+int32_t ngx_instances_per_world(const ngx_ctx *ctx);
+int32_t ngx_instance_count(const ngx_ctx *ctx);   /* over all worlds */
+int32_t ngx_device_id(const ngx_ctx *ctx);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *ngx_last_error(void);
+
+/* Tuning of the trace kernels: tile_w x tile_h pixels per wavefront (64 pixels: 8x8, 16x4, 32x2 or 64x1),
+ * and whether consecutive workgroups are kept on one XCD. */
+ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap);
+
+/* ---- build intermediates, for checking the top-level build against the reference algorithm ---- */
+
+/* Keep (1) or drop (0) the intermediates on later builds: Morton keys, sort order, boxes. */
+ngx_status ngx_debug_keep_build(ngx_ctx *ctx, int32_t keep);
+
+/* Copies the last build to host arrays (NULL skips one):
+ *  nodes (2N-1) x 8 x 4 bytes (TlasNode: lo xyz, left, hi xyz, right), roots (worlds),
+ *  inv (N x 12 world-to-mesh rows), matrices (N x 12), and, when kept: keys (N, world << 32 | morton, by
+ *  instance), sorted_keys (N), sorted_ids (N), aabb (N x 6), world_bounds (worlds x 6). */
+ngx_status ngx_debug_read_build(ngx_ctx *ctx, void *nodes, int32_t *roots, float *inv, float *matrices, uint64_t *keys,
+                                uint64_t *sorted_keys, int32_t *sorted_ids, float *aabb, float *world_bounds);
+
+/* The host-side bottom-level build of one mesh (src/mesh_build.cpp:91-157 with src/bvh.cpp:46-174), no device
+ * needed: nodes (capacity 2 x triangles) x 32 bytes in the reference's Node layout, triangles
+ * (triangles x 12 floats, v0 e1 e2 padded), order (triangles). Returns the node count in *node_count. */
+ngx_status ngx_build_blas_host(const float *vertices, int32_t vertex_count, const uint32_t *indices, int32_t triangle_count,
+                               void *nodes, int32_t *node_count, float *triangles, uint32_t *order);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGX_BACKEND_H */

@@ -1,0 +1,297 @@
+"""BatchRenderer: the Python edge of the MI355X backend, with the surface of the reference's
+ng.BatchRenderer (python/bindings.cpp:667-800): set_instance_transforms, bind_bodies, set_body_poses,
+render, cast_rays and the same properties. Inputs may be NumPy arrays (checked, copied to the device) or
+torch tensors on the GPU (used where they are). Device work is queued on torch's current stream.
+
+Differences from the reference, all deliberate:
+  * the RGB picture is not drawn (rgb defaults to False here; asking for it raises NotImplementedError);
+  * instances_in_world gives every world its own count of the template's first instances;
+  * pose_backward_depth / pose_backward_range give gradients with respect to body poses.
+Errors: ValueError for bad input (the reference's std::invalid_argument), RuntimeError for device failures."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .dlpack import to_torch
+from .geometry import Instance, Mesh, Plane
+
+_T = None
+
+
+def _torch():
+    global _T
+    if _T is None:
+        import torch
+        _T = torch
+    return _T
+
+
+def _is_device_tensor(a):
+    t = _torch() if not isinstance(a, np.ndarray) else None
+    return t is not None and isinstance(a, t.Tensor) and a.is_cuda
+
+
+def _stream():
+    t = _torch()
+    return ctypes.c_void_p(t.cuda.current_stream().cuda_stream)
+
+
+def _as_input(a, what, count=None, dtype=np.float32):
+    """(pointer, memory, keepalive) for a NumPy array or a device tensor of float32 values."""
+    if _is_device_tensor(a):
+        t = _torch()
+        want = t.float32 if dtype == np.float32 else t.int32
+        if a.dtype != want:
+            raise ValueError(f"{what} must be {np.dtype(dtype).name}, got {a.dtype}")
+        a = a.contiguous()
+        if count is not None and a.numel() != count:
+            raise ValueError(f"{what} has {a.numel()} values, but {count} are needed")
+        return ctypes.c_void_p(a.data_ptr()), _lib.NGX_MEM_DEVICE, a
+    arr = np.ascontiguousarray(np.asarray(a, dtype=dtype))
+    if count is not None and arr.size != count:
+        raise ValueError(f"{what} has {arr.size} values, but {count} are needed")
+    return arr.ctypes.data_as(ctypes.c_void_p), _lib.NGX_MEM_HOST, arr
+
+
+class BatchRenderer:
+    """Many worlds rendered at once: thousands of copies of one environment, each seen by its own camera(s).
+
+    All worlds share the meshes, spheres and floor. Each holds the template's instances (its first
+    instances_in_world[w] of them), placed differently, and a camera sees only its own world's. Views are
+    numbered world-major: view v looks at world v // cameras_per_world."""
+
+    def __init__(self, width, height, world_count, cameras_per_world=1, spheres=(), lights=(), floor=None, meshes=(),
+                 instances=(), textures=(), device="auto", instances_in_world=None, device_id=None):
+        lib = _lib.load()
+        if device not in ("auto", "cuda", "gpu"):
+            raise ValueError(f"this backend runs on the GPU; device {device!r} is not one it serves")
+        if textures:
+            raise NotImplementedError("textures only change the RGB picture, which this backend does not draw")
+        floor = floor if floor is not None else Plane()
+        meshes = [m if isinstance(m, Mesh) else Mesh(*m) for m in meshes]
+        instances = [i if isinstance(i, Instance) else Instance(*i) for i in instances]
+        if device_id is None:
+            try:
+                device_id = _torch().cuda.current_device()
+            except Exception:
+                device_id = 0
+        self._keep = []
+        vert = np.concatenate([m.vertices for m in meshes]).astype(np.float32) if meshes else np.zeros((1, 3), np.float32)
+        vo = np.zeros(len(meshes) + 1, np.int32)
+        io = np.zeros(len(meshes) + 1, np.int32)
+        for k, m in enumerate(meshes):
+            vo[k + 1] = vo[k] + len(m.vertices)
+            io[k + 1] = io[k] + m.indices.size
+        idx = np.concatenate([m.indices for m in meshes]).astype(np.uint32) if meshes else np.zeros(1, np.uint32)
+        has_n = np.array([m.normals is not None for m in meshes] or [0], np.uint8)
+        nrm = np.concatenate([m.normals if m.normals is not None else np.zeros_like(m.vertices) for m in meshes]) \
+            if meshes and has_n.any() else None
+        inst_mesh = np.array([i.mesh for i in instances] or [0], np.int32)
+        inst_tf = np.concatenate([i.transform.matrix.reshape(1, 12) for i in instances]).astype(np.float32) \
+            if instances else np.zeros((1, 12), np.float32)
+        sph = np.array([[*s.center, s.radius] for s in spheres], np.float32).reshape(-1, 4) if spheres else np.zeros((1, 4), np.float32)
+        counts = None
+        if instances_in_world is not None:
+            counts = np.ascontiguousarray(np.asarray(instances_in_world, dtype=np.int32))
+            if counts.shape != (world_count,):
+                raise ValueError(f"instances_in_world must hold one count for each of the {world_count} worlds")
+        self._keep += [vert, vo, io, idx, has_n, nrm, inst_mesh, inst_tf, sph, counts]
+
+        def ptr(a, t):
+            return a.ctypes.data_as(t) if a is not None else ctypes.cast(None, t)
+
+        desc = _lib.SceneDesc(
+            width, height, world_count, cameras_per_world,
+            len(meshes), ptr(vert, _lib.c_float_p), ptr(vo, _lib.c_int32_p), ptr(idx, _lib.c_uint32_p),
+            ptr(io, _lib.c_int32_p), ptr(nrm, _lib.c_float_p), ptr(has_n, _lib.c_uint8_p),
+            len(instances), ptr(inst_mesh, _lib.c_int32_p), ptr(inst_tf, _lib.c_float_p), ptr(counts, _lib.c_int32_p),
+            len(spheres), ptr(sph, _lib.c_float_p),
+            float(floor.y), float(floor.half_extent_x), float(floor.z_min), float(floor.z_max),
+            int(device_id))
+        handle = ctypes.c_void_p()
+        _lib.check(lib.ngx_create(ctypes.byref(desc), ctypes.byref(handle)))
+        self._h = handle
+        self._lib = lib
+        self._width, self._height = width, height
+        self._worlds, self._cpw = world_count, cameras_per_world
+        self._per_world = len(instances)
+        self._bodies = None
+        self._device_id = int(device_id)
+        self._keep = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.ngx_destroy(h)
+            self._h = None
+
+    # ---- properties (python/bindings.cpp:787-794)
+    width = property(lambda self: self._width)
+    height = property(lambda self: self._height)
+    world_count = property(lambda self: self._worlds)
+    cameras_per_world = property(lambda self: self._cpw)
+    view_count = property(lambda self: self._worlds * self._cpw)
+    instances_per_world = property(lambda self: self._per_world)
+    device = property(lambda self: "cuda")
+    bodies_per_world = property(lambda self: self._bodies)
+
+    @property
+    def instance_count(self):
+        return int(self._lib.ngx_instance_count(self._h))
+
+    @property
+    def last_kernel_ms(self):
+        return float(self._lib.ngx_last_kernel_ms(self._h))
+
+    def synchronize(self):
+        _lib.check(self._lib.ngx_synchronize(self._h))
+
+    def set_trace_config(self, tile_w=8, xcd_remap=True):
+        _lib.check(self._lib.ngx_set_trace_config(self._h, int(tile_w), int(bool(xcd_remap))))
+
+    # ---- poses
+    def set_instance_transforms(self, matrices):
+        """(world_count, instances_per_world, 3, 4) float32: the top three rows of each instance's 4x4 transform."""
+        n = self._worlds * self._per_world * 12
+        p, mem, keep = _as_input(matrices, "matrices (world_count, instances_per_world, 3, 4)", n)
+        _lib.check(self._lib.ngx_set_instance_matrices(self._h, p, mem, self._stream_for(mem)))
+        self._hold(keep, mem)
+
+    def bind_bodies(self, body_of_instance, local_pose, scale, bodies_per_world):
+        boi = np.ascontiguousarray(np.asarray(body_of_instance, dtype=np.int32).reshape(-1))
+        lp = np.ascontiguousarray(np.asarray(local_pose, dtype=np.float32).reshape(-1))
+        sc = np.ascontiguousarray(np.asarray(scale, dtype=np.float32).reshape(-1))
+        n = self._per_world
+        if boi.size != n or lp.size != 7 * n or sc.size != 3 * n:
+            raise ValueError(f"a body binding needs one body index, 7 pose floats and 3 scale floats for each of the {n} "
+                             "instances of a world")
+        _lib.check(self._lib.ngx_bind_bodies(self._h, int(bodies_per_world), boi.ctypes.data_as(ctypes.c_void_p),
+                                             lp.ctypes.data_as(ctypes.c_void_p), sc.ctypes.data_as(ctypes.c_void_p)))
+        self._bodies = int(bodies_per_world)
+
+    def set_body_poses(self, body_poses):
+        """(world_count, bodies_per_world, 7) float32: x, y, z, then a quaternion x, y, z, w."""
+        if self._bodies is None:
+            raise ValueError("set_body_poses needs bind_bodies to be called first")
+        n = self._worlds * self._bodies * 7
+        p, mem, keep = _as_input(body_poses, "body poses (world_count, bodies_per_world, 7)", n)
+        _lib.check(self._lib.ngx_set_body_poses(self._h, p, mem, self._stream_for(mem)))
+        self._hold(keep, mem)
+
+    # ---- images and rays
+    def render(self, cameras, fov=1.0, samples_per_pixel=1, *, rgb=False, depth=False, normals=False, object_ids=False,
+               frame=0, zero_copy=False):
+        """The images asked for, as a dict: 'depth' (views, H, W) float32, 'normals' (views, H, W, 3) float32 and
+        'object_ids' (views, H, W) int32. With zero_copy the values are torch tensors viewing the renderer's
+        buffers (overwritten by the next render); otherwise NumPy copies."""
+        outputs = (_lib.OUTPUT_RGB if rgb else 0) | (_lib.OUTPUT_DEPTH if depth else 0) | \
+                  (_lib.OUTPUT_NORMAL if normals else 0) | (_lib.OUTPUT_OBJECT_ID if object_ids else 0)
+        views = self.view_count
+        p, mem, keep = _as_input(cameras, f"cameras ({views} cameras of 3 x 4 numbers, camera-to-world)", views * 12)
+        if mem == _lib.NGX_MEM_DEVICE and not zero_copy:
+            raise ValueError("cameras on the GPU need zero_copy=True: the results then stay on the GPU too")
+        f = _lib.BatchFrame()
+        _lib.check(self._lib.ngx_render_batch(self._h, p, mem, float(fov), int(samples_per_pixel), outputs, int(frame),
+                                              self._stream_for(mem), ctypes.byref(f)))
+        self._hold(keep, mem)
+        v, h, w = views, self._height, self._width
+        shapes = {"depth": ((v, h, w), np.float32, f.depth), "normals": ((v, h, w, 3), np.float32, f.normal),
+                  "object_ids": ((v, h, w), np.int32, f.object_id)}
+        return self._outputs(shapes, zero_copy)
+
+    def cast_rays(self, origins, directions, max_distance=float("inf"), zero_copy=None):
+        """Rays in every world: origins and directions (world_count, rays_per_world, 3). Returns 'distance'
+        (world_count, rays_per_world), 'normal' (..., 3) and 'object_id'. Device tensors in give device tensors
+        out, with no copy."""
+        w = self._worlds
+        po, mo, ko = _as_input(origins, "origins")
+        pd, md, kd = _as_input(directions, "directions")
+        n_o = ko.numel() if mo == _lib.NGX_MEM_DEVICE else ko.size
+        n_d = kd.numel() if md == _lib.NGX_MEM_DEVICE else kd.size
+        if n_o % (3 * w) != 0:
+            raise ValueError(f"origins must hold 3 floats for each ray of each of the {w} worlds: (world_count, rays_per_world, 3)")
+        if n_d != n_o:
+            raise ValueError("directions must have the same shape as origins")
+        if mo != md:
+            raise ValueError("origins and directions must be on the same device")
+        r = n_o // (3 * w)
+        hits = _lib.RayHits()
+        _lib.check(self._lib.ngx_cast_rays_batch(self._h, po, pd, int(r), float(max_distance), mo, self._stream_for(mo),
+                                                 ctypes.byref(hits)))
+        self._hold((ko, kd), mo)
+        if zero_copy is None:
+            zero_copy = mo == _lib.NGX_MEM_DEVICE
+        shapes = {"distance": ((w, r), np.float32, hits.distance), "normal": ((w, r, 3), np.float32, hits.normal),
+                  "object_id": ((w, r), np.int32, hits.object_id)}
+        return self._outputs(shapes, zero_copy)
+
+    # ---- gradients (no counterpart in the reference)
+    def pose_backward_depth(self, body_poses, cameras, fov, depth, grad_depth):
+        """dL/d(body_poses) given dL/d(depth) of a render from these poses and cameras (all torch device tensors)."""
+        t = _torch()
+        if self._bodies is None:
+            raise ValueError("the pose backward needs bind_bodies first")
+        grad = t.empty((self._worlds, self._bodies, 7), dtype=t.float32, device=body_poses.device)
+        args = [x.contiguous() for x in (body_poses, cameras, depth, grad_depth)]
+        _lib.check(self._lib.ngx_pose_backward_depth(self._h, args[0].data_ptr(), args[1].data_ptr(), float(fov),
+                                                     args[2].data_ptr(), args[3].data_ptr(), grad.data_ptr(), _stream()))
+        return grad
+
+    def pose_backward_range(self, body_poses, origins, directions, max_distance, distance, grad_distance):
+        t = _torch()
+        if self._bodies is None:
+            raise ValueError("the pose backward needs bind_bodies first")
+        grad = t.empty((self._worlds, self._bodies, 7), dtype=t.float32, device=body_poses.device)
+        args = [x.contiguous() for x in (body_poses, origins, directions, distance, grad_distance)]
+        r = args[1].numel() // (3 * self._worlds)
+        _lib.check(self._lib.ngx_pose_backward_range(self._h, args[0].data_ptr(), args[1].data_ptr(), args[2].data_ptr(), int(r),
+                                                     float(max_distance), args[3].data_ptr(), args[4].data_ptr(),
+                                                     grad.data_ptr(), _stream()))
+        return grad
+
+    # ---- build read-back, for checking the top-level build
+    def keep_build_intermediates(self, keep=True):
+        _lib.check(self._lib.ngx_debug_keep_build(self._h, int(bool(keep))))
+
+    def read_build(self, intermediates=False):
+        from .layouts import TLAS_NODE
+        n = self.instance_count
+        out = {"nodes": np.zeros(max(2 * n - 1, 0), TLAS_NODE), "roots": np.zeros(self._worlds, np.int32),
+               "inv": np.zeros((n, 12), np.float32), "matrices": np.zeros((n, 12), np.float32)}
+        if intermediates:
+            out.update(keys=np.zeros(n, np.uint64), sorted_keys=np.zeros(n, np.uint64), sorted_ids=np.zeros(n, np.int32),
+                       aabb=np.zeros((n, 6), np.float32), world_bounds=np.zeros((self._worlds, 6), np.float32))
+        names = ["nodes", "roots", "inv", "matrices", "keys", "sorted_keys", "sorted_ids", "aabb", "world_bounds"]
+        ptrs = [out[k].ctypes.data_as(ctypes.c_void_p) if k in out else None for k in names]
+        _lib.check(self._lib.ngx_debug_read_build(self._h, *ptrs))
+        return out
+
+    # ---- helpers
+    def _stream_for(self, mem):
+        if mem == _lib.NGX_MEM_DEVICE:
+            return _stream()
+        try:
+            return _stream()
+        except Exception:
+            return ctypes.c_void_p(None)
+
+    def _hold(self, keep, mem):
+        # host inputs are copied asynchronously from pageable memory: wait before the caller may reuse them
+        if mem == _lib.NGX_MEM_HOST:
+            self.synchronize()
+
+    def _outputs(self, shapes, zero_copy):
+        out = {}
+        if zero_copy:
+            for k, (shape, dt, ptr) in shapes.items():
+                if ptr:
+                    out[k] = to_torch(ptr, shape, dt, self._device_id, self)
+            return out
+        self.synchronize()
+        for k, (shape, dt, ptr) in shapes.items():
+            if ptr:
+                a = np.empty(shape, dt)
+                _lib.check(self._lib.ngx_copy_to_host(self._h, ctypes.c_void_p(ptr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+                out[k] = a
+        return out

@@ -1,0 +1,790 @@
+// The C ABI (include/ngx_backend.h): a context that owns every device buffer of a batch, checks input
+// the way the reference's front-end does (src/batch.cpp:47-192), and queues the kernels of ngx_tlas.hip
+// and ngx_trace.hip on the caller's stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "../../include/ngx_backend.h"
+#include "ngx_host.h"
+#include "ngx_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+    ngx_status status;
+    std::string message;
+};
+
+[[noreturn]] void fail(const ngx_status status, const std::string &message) { throw Failure{status, message}; }
+void require(const bool ok, const std::string &message) {
+    if (!ok) fail(NGX_INVALID_ARGUMENT, message);
+}
+void hip_check(const hipError_t err, const char *what) {
+    if (err == hipSuccess) return;
+    (void)hipGetLastError();
+    fail(err == hipErrorOutOfMemory ? NGX_OUT_OF_MEMORY : NGX_DEVICE_ERROR,
+         std::string(what) + " failed: " + hipGetErrorString(err) + " (" + hipGetErrorName(err) + ")");
+}
+#define HIP_CHECK(call) hip_check((call), #call)
+
+template <class Fn>
+ngx_status guarded(Fn &&fn) {
+    try {
+        fn();
+        g_last_error.clear();
+        return NGX_OK;
+    } catch (const Failure &f) {
+        g_last_error = f.message;
+        return f.status;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host out of memory";
+        return NGX_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return NGX_DEVICE_ERROR;
+    }
+}
+
+bool finite_all(const float *v, const size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+bool quaternion_usable(const float *q) {
+    return std::isfinite(q[0]) && std::isfinite(q[1]) && std::isfinite(q[2]) && std::isfinite(q[3]) &&
+           q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3] > 0.0f;
+}
+
+template <class T>
+void device_alloc(T *&ptr, const size_t count) {
+    if (ptr) {
+        (void)hipFree(ptr);
+        ptr = nullptr;
+    }
+    if (count == 0) return;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T)));
+}
+
+template <class T>
+void upload(T *&ptr, const T *src, const size_t count) {
+    device_alloc(ptr, count);
+    if (count) HIP_CHECK(hipMemcpy(ptr, src, count * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void release(T *&ptr) {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+}
+
+}  // namespace
+
+struct ngx_ctx {
+    int width = 0, height = 0, worlds = 0, cpw = 1, views = 0;
+    int per_world = 0, total = 0, max_count = 0, device = 0;
+    std::vector<int> counts, world_off;
+    int sphere_count = 0;
+    float floor[4] = {0, 0, 0, 0};
+
+    // scene, fixed at creation
+    int *d_world_off = nullptr;
+    uint8_t *d_root_at_end = nullptr;
+    int *d_tmpl_mesh = nullptr;
+    float *d_mesh_bounds = nullptr;
+    ngx::WideNode *d_blas_wide = nullptr;
+    ngx::MeshRoot *d_mesh_roots = nullptr;
+    float4 *d_tris = nullptr;
+    float4 *d_attrs = nullptr;
+    float4 *d_spheres = nullptr;
+
+    // the instances' state, rewritten by every pose update
+    float *d_matrices = nullptr;        // [N][12] as placed
+    float *d_matrix_stage = nullptr;    // [W][P][12] host matrices land here
+    ngx::InstanceRec *d_inst = nullptr;
+    ngx::TlasNode *d_nodes = nullptr;
+    ngx::WideNode *d_wide = nullptr;
+    int *d_roots = nullptr;
+
+    // bodies
+    bool bound = false;
+    int bodies = 0;
+    int *d_body_of_instance = nullptr;
+    float *d_local_pose = nullptr, *d_scale = nullptr, *d_local_rot_scale = nullptr;
+    float *d_body_stage = nullptr;
+    const float *last_body_poses = nullptr;
+
+    // outputs and staging, grown as needed
+    float *d_depth = nullptr, *d_normal = nullptr;
+    int32_t *d_id = nullptr;
+    float *d_cam_stage = nullptr;
+    float *d_ray_o = nullptr, *d_ray_d = nullptr;
+    long long ray_in_cap = 0;
+    float *d_hit_dist = nullptr, *d_hit_normal = nullptr;
+    int32_t *d_hit_id = nullptr;
+    long long ray_cap = 0;
+    float *d_partial = nullptr;
+    size_t partial_cap = 0;
+
+    // build intermediates
+    bool keep_build = false;
+    uint64_t *d_dbg_keys = nullptr, *d_dbg_sorted_keys = nullptr;
+    int *d_dbg_sorted_ids = nullptr;
+    float *d_dbg_aabb = nullptr, *d_dbg_world_bounds = nullptr;
+
+    // timing and tuning
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    hipStream_t last_stream = nullptr;
+    int tile_w = 8, xcd_remap = 1;
+
+    ~ngx_ctx() {
+        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids};
+        for (int *p : ints) if (p) (void)hipFree(p);
+        float *floats[] = {d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
+                           d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
+                           d_dbg_world_bounds};
+        for (float *p : floats) if (p) (void)hipFree(p);
+        void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
+                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys};
+        for (void *p : others) if (p) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+
+    ngx::TraceScene trace_scene() const {
+        ngx::TraceScene s{};
+        s.tlas_wide = d_wide;
+        s.roots = d_roots;
+        s.total = total;
+        s.inst = d_inst;
+        s.mesh_roots = d_mesh_roots;
+        s.blas_wide = d_blas_wide;
+        s.tris = d_tris;
+        s.attrs = d_attrs;
+        s.sphere_count = sphere_count;
+        s.spheres = d_spheres;
+        s.floor_y = floor[0];
+        s.floor_hx = floor[1];
+        s.floor_zmin = floor[2];
+        s.floor_zmax = floor[3];
+        return s;
+    }
+
+    ngx::TlasBuildArgs build_args() const {
+        ngx::TlasBuildArgs a{};
+        a.worlds = worlds;
+        a.total = total;
+        a.world_off = d_world_off;
+        a.root_at_end = d_root_at_end;
+        a.matrices_stride = per_world;
+        a.bodies_per_world = bodies;
+        a.body_of_instance = d_body_of_instance;
+        a.local_pose = d_local_pose;
+        a.scale = d_scale;
+        a.matrices_out = d_matrices;
+        a.tmpl_mesh = d_tmpl_mesh;
+        a.mesh_bounds = d_mesh_bounds;
+        a.inst = d_inst;
+        a.nodes = d_nodes;
+        a.wide = d_wide;
+        a.roots = d_roots;
+        if (keep_build) {
+            a.dbg_keys = d_dbg_keys;
+            a.dbg_sorted_keys = d_dbg_sorted_keys;
+            a.dbg_sorted_ids = d_dbg_sorted_ids;
+            a.dbg_aabb = d_dbg_aabb;
+            a.dbg_world_bounds = d_dbg_world_bounds;
+        }
+        return a;
+    }
+
+    void set_device() const { HIP_CHECK(hipSetDevice(device)); }
+
+    bool capturing(hipStream_t s) const {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+        return st != hipStreamCaptureStatusNone;
+    }
+
+    void build_from_matrices(const float *matrices_wp12, hipStream_t stream) {
+        if (total == 0) return;
+        ngx::TlasBuildArgs a = build_args();
+        a.from_bodies = 0;
+        a.matrices_in = matrices_wp12;
+        HIP_CHECK(ngx::launch_tlas_build(a, max_count, stream));
+        last_body_poses = nullptr;
+    }
+
+    void build_from_bodies(const float *poses, hipStream_t stream) {
+        if (total == 0) return;
+        ngx::TlasBuildArgs a = build_args();
+        a.from_bodies = 1;
+        a.body_poses = poses;
+        HIP_CHECK(ngx::launch_tlas_build(a, max_count, stream));
+        last_body_poses = poses;
+    }
+};
+
+extern "C" {
+
+const char *ngx_last_error(void) { return g_last_error.c_str(); }
+
+ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
+    return guarded([&] {
+        require(scene != nullptr && out != nullptr, "scene and out must not be null");
+        *out = nullptr;
+        const ngx_scene_desc &s = *scene;
+        require(s.width > 0 && s.height > 0,
+                "frame size must be positive, got " + std::to_string(s.width) + "x" + std::to_string(s.height));
+        require((long long)s.width * s.height <= INT32_MAX, "frame is too large");
+        require(s.worlds >= 1, "world_count must be 1 or more, got " + std::to_string(s.worlds));
+        require(s.cameras_per_world >= 1, "cameras_per_world must be 1 or more, got " + std::to_string(s.cameras_per_world));
+        const long long views = (long long)s.worlds * s.cameras_per_world;
+        require(views * s.height <= INT32_MAX, std::to_string(views) + " views of height " + std::to_string(s.height) + " are too many");
+        require(s.sphere_count >= 0 && s.sphere_count <= NGX_MAX_SPHERES,
+                "scene has " + std::to_string(s.sphere_count) + " spheres, at most " + std::to_string(NGX_MAX_SPHERES) + " fit");
+        for (int i = 0; i < s.sphere_count; ++i) {
+            require(finite_all(s.spheres + 4 * i, 4), "sphere " + std::to_string(i) + " has a NaN or infinite value");
+            require(s.spheres[4 * i + 3] > 0.0f, "sphere " + std::to_string(i) + " radius must be positive");
+        }
+        const float fl[4] = {s.floor_y, s.floor_half_extent_x, s.floor_z_min, s.floor_z_max};
+        require(finite_all(fl, 4), "floor has a NaN or infinite value");
+        require(s.mesh_count >= 0 && s.instances_per_world >= 0, "mesh and instance counts must not be negative");
+        for (int m = 0; m < s.mesh_count; ++m) {
+            const std::string name = "mesh " + std::to_string(m);
+            const int nv = s.vertex_offsets[m + 1] - s.vertex_offsets[m];
+            const int ni = s.index_offsets[m + 1] - s.index_offsets[m];
+            require(nv >= 0 && ni > 0, name + " has no triangles");
+            require(ni % 3 == 0, name + " has " + std::to_string(ni) + " indices, which is not a multiple of 3");
+            require(ni / 3 <= 100000000, name + " has too many triangles");
+            require(finite_all(s.vertices + 3 * (size_t)s.vertex_offsets[m], 3 * (size_t)nv), name + " has a NaN or infinite vertex");
+            for (int i = 0; i < ni; ++i)
+                require(s.indices[s.index_offsets[m] + i] < (uint32_t)nv,
+                        name + " index " + std::to_string(i) + " is " + std::to_string(s.indices[s.index_offsets[m] + i]) +
+                            ", but there are only " + std::to_string(nv) + " vertices");
+            if (s.normals && s.mesh_has_normals && s.mesh_has_normals[m])
+                require(finite_all(s.normals + 3 * (size_t)s.vertex_offsets[m], 3 * (size_t)nv), name + " has a NaN or infinite normal");
+        }
+        for (int i = 0; i < s.instances_per_world; ++i) {
+            const std::string name = "instance " + std::to_string(i);
+            require(s.instance_mesh[i] >= 0 && s.instance_mesh[i] < s.mesh_count,
+                    name + " uses mesh " + std::to_string(s.instance_mesh[i]) + ", but there are " + std::to_string(s.mesh_count) + " meshes");
+            require(finite_all(s.instance_transforms + 12 * (size_t)i, 12), name + " transform has a NaN or infinite value");
+            require(ngx::host::transform_ok(s.instance_transforms + 12 * (size_t)i),
+                    name + " transform is singular (it flattens space, so it cannot be inverted)");
+        }
+        std::vector<int> counts(s.worlds, s.instances_per_world);
+        if (s.instances_in_world)
+            for (int w = 0; w < s.worlds; ++w) {
+                require(s.instances_in_world[w] >= 0 && s.instances_in_world[w] <= s.instances_per_world,
+                        "world " + std::to_string(w) + " holds " + std::to_string(s.instances_in_world[w]) +
+                            " instances, but the template has " + std::to_string(s.instances_per_world));
+                counts[w] = s.instances_in_world[w];
+            }
+        long long total = 0;
+        int max_count = 0;
+        for (const int c : counts) {
+            total += c;
+            max_count = std::max(max_count, c);
+        }
+        require(total <= INT32_MAX / 4, "too many instances over all worlds: " + std::to_string(total));
+        if (max_count > ngx::MAX_WORLD_INSTANCES)
+            fail(NGX_UNSUPPORTED, "a world of " + std::to_string(max_count) + " instances: this backend builds worlds of up to " +
+                                      std::to_string(ngx::MAX_WORLD_INSTANCES));
+
+        auto ctx = std::make_unique<ngx_ctx>();
+        ngx_ctx &c = *ctx;
+        c.width = s.width;
+        c.height = s.height;
+        c.worlds = s.worlds;
+        c.cpw = s.cameras_per_world;
+        c.views = (int)views;
+        c.per_world = s.instances_per_world;
+        c.device = s.device_id;
+        c.counts = counts;
+        c.world_off.assign(s.worlds + 1, 0);
+        for (int w = 0; w < s.worlds; ++w) c.world_off[w + 1] = c.world_off[w] + counts[w];
+        c.total = (int)total;
+        c.max_count = max_count;
+        c.sphere_count = s.sphere_count;
+        std::memcpy(c.floor, fl, sizeof(fl));
+        c.set_device();
+        HIP_CHECK(hipEventCreate(&c.ev0));
+        HIP_CHECK(hipEventCreate(&c.ev1));
+
+        if (s.sphere_count > 0) upload(c.d_spheres, reinterpret_cast<const float4 *>(s.spheres), (size_t)s.sphere_count);
+        if (c.total > 0) {
+            ngx::host::MeshInput mi{s.mesh_count, s.vertices, s.vertex_offsets, s.indices, s.index_offsets, s.normals, s.mesh_has_normals};
+            ngx::host::BuiltMeshes built;
+            ngx::host::build_meshes(mi, built);
+            upload(c.d_blas_wide, built.wide.data(), built.wide.size() ? built.wide.size() : 0);
+            if (!c.d_blas_wide) device_alloc(c.d_blas_wide, 1);
+            upload(c.d_mesh_roots, built.mesh_roots.data(), built.mesh_roots.size());
+            upload(c.d_tris, reinterpret_cast<const float4 *>(built.triangles.data()), built.triangles.size() / 4);
+            if (!built.attributes.empty())
+                upload(c.d_attrs, reinterpret_cast<const float4 *>(built.attributes.data()), built.attributes.size() / 4);
+            std::vector<float> mesh_bounds;
+            for (const auto &b : built.bounds) mesh_bounds.insert(mesh_bounds.end(), {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]});
+            upload(c.d_mesh_bounds, mesh_bounds.data(), mesh_bounds.size());
+            upload(c.d_tmpl_mesh, s.instance_mesh, (size_t)s.instances_per_world);
+            upload(c.d_world_off, c.world_off.data(), c.world_off.size());
+            // Which end of its run carries a world's root in the reference's global tree: the end next to the
+            // neighbouring world that shares fewer leading key bits with it (see ngx_tlas.hip).
+            std::vector<uint8_t> root_at_end(s.worlds, 0);
+            int prev = -1;
+            std::vector<int> next_nonempty(s.worlds, -1);
+            for (int w = s.worlds - 1, nx = -1; w >= 0; --w) {
+                next_nonempty[w] = nx;
+                if (counts[w] > 0) nx = w;
+            }
+            for (int w = 0; w < s.worlds; ++w) {
+                if (counts[w] == 0) continue;
+                const int p_prev = prev < 0 ? -1 : __builtin_clz((unsigned)(w ^ prev));
+                const int p_next = next_nonempty[w] < 0 ? -1 : __builtin_clz((unsigned)(w ^ next_nonempty[w]));
+                root_at_end[w] = p_next > p_prev ? 1 : 0;
+                prev = w;
+            }
+            upload(c.d_root_at_end, root_at_end.data(), root_at_end.size());
+            device_alloc(c.d_matrices, 12 * (size_t)c.total);
+            device_alloc(c.d_matrix_stage, 12 * (size_t)s.worlds * s.instances_per_world);
+            device_alloc(c.d_inst, (size_t)c.total);
+            device_alloc(c.d_nodes, 2 * (size_t)c.total - 1);
+            device_alloc(c.d_wide, std::max<size_t>(1, (size_t)c.total - 1));
+            // nodes that join worlds are never written by a build: give them a fixed value once
+            HIP_CHECK(hipMemset(c.d_nodes, 0, sizeof(ngx::TlasNode) * (2 * (size_t)c.total - 1)));
+            HIP_CHECK(hipMemset(c.d_wide, 0, sizeof(ngx::WideNode) * std::max<size_t>(1, (size_t)c.total - 1)));
+        }
+        device_alloc(c.d_roots, (size_t)s.worlds);
+        HIP_CHECK(hipMemset(c.d_roots, 0xFF, sizeof(int) * s.worlds));
+        if (c.total > 0) {
+            // every world starts where the template puts its instances
+            std::vector<float> start(12 * (size_t)s.worlds * s.instances_per_world);
+            for (int w = 0; w < s.worlds; ++w)
+                std::memcpy(&start[12 * (size_t)w * s.instances_per_world], s.instance_transforms, sizeof(float) * 12 * s.instances_per_world);
+            HIP_CHECK(hipMemcpy(c.d_matrix_stage, start.data(), start.size() * sizeof(float), hipMemcpyHostToDevice));
+            c.build_from_matrices(c.d_matrix_stage, nullptr);
+            HIP_CHECK(hipDeviceSynchronize());
+        }
+        *out = ctx.release();
+    });
+}
+
+void ngx_destroy(ngx_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    delete ctx;
+}
+
+ngx_status ngx_set_instance_matrices(ngx_ctx *ctx, const float *matrices, ngx_memory memory, void *stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        if (ctx->per_world == 0) return;
+        require(matrices != nullptr, "matrices must not be null");
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const float *src = matrices;
+        if (memory == NGX_MEM_HOST) {
+            for (int w = 0; w < ctx->worlds; ++w)
+                for (int i = 0; i < ctx->counts[w]; ++i)
+                    if (!ngx::host::transform_ok(matrices + 12 * ((size_t)w * ctx->per_world + i))) {
+                        const float *m = matrices + 12 * ((size_t)w * ctx->per_world + i);
+                        const std::string name = "world " + std::to_string(w) + " instance " + std::to_string(i) + " transform";
+                        require(finite_all(m, 12), name + " has a NaN or infinite value");
+                        require(false, name + " is singular (it flattens space, so it cannot be inverted)");
+                    }
+            HIP_CHECK(hipMemcpyAsync(ctx->d_matrix_stage, matrices, sizeof(float) * 12 * (size_t)ctx->worlds * ctx->per_world,
+                                     hipMemcpyHostToDevice, s));
+            src = ctx->d_matrix_stage;
+        }
+        ctx->last_stream = s;
+        ctx->build_from_matrices(src, s);
+    });
+}
+
+ngx_status ngx_bind_bodies(ngx_ctx *ctx, int32_t bodies_per_world, const int32_t *body_of_instance, const float *local_pose7,
+                           const float *scale3) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        const int n = ctx->per_world;
+        require(bodies_per_world >= 0, "bodies_per_world must be 0 or more");
+        require(n == 0 || (body_of_instance && local_pose7 && scale3),
+                "a body binding needs one body index, 7 pose floats and 3 scale floats for each of the " + std::to_string(n) +
+                    " instances of a world");
+        for (int i = 0; i < n; ++i) {
+            const int b = body_of_instance[i];
+            require(b >= -1 && b < bodies_per_world, "instance " + std::to_string(i) + " is on body " + std::to_string(b) +
+                                                         ", but there are " + std::to_string(bodies_per_world) + " bodies per world");
+            require(finite_all(local_pose7 + 7 * i, 7) && quaternion_usable(local_pose7 + 7 * i + 3),
+                    "instance " + std::to_string(i) + " has a pose with a NaN or infinite value, or a zero quaternion");
+            require(finite_all(scale3 + 3 * i, 3), "instance " + std::to_string(i) + " has a scale with a NaN or infinite value");
+        }
+        ctx->set_device();
+        HIP_CHECK(hipDeviceSynchronize());
+        upload(ctx->d_body_of_instance, body_of_instance, (size_t)n);
+        upload(ctx->d_local_pose, local_pose7, 7 * (size_t)n);
+        upload(ctx->d_scale, scale3, 3 * (size_t)n);
+        // Each instance in its body's frame, Rl diag(s) | pl, for the backward pass.
+        std::vector<float> lrs(12 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            const float *q = local_pose7 + 7 * i + 3;
+            double x = q[0], y = q[1], z = q[2], w = q[3];
+            const double nn = std::sqrt(x * x + y * y + z * z + w * w);
+            x /= nn; y /= nn; z /= nn; w /= nn;
+            const double r[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                                 2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                                 2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)};
+            for (int row = 0; row < 3; ++row) {
+                for (int col = 0; col < 3; ++col) lrs[12 * i + 4 * row + col] = (float)(r[3 * row + col] * scale3[3 * i + col]);
+                lrs[12 * i + 4 * row + 3] = local_pose7[7 * i + row];
+            }
+        }
+        upload(ctx->d_local_rot_scale, lrs.data(), lrs.size());
+        device_alloc(ctx->d_body_stage, 7 * (size_t)ctx->worlds * bodies_per_world);
+        ctx->bodies = bodies_per_world;
+        ctx->bound = true;
+    });
+}
+
+ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memory, void *stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(ctx->bound, "set_body_poses needs bind_bodies to be called first");
+        const size_t bodies = (size_t)ctx->worlds * ctx->bodies;
+        require(bodies == 0 || poses7 != nullptr, "body_poses must not be null");
+        if (memory == NGX_MEM_HOST)
+            for (size_t b = 0; b < bodies; ++b)
+                require(finite_all(poses7 + 7 * b, 7) && quaternion_usable(poses7 + 7 * b + 3),
+                        "body " + std::to_string(b) + " has a NaN or infinite value, or a zero quaternion");
+        if (ctx->per_world == 0) return;
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const float *src = poses7;
+        if (memory == NGX_MEM_HOST && bodies > 0) {
+            HIP_CHECK(hipMemcpyAsync(ctx->d_body_stage, poses7, 7 * sizeof(float) * bodies, hipMemcpyHostToDevice, s));
+            src = ctx->d_body_stage;
+        }
+        ctx->last_stream = s;
+        ctx->build_from_bodies(src, s);
+    });
+}
+
+ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, int32_t spp, uint32_t outputs,
+                            int32_t frame, void *stream, ngx_batch_frame *out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "context and out must not be null");
+        const unsigned all = NGX_OUTPUT_RGB | NGX_OUTPUT_DEPTH | NGX_OUTPUT_NORMAL | NGX_OUTPUT_OBJECT_ID;
+        require(outputs != 0, "render needs at least one output");
+        require((outputs & ~all) == 0, "unknown output bits in " + std::to_string(outputs));
+        require(spp >= 1, "samples_per_pixel must be 1 or more, got " + std::to_string(spp));
+        require(frame >= 0, "frame must be 0 or more, got " + std::to_string(frame));
+        require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
+        require(cams12 != nullptr, "camera_matrices must not be null");
+        if (memory == NGX_MEM_HOST) require(finite_all(cams12, 12 * (size_t)ctx->views), "a camera has a NaN or infinite value");
+        if (outputs & NGX_OUTPUT_RGB) fail(NGX_UNSUPPORTED, "this backend renders depth, normal and object ID, not the RGB picture");
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t pixels = (size_t)ctx->views * ctx->width * ctx->height;
+        if ((outputs & NGX_OUTPUT_DEPTH) && !ctx->d_depth) device_alloc(ctx->d_depth, pixels);
+        if ((outputs & NGX_OUTPUT_NORMAL) && !ctx->d_normal) device_alloc(ctx->d_normal, 3 * pixels);
+        if ((outputs & NGX_OUTPUT_OBJECT_ID) && !ctx->d_id) device_alloc(ctx->d_id, pixels);
+        const float *cams = cams12;
+        if (memory == NGX_MEM_HOST) {
+            if (!ctx->d_cam_stage) device_alloc(ctx->d_cam_stage, 12 * (size_t)ctx->views);
+            HIP_CHECK(hipMemcpyAsync(ctx->d_cam_stage, cams12, 12 * sizeof(float) * ctx->views, hipMemcpyHostToDevice, s));
+            cams = ctx->d_cam_stage;
+        }
+        ngx::RenderArgs a{};
+        a.s = ctx->trace_scene();
+        a.cams = cams;
+        a.fov_scale = std::tan(fov * 0.5f);
+        a.aspect = (float)ctx->width / (float)ctx->height;
+        a.width = ctx->width;
+        a.height = ctx->height;
+        a.cameras_per_world = ctx->cpw;
+        a.views = ctx->views;
+        a.tile_w = ctx->tile_w;
+        a.tile_h = 64 / ctx->tile_w;
+        a.tiles_x = (ctx->width + a.tile_w - 1) / a.tile_w;
+        a.tiles_per_view = a.tiles_x * ((ctx->height + a.tile_h - 1) / a.tile_h);
+        a.total_tiles = (long long)a.tiles_per_view * ctx->views;
+        a.depth = (outputs & NGX_OUTPUT_DEPTH) ? ctx->d_depth : nullptr;
+        a.normal = (outputs & NGX_OUTPUT_NORMAL) ? ctx->d_normal : nullptr;
+        a.object_id = (outputs & NGX_OUTPUT_OBJECT_ID) ? ctx->d_id : nullptr;
+        a.xcd_remap = ctx->xcd_remap;
+        const bool timed = !ctx->capturing(s);
+        if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
+        HIP_CHECK(ngx::launch_render(a, s));
+        if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
+        ctx->timed = timed;
+        ctx->last_stream = s;
+        *out = ngx_batch_frame{NGX_MEM_DEVICE, ctx->device, ctx->views, ctx->width, ctx->height, nullptr, a.depth, a.normal, a.object_id};
+    });
+}
+
+ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *directions, int32_t rays_per_world,
+                               float max_distance, ngx_memory memory, void *stream, ngx_ray_hits *out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "context and out must not be null");
+        require(rays_per_world >= 0, "rays_per_world must be 0 or more, got " + std::to_string(rays_per_world));
+        const long long count = (long long)rays_per_world * ctx->worlds;
+        require(count <= INT32_MAX / 3, "too many rays: " + std::to_string(count));
+        require(count == 0 || (origins != nullptr && directions != nullptr), "ray origins and directions must not be null");
+        require(max_distance > 0.0f, "max_distance must be positive, got " + std::to_string(max_distance));
+        if (memory == NGX_MEM_HOST) require(finite_all(origins, 3 * (size_t)count), "a ray origin has a NaN or infinite value");
+        ctx->set_device();
+        if (count == 0) {
+            *out = ngx_ray_hits{NGX_MEM_DEVICE, ctx->device, 0, nullptr, nullptr, nullptr};
+            return;
+        }
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const float *o = origins, *d = directions;
+        if (memory == NGX_MEM_HOST) {
+            if (count > ctx->ray_in_cap) {
+                HIP_CHECK(hipStreamSynchronize(s));
+                device_alloc(ctx->d_ray_o, 3 * (size_t)count);
+                device_alloc(ctx->d_ray_d, 3 * (size_t)count);
+                ctx->ray_in_cap = count;
+            }
+            HIP_CHECK(hipMemcpyAsync(ctx->d_ray_o, origins, 3 * sizeof(float) * count, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(ctx->d_ray_d, directions, 3 * sizeof(float) * count, hipMemcpyHostToDevice, s));
+            o = ctx->d_ray_o;
+            d = ctx->d_ray_d;
+        }
+        if (count > ctx->ray_cap) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            device_alloc(ctx->d_hit_dist, (size_t)count);
+            device_alloc(ctx->d_hit_normal, 3 * (size_t)count);
+            device_alloc(ctx->d_hit_id, (size_t)count);
+            ctx->ray_cap = count;
+        }
+        ngx::RayArgs a{};
+        a.s = ctx->trace_scene();
+        a.origins = o;
+        a.dirs = d;
+        a.rays_per_world = rays_per_world;
+        a.count = count;
+        a.max_distance = max_distance;
+        a.distance = ctx->d_hit_dist;
+        a.normal = ctx->d_hit_normal;
+        a.object_id = ctx->d_hit_id;
+        const bool timed = !ctx->capturing(s);
+        if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
+        HIP_CHECK(ngx::launch_rays(a, s));
+        if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
+        ctx->timed = timed;
+        ctx->last_stream = s;
+        *out = ngx_ray_hits{NGX_MEM_DEVICE, ctx->device, (int32_t)count, ctx->d_hit_dist, ctx->d_hit_normal, ctx->d_hit_id};
+    });
+}
+
+ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov, const float *depth,
+                                   const float *grad_depth, float *grad_out, void *stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(ctx->bound && ctx->bodies > 0, "the pose backward needs bodies: call bind_bodies first");
+        require(body_poses && cams12 && depth && grad_depth && grad_out, "device arrays must not be null");
+        require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t need = (size_t)ctx->views * ctx->bodies * 7;
+        if (need > ctx->partial_cap) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            device_alloc(ctx->d_partial, need);
+            ctx->partial_cap = need;
+        }
+        ctx->build_from_bodies(body_poses, s);
+        ngx::DepthBackwardArgs a{};
+        a.s = ctx->trace_scene();
+        a.cams = cams12;
+        a.fov_scale = std::tan(fov * 0.5f);
+        a.aspect = (float)ctx->width / (float)ctx->height;
+        a.width = ctx->width;
+        a.height = ctx->height;
+        a.cameras_per_world = ctx->cpw;
+        a.views = ctx->views;
+        a.depth = depth;
+        a.grad_depth = grad_depth;
+        a.bodies_per_world = ctx->bodies;
+        a.body_of_instance = ctx->d_body_of_instance;
+        a.local_rot_scale = ctx->d_local_rot_scale;
+        a.body_poses = body_poses;
+        a.partial = ctx->d_partial;
+        HIP_CHECK(ngx::launch_depth_backward(a, s));
+        ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw, ctx->d_partial, body_poses, grad_out};
+        HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
+        ctx->last_stream = s;
+    });
+}
+
+ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const float *origins, const float *directions,
+                                   int32_t rays_per_world, float max_distance, const float *distance, const float *grad_distance,
+                                   float *grad_out, void *stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(ctx->bound && ctx->bodies > 0, "the pose backward needs bodies: call bind_bodies first");
+        require(body_poses && origins && directions && distance && grad_distance && grad_out, "device arrays must not be null");
+        require(rays_per_world >= 0, "rays_per_world must be 0 or more");
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const int chunks = std::max(1, std::min(64, rays_per_world / 4096));
+        const size_t need = (size_t)ctx->worlds * chunks * ctx->bodies * 7;
+        if (need > ctx->partial_cap) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            device_alloc(ctx->d_partial, need);
+            ctx->partial_cap = need;
+        }
+        ctx->build_from_bodies(body_poses, s);
+        ngx::RangeBackwardArgs a{};
+        a.s = ctx->trace_scene();
+        a.origins = origins;
+        a.dirs = directions;
+        a.rays_per_world = rays_per_world;
+        a.chunks = chunks;
+        a.max_distance = max_distance;
+        a.distance = distance;
+        a.grad_distance = grad_distance;
+        a.bodies_per_world = ctx->bodies;
+        a.body_of_instance = ctx->d_body_of_instance;
+        a.local_rot_scale = ctx->d_local_rot_scale;
+        a.body_poses = body_poses;
+        a.worlds = ctx->worlds;
+        a.partial = ctx->d_partial;
+        HIP_CHECK(ngx::launch_range_backward(a, s));
+        ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, chunks, ctx->d_partial, body_poses, grad_out};
+        HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
+        ctx->last_stream = s;
+    });
+}
+
+ngx_status ngx_copy_to_host(ngx_ctx *ctx, const void *src, void *dst, size_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        ctx->set_device();
+        if (bytes) HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+ngx_status ngx_synchronize(ngx_ctx *ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        ctx->set_device();
+        HIP_CHECK(hipDeviceSynchronize());
+    });
+}
+
+float ngx_last_kernel_ms(ngx_ctx *ctx) {
+    if (!ctx || !ctx->timed) return 0.0f;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
+int32_t ngx_world_count(const ngx_ctx *ctx) { return ctx ? ctx->worlds : 0; }
+int32_t ngx_view_count(const ngx_ctx *ctx) { return ctx ? ctx->views : 0; }
+int32_t ngx_instances_per_world(const ngx_ctx *ctx) { return ctx ? ctx->per_world : 0; }
+int32_t ngx_instance_count(const ngx_ctx *ctx) { return ctx ? ctx->total : 0; }
+int32_t ngx_device_id(const ngx_ctx *ctx) { return ctx ? ctx->device : 0; }
+
+ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(tile_w == 8 || tile_w == 16 || tile_w == 32 || tile_w == 64, "tile_w must be 8, 16, 32 or 64");
+        ctx->tile_w = tile_w;
+        ctx->xcd_remap = xcd_remap ? 1 : 0;
+    });
+}
+
+ngx_status ngx_debug_keep_build(ngx_ctx *ctx, int32_t keep) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        ctx->set_device();
+        ctx->keep_build = keep != 0;
+        if (ctx->keep_build && !ctx->d_dbg_keys && ctx->total > 0) {
+            device_alloc(ctx->d_dbg_keys, (size_t)ctx->total);
+            device_alloc(ctx->d_dbg_sorted_keys, (size_t)ctx->total);
+            device_alloc(ctx->d_dbg_sorted_ids, (size_t)ctx->total);
+            device_alloc(ctx->d_dbg_aabb, 6 * (size_t)ctx->total);
+            device_alloc(ctx->d_dbg_world_bounds, 6 * (size_t)ctx->worlds);
+        }
+    });
+}
+
+ngx_status ngx_debug_read_build(ngx_ctx *ctx, void *nodes, int32_t *roots, float *inv, float *matrices, uint64_t *keys,
+                                uint64_t *sorted_keys, int32_t *sorted_ids, float *aabb, float *world_bounds) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        ctx->set_device();
+        HIP_CHECK(hipDeviceSynchronize());
+        const size_t n = (size_t)ctx->total;
+        if (roots) HIP_CHECK(hipMemcpy(roots, ctx->d_roots, sizeof(int) * ctx->worlds, hipMemcpyDeviceToHost));
+        if (n == 0) return;
+        if (nodes) HIP_CHECK(hipMemcpy(nodes, ctx->d_nodes, sizeof(ngx::TlasNode) * (2 * n - 1), hipMemcpyDeviceToHost));
+        if (matrices) HIP_CHECK(hipMemcpy(matrices, ctx->d_matrices, sizeof(float) * 12 * n, hipMemcpyDeviceToHost));
+        if (inv) {
+            std::vector<ngx::InstanceRec> rec(n);
+            HIP_CHECK(hipMemcpy(rec.data(), ctx->d_inst, sizeof(ngx::InstanceRec) * n, hipMemcpyDeviceToHost));
+            for (size_t g = 0; g < n; ++g)
+                for (int row = 0; row < 3; ++row) {
+                    inv[12 * g + 4 * row] = rec[g].inv[row].x;
+                    inv[12 * g + 4 * row + 1] = rec[g].inv[row].y;
+                    inv[12 * g + 4 * row + 2] = rec[g].inv[row].z;
+                    inv[12 * g + 4 * row + 3] = rec[g].inv[row].w;
+                }
+        }
+        const bool want_dbg = keys || sorted_keys || sorted_ids || aabb || world_bounds;
+        if (want_dbg) require(ctx->keep_build && ctx->d_dbg_keys, "build intermediates were not kept: call ngx_debug_keep_build(ctx, 1) first");
+        if (keys) HIP_CHECK(hipMemcpy(keys, ctx->d_dbg_keys, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+        if (sorted_keys) HIP_CHECK(hipMemcpy(sorted_keys, ctx->d_dbg_sorted_keys, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+        if (sorted_ids) HIP_CHECK(hipMemcpy(sorted_ids, ctx->d_dbg_sorted_ids, sizeof(int) * n, hipMemcpyDeviceToHost));
+        if (aabb) HIP_CHECK(hipMemcpy(aabb, ctx->d_dbg_aabb, sizeof(float) * 6 * n, hipMemcpyDeviceToHost));
+        if (world_bounds) HIP_CHECK(hipMemcpy(world_bounds, ctx->d_dbg_world_bounds, sizeof(float) * 6 * ctx->worlds, hipMemcpyDeviceToHost));
+    });
+}
+
+ngx_status ngx_build_blas_host(const float *vertices, int32_t vertex_count, const uint32_t *indices, int32_t triangle_count,
+                               void *nodes, int32_t *node_count, float *triangles, uint32_t *order) {
+    return guarded([&] {
+        require(vertices && indices && nodes && node_count && triangles && order, "arrays must not be null");
+        require(triangle_count > 0 && vertex_count > 0, "a mesh needs triangles");
+        for (int i = 0; i < 3 * triangle_count; ++i) require(indices[i] < (uint32_t)vertex_count, "index out of range");
+        std::vector<ngx::host::Box> tb(triangle_count);
+        for (int t = 0; t < triangle_count; ++t) {
+            ngx::host::Box b{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    b.lo[a] = std::fmin(b.lo[a], vertices[3 * (size_t)indices[3 * t + k] + a]);
+                    b.hi[a] = std::fmax(b.hi[a], vertices[3 * (size_t)indices[3 * t + k] + a]);
+                }
+            tb[t] = b;
+        }
+        std::vector<ngx::BlasNode> tree;
+        std::vector<uint32_t> ord;
+        const int nn = ngx::host::sah_build(tb.data(), triangle_count, 4, tree, ord);
+        std::memcpy(nodes, tree.data(), sizeof(ngx::BlasNode) * nn);
+        std::memcpy(order, ord.data(), sizeof(uint32_t) * ord.size());
+        *node_count = nn;
+        for (int k = 0; k < triangle_count; ++k) {
+            const uint32_t t = ord[k];
+            const float *a = vertices + 3 * (size_t)indices[3 * t], *b = vertices + 3 * (size_t)indices[3 * t + 1],
+                        *c = vertices + 3 * (size_t)indices[3 * t + 2];
+            float *o = triangles + 12 * (size_t)k;
+            const float packed[12] = {a[0], a[1], a[2], 0.0f, b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.0f,
+                                      c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.0f};
+            std::memcpy(o, packed, sizeof(packed));
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Kernel argument blocks and launchers of the MI355X backend. Included by the HIP translation units
+// that define the kernels and by the host code (ngx_api.cpp) that owns the buffers and launches them.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "ngx_device.h"
+
+namespace ngx {
+
+// The instances of a batch are world-major: world w holds template entries 0 .. count(w)-1 at global
+// positions world_off[w] .. world_off[w+1]-1. The reference has count(w) equal for every world.
+struct TlasBuildArgs {
+    int worlds;
+    int total;                 // N = world_off[worlds]
+    const int *world_off;      // [worlds + 1]
+    const uint8_t *root_at_end; // [worlds]: the world's subtree root is node world_off[w+1]-1, not world_off[w]
+
+    int from_bodies;           // 0: matrices_in; 1: body poses composed with the binding
+    const float *matrices_in;  // world w's instance i at 12 * (w * matrices_stride + i)
+    int matrices_stride;       // instances per world in matrices_in (the template size)
+    const float *body_poses;   // [worlds][bodies_per_world][7]
+    int bodies_per_world;
+    const int *body_of_instance; // [P]
+    const float *local_pose;   // [P][7]
+    const float *scale;        // [P][3]
+    float *matrices_out;       // [N][12]: every instance's matrix as placed
+
+    const int *tmpl_mesh;      // [P]
+    const float *mesh_bounds;  // [meshes][6]
+
+    InstanceRec *inst;         // [N]
+    TlasNode *nodes;           // [2N-1], the reference layout
+    WideNode *wide;            // [N-1], the traversal layout
+    int *roots;                // [worlds]: root in `nodes` (a leaf index >= N-1 for a one-instance world), -1 if empty
+
+    // Optional build intermediates, for checking the build against the reference algorithm (null: skipped).
+    uint64_t *dbg_keys;        // [N] by instance: world << 32 | morton
+    uint64_t *dbg_sorted_keys; // [N] in sorted order
+    int *dbg_sorted_ids;       // [N]
+    float *dbg_aabb;           // [N][6]
+    float *dbg_world_bounds;   // [worlds][6]: min and max of the instances' box centres
+};
+
+struct TraceScene {
+    const WideNode *tlas_wide;
+    const int *roots;
+    int total;
+    const InstanceRec *inst;
+    const MeshRoot *mesh_roots;
+    const WideNode *blas_wide;
+    const float4 *tris;        // 3 per triangle: v0, e1, e2
+    const float4 *attrs;       // 4 per triangle (normals + flags) or null
+    int sphere_count;
+    const float4 *spheres;     // centre xyz, radius
+    float floor_y, floor_hx, floor_zmin, floor_zmax;
+};
+
+struct RenderArgs {
+    TraceScene s;
+    const float *cams;         // [views][12]
+    float fov_scale;           // tanf(fov / 2), computed on the host as the reference's CPU backend does
+    float aspect;              // width / height
+    int width, height, cameras_per_world, views;
+    int tile_w, tile_h, tiles_x, tiles_per_view;
+    long long total_tiles;
+    float *depth;              // [views][H][W] or null
+    float *normal;             // [views][H][W][3] or null
+    int32_t *object_id;        // [views][H][W] or null
+    int xcd_remap;             // 1: consecutive tiles stay on one XCD
+};
+
+struct RayArgs {
+    TraceScene s;
+    const float *origins;      // [worlds][R][3]
+    const float *dirs;
+    int rays_per_world;
+    long long count;
+    float max_distance;
+    float *distance;
+    float *normal;
+    int32_t *object_id;
+};
+
+// The pose backward of depth: dL/d(body pose) for every world, through each pixel's fixed hit triangle.
+struct DepthBackwardArgs {
+    TraceScene s;
+    const float *cams;
+    float fov_scale, aspect;
+    int width, height, cameras_per_world, views;
+    const float *depth;        // the forward's depth: bounds the re-trace, and skips misses
+    const float *grad_depth;   // [views][H][W]
+    int bodies_per_world;
+    const int *body_of_instance;     // [P]
+    const float *local_rot_scale;    // [P][12]: Rl diag(s) | pl, the instance in its body's frame
+    const float *body_poses;   // [worlds][bodies][7], the poses the tree was built from
+    float *partial;            // [views][bodies][7]
+};
+
+struct RangeBackwardArgs {
+    TraceScene s;
+    const float *origins, *dirs;
+    int rays_per_world;
+    int chunks;                // per world
+    float max_distance;
+    const float *distance;     // the forward's distances
+    const float *grad_distance;
+    int bodies_per_world;
+    const int *body_of_instance;
+    const float *local_rot_scale;
+    const float *body_poses;
+    int worlds;
+    float *partial;            // [worlds][chunks][bodies][7]
+};
+
+struct PoseGradFinishArgs {
+    int worlds, bodies_per_world, parts;  // parts summed per world, in order
+    const float *partial;      // [worlds][parts][bodies][7]
+    const float *body_poses;   // [worlds][bodies][7]
+    float *grad;               // [worlds][bodies][7]
+};
+
+hipError_t launch_tlas_build(const TlasBuildArgs &a, int max_count, hipStream_t stream);
+hipError_t launch_render(const RenderArgs &a, hipStream_t stream);
+hipError_t launch_rays(const RayArgs &a, hipStream_t stream);
+hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream);
+hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream);
+hipError_t launch_pose_grad_finish(const PoseGradFinishArgs &a, hipStream_t stream);
+
+// The largest world the one-workgroup-per-world build takes.
+constexpr int MAX_WORLD_INSTANCES = 1024;
+
+}  // namespace ngx

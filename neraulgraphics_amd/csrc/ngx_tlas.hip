@@ -1,0 +1,381 @@
+// Top-level tree build for every world of a batch, one workgroup per world, everything in LDS.
+//
+// The reference builds one linear BVH over all worlds at once (src/cuda_tlas.cuh:215-306): seven
+// kernels plus a device-wide radix sort of (world << 32 | morton) keys, then finds each world's root in
+// the global tree. Because the world number sits above the Morton code, the sort never moves an
+// instance out of its world, and the Karras hierarchy (cuda_tlas.cuh:128-159) over a world's run of
+// keys depends only on those keys and on the positions used to tell equal keys apart. So this kernel
+// builds each world's subtree independently, in one launch for the whole batch:
+//
+//   place     body pose (FP64, instances.cpp:137-164) -> matrix, inverse (cuda_tlas.cuh:35-47),
+//             8-corner world box, the world's centre bounds (cuda_tlas.cuh:69-96)
+//   keys      30-bit Morton code of the box centre (cuda_tlas.cuh:99-115)
+//   sort      stable LSD radix sort, 4-bit digits, ranks from wave ballots (multisplit), staged in LDS
+//   hierarchy Karras 2012 per node, equal keys told apart by their GLOBAL positions exactly as the
+//             reference's common_prefix does (cuda_tlas.cuh:119-124)
+//   boxes     bottom-up union with LDS arrival counters (the fit_boxes climb, cuda_tlas.cuh:179-196)
+//   write     the reference's node layout at the reference's node indices, plus the 64-byte wide
+//             nodes the tracer reads
+//
+// Node numbering is the reference's: internal nodes are numbered by sorted position, leaf k is node
+// N-1+k. Inside world w (positions a..b) every internal node a < i < b is the same as in the global
+// tree. The world's own root covers exactly a..b; it carries index a or index b, whichever the global
+// tree does not give to the ancestor that joins this world to its neighbours, which the host decides
+// once from the neighbouring world numbers (root_at_end). The ancestors joining worlds are never
+// traversed (a ray starts at its world's root) and are not built.
+#include "ngx_kernels.h"
+
+namespace ngx {
+namespace {
+
+__device__ __forceinline__ int to_ordered(const float f) {
+    const int i = __float_as_int(f);
+    return i ^ ((i >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float from_ordered(const int i) { return __int_as_float(i ^ ((i >> 31) & 0x7fffffff)); }
+
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// Rotation of a quaternion x, y, z, w in FP64, normalized first; zero gives the identity (instances.cpp:126-133).
+__device__ __forceinline__ void quat_to_matrix_d(const float *q, double r[9]) {
+    double x = q[0], y = q[1], z = q[2], w = q[3];
+    const double n = sqrt(x * x + y * y + z * z + w * w);
+    if (n > 0.0) {
+        x /= n; y /= n; z /= n; w /= n;
+    } else {
+        x = y = z = 0.0; w = 1.0;
+    }
+    r[0] = 1 - 2 * (y * y + z * z); r[1] = 2 * (x * y - z * w);     r[2] = 2 * (x * z + y * w);
+    r[3] = 2 * (x * y + z * w);     r[4] = 1 - 2 * (x * x + z * z); r[5] = 2 * (y * z - x * w);
+    r[6] = 2 * (x * z - y * w);     r[7] = 2 * (y * z + x * w);     r[8] = 1 - 2 * (x * x + y * y);
+}
+
+// An instance's matrix from its body's pose, its own offset and its scale, in FP64 and rounded once, as the
+// reference's host does (instances.cpp:137-164); the reference's GPU path composes in FP32.
+__device__ void compose_body_pose(const float *body, const float *local, const float *scale, float *m) {
+    double rl[9], r[9], p[3] = {local[0], local[1], local[2]};
+    quat_to_matrix_d(local + 3, rl);
+    if (body == nullptr) {
+        for (int k = 0; k < 9; ++k) r[k] = rl[k];
+    } else {
+        double rb[9];
+        quat_to_matrix_d(body + 3, rb);
+        for (int row = 0; row < 3; ++row) {
+            for (int col = 0; col < 3; ++col)
+                r[3 * row + col] = rb[3 * row] * rl[col] + rb[3 * row + 1] * rl[3 + col] + rb[3 * row + 2] * rl[6 + col];
+            p[row] = (double)body[row] + rb[3 * row] * (double)local[0] + rb[3 * row + 1] * (double)local[1] +
+                     rb[3 * row + 2] * (double)local[2];
+        }
+    }
+    for (int row = 0; row < 3; ++row) {
+        for (int col = 0; col < 3; ++col) m[4 * row + col] = (float)(r[3 * row + col] * (double)scale[col]);
+        m[4 * row + 3] = (float)p[row];
+    }
+}
+
+__device__ __forceinline__ void invert_affine(const float *m, float *out) {
+    const float c00 = m[5] * m[10] - m[6] * m[9];
+    const float c01 = m[6] * m[8] - m[4] * m[10];
+    const float c02 = m[4] * m[9] - m[5] * m[8];
+    const float inv = 1.0f / (m[0] * c00 + m[1] * c01 + m[2] * c02);
+    const float r[9] = {c00 * inv, (m[2] * m[9] - m[1] * m[10]) * inv, (m[1] * m[6] - m[2] * m[5]) * inv,
+                        c01 * inv, (m[0] * m[10] - m[2] * m[8]) * inv, (m[2] * m[4] - m[0] * m[6]) * inv,
+                        c02 * inv, (m[1] * m[8] - m[0] * m[9]) * inv, (m[0] * m[5] - m[1] * m[4]) * inv};
+    for (int row = 0; row < 3; ++row) {
+        for (int col = 0; col < 3; ++col) out[4 * row + col] = r[3 * row + col];
+        out[4 * row + 3] = -(r[3 * row] * m[3] + r[3 * row + 1] * m[7] + r[3 * row + 2] * m[11]);
+    }
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Exclusive scan of data[0 .. count) in place, count <= blockDim.x; wave_tot has one slot per wave.
+__device__ void block_exclusive_scan(int *data, const int count, int *wave_tot) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x = tid < count ? data[tid] : 0;
+    int incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += wave_tot[w];
+    if (tid < count) data[tid] = base + incl - x;
+    __syncthreads();
+}
+
+template <int MAXN>
+struct BuildShared {
+    static constexpr int NW = MAXN / 64;
+    float box_lo[3][MAXN], box_hi[3][MAXN];    // instance boxes by template index
+    float node_lo[3][MAXN], node_hi[3][MAXN];  // interior nodes by local index
+    uint32_t key[2][MAXN];
+    uint32_t val[2][MAXN];
+    int left[MAXN], right[MAXN], arrivals[MAXN];
+    int parent[2 * MAXN];
+    int hist[16 * NW];
+    int wave_tot[NW];
+    int bounds[6][NW];
+    int world_bounds[6];
+};
+
+template <int MAXN>
+__global__ __launch_bounds__(MAXN) void tlas_build_kernel(const TlasBuildArgs A) {
+    __shared__ BuildShared<MAXN> sh;
+    constexpr int NW = MAXN / 64;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int a = A.world_off[w];
+    const int n = A.world_off[w + 1] - a;
+    const int N = A.total;
+    if (n == 0) {
+        if (tid == 0) A.roots[w] = -1;
+        return;
+    }
+    const bool has = tid < n;
+
+    // ---- place: matrix, inverse, box, vote on the centre bounds
+    int cen[3] = {to_ordered(1e30f), to_ordered(1e30f), to_ordered(1e30f)};
+    int cmax[3] = {to_ordered(-1e30f), to_ordered(-1e30f), to_ordered(-1e30f)};
+    if (has) {
+        const int g = a + tid;
+        float m[12];
+        if (A.from_bodies) {
+            const int b = A.body_of_instance[tid];
+            const float *body = b < 0 ? nullptr : A.body_poses + 7 * ((size_t)w * A.bodies_per_world + b);
+            compose_body_pose(body, A.local_pose + 7 * tid, A.scale + 3 * tid, m);
+        } else {
+            const float4 *mi = reinterpret_cast<const float4 *>(A.matrices_in + 12 * ((size_t)w * A.matrices_stride + tid));
+            const float4 r0 = mi[0], r1 = mi[1], r2 = mi[2];
+            m[0] = r0.x; m[1] = r0.y; m[2] = r0.z; m[3] = r0.w;
+            m[4] = r1.x; m[5] = r1.y; m[6] = r1.z; m[7] = r1.w;
+            m[8] = r2.x; m[9] = r2.y; m[10] = r2.z; m[11] = r2.w;
+        }
+        float4 *mo = reinterpret_cast<float4 *>(A.matrices_out + 12 * (size_t)g);
+        mo[0] = make_float4(m[0], m[1], m[2], m[3]);
+        mo[1] = make_float4(m[4], m[5], m[6], m[7]);
+        mo[2] = make_float4(m[8], m[9], m[10], m[11]);
+        float inv[12];
+        invert_affine(m, inv);
+        const int mesh = A.tmpl_mesh[tid];
+        InstanceRec rec;
+        rec.inv[0] = make_float4(inv[0], inv[1], inv[2], inv[3]);
+        rec.inv[1] = make_float4(inv[4], inv[5], inv[6], inv[7]);
+        rec.inv[2] = make_float4(inv[8], inv[9], inv[10], inv[11]);
+        rec.mesh = mesh;
+        rec.tmpl = tid;
+        rec.pad0 = rec.pad1 = 0;
+        A.inst[g] = rec;
+
+        const float *lb = A.mesh_bounds + 6 * mesh;
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+        for (int corner = 0; corner < 8; ++corner) {
+            const float p[3] = {(corner & 1) ? lb[3] : lb[0], (corner & 2) ? lb[4] : lb[1], (corner & 4) ? lb[5] : lb[2]};
+            for (int ax = 0; ax < 3; ++ax) {
+                const float v = m[4 * ax] * p[0] + m[4 * ax + 1] * p[1] + m[4 * ax + 2] * p[2] + m[4 * ax + 3];
+                lo[ax] = fminf(lo[ax], v);
+                hi[ax] = fmaxf(hi[ax], v);
+            }
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            sh.box_lo[ax][tid] = lo[ax];
+            sh.box_hi[ax][tid] = hi[ax];
+            const int c = to_ordered(0.5f * (lo[ax] + hi[ax]));
+            cen[ax] = c;
+            cmax[ax] = c;
+        }
+        if (A.dbg_aabb) {
+            float *o = A.dbg_aabb + 6 * (size_t)g;
+            for (int ax = 0; ax < 3; ++ax) { o[ax] = lo[ax]; o[3 + ax] = hi[ax]; }
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        const int mn = wave_min(cen[ax]), mx = wave_max(cmax[ax]);
+        if (lane == 0) { sh.bounds[ax][wave] = mn; sh.bounds[3 + ax][wave] = mx; }
+    }
+    __syncthreads();
+    if (tid < 6) {
+        int v = sh.bounds[tid][0];
+        for (int k = 1; k < NW; ++k) v = tid < 3 ? min(v, sh.bounds[tid][k]) : max(v, sh.bounds[tid][k]);
+        sh.world_bounds[tid] = v;
+        if (A.dbg_world_bounds) A.dbg_world_bounds[6 * w + tid] = from_ordered(v);
+    }
+    __syncthreads();
+
+    // ---- keys
+    if (has) {
+        uint32_t q[3];
+        for (int ax = 0; ax < 3; ++ax) {
+            const float lo = from_ordered(sh.world_bounds[ax]), hi = from_ordered(sh.world_bounds[3 + ax]);
+            const float centre = 0.5f * (sh.box_lo[ax][tid] + sh.box_hi[ax][tid]);
+            const float extent = hi - lo;
+            const float t = extent > 1e-20f ? (centre - lo) / extent : 0.0f;
+            q[ax] = (uint32_t)(fminf(fmaxf(t, 0.0f), 1.0f) * 1023.0f);
+        }
+        const uint32_t morton = (spread_bits(q[0]) << 2) | (spread_bits(q[1]) << 1) | spread_bits(q[2]);
+        sh.key[0][tid] = morton;
+        sh.val[0][tid] = (uint32_t)tid;
+        if (A.dbg_keys) A.dbg_keys[a + tid] = ((uint64_t)w << 32) | morton;
+    }
+
+    // ---- stable LSD radix sort of the n keys, 4 bits a pass; 30-bit keys need passes at bits 0..28
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int pass = 0; pass < 8; ++pass) {
+        const int src = pass & 1, dst = src ^ 1, shift = 4 * pass;
+        for (int i = tid; i < 16 * NW; i += MAXN) sh.hist[i] = 0;
+        __syncthreads();
+        const uint32_t k = has ? sh.key[src][tid] : 0u, v = has ? sh.val[src][tid] : 0u;
+        const uint32_t digit = (k >> shift) & 15u;
+        uint64_t peers = ballot(has);
+        for (int bit = 0; bit < 4; ++bit) {
+            const bool set = (digit >> bit) & 1u;
+            const uint64_t vote = ballot(has && set);
+            peers &= set ? vote : ~vote;
+        }
+        const int rank = popc64(peers & below);
+        if (has && rank == 0) sh.hist[digit * NW + wave] = popc64(peers);
+        __syncthreads();
+        block_exclusive_scan(sh.hist, 16 * NW, sh.wave_tot);
+        if (has) {
+            const int dest = sh.hist[digit * NW + wave] + rank;
+            sh.key[dst][dest] = k;
+            sh.val[dst][dest] = v;
+        }
+        __syncthreads();
+    }
+    // eight passes end in buffer 0
+    const uint32_t *key = sh.key[0];
+    const uint32_t *val = sh.val[0];
+    if (has) {
+        if (A.dbg_sorted_keys) A.dbg_sorted_keys[a + tid] = ((uint64_t)w << 32) | key[tid];
+        if (A.dbg_sorted_ids) A.dbg_sorted_ids[a + tid] = a + (int)val[tid];
+    }
+
+    // ---- leaves, and the one-instance world
+    if (has) {
+        const int id = (int)val[tid];
+        TlasNode leaf;
+        for (int ax = 0; ax < 3; ++ax) { leaf.lo[ax] = sh.box_lo[ax][id]; leaf.hi[ax] = sh.box_hi[ax][id]; }
+        leaf.left = ~(a + id);
+        leaf.right = -1;
+        A.nodes[(size_t)(N - 1) + a + tid] = leaf;
+    }
+    if (n == 1) {
+        if (tid == 0) A.roots[w] = N - 1 + a;
+        return;
+    }
+
+    // ---- Karras hierarchy over local positions 0..n-1 (global position a + i)
+    for (int i = tid; i < 2 * n - 1; i += MAXN) sh.parent[i] = -1;
+    if (tid < n - 1) sh.arrivals[tid] = 0;
+    __syncthreads();
+    if (tid < n - 1) {
+        const int i = tid;
+        auto delta = [&](const int p, const int q) -> int {
+            if (q < 0 || q >= n) return -1;
+            const uint32_t kp = key[p], kq = key[q];
+            if (kp == kq) return 64 + __clz((a + p) ^ (a + q));
+            return 32 + __clz(kp ^ kq);
+        };
+        const int d = delta(i, i + 1) > delta(i, i - 1) ? 1 : -1;
+        const int min_prefix = delta(i, i - d);
+        int max_length = 2;
+        while (delta(i, i + max_length * d) > min_prefix) max_length *= 2;
+        int length = 0;
+        for (int t = max_length / 2; t >= 1; t /= 2)
+            if (delta(i, i + (length + t) * d) > min_prefix) length += t;
+        const int j = i + length * d;
+        const int node_prefix = delta(i, j);
+        int s = 0;
+        for (int t = (length + 1) / 2;; t = (t + 1) / 2) {
+            if (delta(i, i + (s + t) * d) > node_prefix) s += t;
+            if (t == 1) break;
+        }
+        const int gamma = i + s * d + (d < 0 ? d : 0);
+        const int lo = min(i, j), hi = max(i, j);
+        const int l = (lo == gamma) ? n - 1 + gamma : gamma;
+        const int r = (hi == gamma + 1) ? n - 1 + gamma + 1 : gamma + 1;
+        sh.left[i] = l;
+        sh.right[i] = r;
+        sh.parent[l] = i;
+        sh.parent[r] = i;
+    }
+    __syncthreads();
+
+    // ---- boxes, bottom up: the second of two children to arrive at a node computes its box
+    if (has) {
+        int node = n - 1 + tid;
+        while (true) {
+            const int p = sh.parent[node];
+            if (p < 0) break;
+            const int prior = __hip_atomic_fetch_add(&sh.arrivals[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prior == 0) break;
+            const int l = sh.left[p], r = sh.right[p];
+            for (int ax = 0; ax < 3; ++ax) {
+                const float llo = l >= n - 1 ? sh.box_lo[ax][val[l - (n - 1)]] : sh.node_lo[ax][l];
+                const float lhi = l >= n - 1 ? sh.box_hi[ax][val[l - (n - 1)]] : sh.node_hi[ax][l];
+                const float rlo = r >= n - 1 ? sh.box_lo[ax][val[r - (n - 1)]] : sh.node_lo[ax][r];
+                const float rhi = r >= n - 1 ? sh.box_hi[ax][val[r - (n - 1)]] : sh.node_hi[ax][r];
+                sh.node_lo[ax][p] = fminf(llo, rlo);
+                sh.node_hi[ax][p] = fmaxf(lhi, rhi);
+            }
+            node = p;
+        }
+    }
+    __syncthreads();
+
+    // ---- write the interior nodes in both layouts
+    const int root_index = A.root_at_end[w] ? a + n - 1 : a;
+    if (tid == 0) A.roots[w] = root_index;
+    if (tid < n - 1) {
+        const int i = tid;
+        const int gi = i == 0 ? root_index : a + i;
+        const int l = sh.left[i], r = sh.right[i];
+        TlasNode node;
+        for (int ax = 0; ax < 3; ++ax) { node.lo[ax] = sh.node_lo[ax][i]; node.hi[ax] = sh.node_hi[ax][i]; }
+        node.left = l >= n - 1 ? N - 1 + a + (l - (n - 1)) : a + l;
+        node.right = r >= n - 1 ? N - 1 + a + (r - (n - 1)) : a + r;
+        A.nodes[gi] = node;
+        WideNode wn;
+        for (int ax = 0; ax < 3; ++ax) {
+            wn.l_lo[ax] = l >= n - 1 ? sh.box_lo[ax][val[l - (n - 1)]] : sh.node_lo[ax][l];
+            wn.l_hi[ax] = l >= n - 1 ? sh.box_hi[ax][val[l - (n - 1)]] : sh.node_hi[ax][l];
+            wn.r_lo[ax] = r >= n - 1 ? sh.box_lo[ax][val[r - (n - 1)]] : sh.node_lo[ax][r];
+            wn.r_hi[ax] = r >= n - 1 ? sh.box_hi[ax][val[r - (n - 1)]] : sh.node_hi[ax][r];
+        }
+        wn.left = l >= n - 1 ? ~(a + (int)val[l - (n - 1)]) : a + l;
+        wn.right = r >= n - 1 ? ~(a + (int)val[r - (n - 1)]) : a + r;
+        wn.left_count = wn.right_count = 1;
+        A.wide[gi] = wn;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_tlas_build(const TlasBuildArgs &a, const int max_count, hipStream_t stream) {
+    if (a.worlds <= 0) return hipSuccess;
+    if (max_count <= 64) tlas_build_kernel<64><<<a.worlds, 64, 0, stream>>>(a);
+    else if (max_count <= 128) tlas_build_kernel<128><<<a.worlds, 128, 0, stream>>>(a);
+    else if (max_count <= 256) tlas_build_kernel<256><<<a.worlds, 256, 0, stream>>>(a);
+    else if (max_count <= 512) tlas_build_kernel<512><<<a.worlds, 512, 0, stream>>>(a);
+    else if (max_count <= MAX_WORLD_INSTANCES) tlas_build_kernel<1024><<<a.worlds, 1024, 0, stream>>>(a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace ngx

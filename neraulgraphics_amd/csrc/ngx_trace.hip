@@ -1,0 +1,637 @@
+// Closest-hit tracing for batched worlds on CDNA4: depth / normal / object-ID images, ray queries, and the
+// pose backward pass.
+//
+// The reference walks each ray alone with a private stack of 128 + 64 entries (src/mesh_dev.h:113-159),
+// which on a 64-wide wavefront means 64 diverging walks and stacks spilled to scratch. Here a wavefront
+// walks as a packet: it holds 64 rays of one world (an 8x8 pixel tile, or 64 consecutive rays of a
+// query), fetches each node once for the whole wave through the scalar cache, and every lane tests the
+// node's boxes with its own ray and its own closest distance so far. A child is entered when any lane's
+// ray enters its box; the lanes that do not are masked out of it. The walk's stack is wave-uniform and
+// lives one entry per lane of a vector register (written and read with writelane/readlane), so nothing
+// spills. Each lane still finds exactly its own closest hit: a lane only tests triangles inside boxes its
+// ray enters before its current closest distance, and keeps a hit only when strictly closer
+// (mesh_dev.h:98), so the result equals the per-ray walk's except where two surfaces meet the ray at the
+// same distance.
+#include "ngx_kernels.h"
+
+namespace ngx {
+namespace {
+
+struct LaneHit {
+    float t;
+    int inst;
+    int tri;
+    float u, v;
+};
+
+// A wave-uniform stack of (node reference, lane mask) entries, entry k held in lane k & 63 of register
+// set k >> 6: a push is a per-lane select, a pop a readlane.
+template <int SETS>
+struct WaveStack {
+    int ref[SETS];
+    uint32_t lo[SETS], hi[SETS];
+    int sp;
+    __device__ __forceinline__ void init() {
+        sp = 0;
+        for (int s = 0; s < SETS; ++s) { ref[s] = 0; lo[s] = 0; hi[s] = 0; }
+    }
+    __device__ __forceinline__ bool full() const { return sp >= 64 * SETS; }
+    __device__ __forceinline__ void push(const int r, const uint64_t m) {
+        const int set = sp >> 6, slot = sp & 63;
+        const bool mine = lane_id() == slot;
+#pragma unroll
+        for (int s = 0; s < SETS; ++s)
+            if (s == set) {
+                ref[s] = mine ? r : ref[s];
+                lo[s] = mine ? (uint32_t)m : lo[s];
+                hi[s] = mine ? (uint32_t)(m >> 32) : hi[s];
+            }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int &r, uint64_t &m) {
+        --sp;
+        const int set = sp >> 6, slot = sp & 63;
+        int rr = 0;
+        uint32_t l = 0, h = 0;
+#pragma unroll
+        for (int s = 0; s < SETS; ++s)
+            if (s == set) {
+                rr = __builtin_amdgcn_readlane(ref[s], slot);
+                l = (uint32_t)__builtin_amdgcn_readlane((int)lo[s], slot);
+                h = (uint32_t)__builtin_amdgcn_readlane((int)hi[s], slot);
+            }
+        r = rr;
+        m = ((uint64_t)h << 32) | l;
+    }
+};
+
+struct Wide {
+    float l_lo[3], l_hi[3], r_lo[3], r_hi[3];
+    int left, right, lcount, rcount;
+};
+
+__device__ __forceinline__ Wide load_wide(const WideNode *nodes, const int index) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + index);
+    const float4 f0 = sload4(p), f1 = sload4(p + 1), f2 = sload4(p + 2);
+    const int4 f3 = sload4i(reinterpret_cast<const int4 *>(p + 3));
+    Wide w;
+    w.l_lo[0] = f0.x; w.l_lo[1] = f0.y; w.l_lo[2] = f0.z; w.l_hi[0] = f0.w;
+    w.l_hi[1] = f1.x; w.l_hi[2] = f1.y; w.r_lo[0] = f1.z; w.r_lo[1] = f1.w;
+    w.r_lo[2] = f2.x; w.r_hi[0] = f2.y; w.r_hi[1] = f2.z; w.r_hi[2] = f2.w;
+    w.left = f3.x; w.right = f3.y; w.lcount = f3.z; w.rcount = f3.w;
+    return w;
+}
+
+// Tests the triangles first .. first+count-1 for the lanes in mask (mesh_dev.h:186-199).
+__device__ __forceinline__ void leaf_triangles(const TraceScene &S, const int first, const int count, const uint64_t mask,
+                                               const int inst, const float3 lo, const float3 ld, float &tmax, LaneHit &h) {
+    const bool in = (mask >> lane_id()) & 1ull;
+    for (int j = first; j < first + count; ++j) {
+        const float4 *tp = S.tris + 3 * (size_t)j;
+        const float3 v0 = xyz(sload4(tp)), e1 = xyz(sload4(tp + 1)), e2 = xyz(sload4(tp + 2));
+        float t, bu, bv;
+        if (triangle_hit(v0, e1, e2, lo, ld, tmax, t, bu, bv) && in) {
+            tmax = t;
+            h.t = t;
+            h.inst = inst;
+            h.tri = j;
+            h.u = bu;
+            h.v = bv;
+        }
+    }
+}
+
+// One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
+// renormalized so t is shared), then the mesh's own tree.
+__device__ void visit_instance(const TraceScene &S, const int g, const uint64_t mask, const float3 o, const float3 d,
+                               float &tmax, LaneHit &h) {
+    const float4 *rp = reinterpret_cast<const float4 *>(S.inst + g);
+    const float4 r0 = sload4(rp), r1 = sload4(rp + 1), r2 = sload4(rp + 2);
+    const int4 meta = sload4i(reinterpret_cast<const int4 *>(rp + 3));
+    const float3 lo = mk3(dot3(xyz(r0), o) + r0.w, dot3(xyz(r1), o) + r1.w, dot3(xyz(r2), o) + r2.w);
+    const float3 ld = mk3(dot3(xyz(r0), d), dot3(xyz(r1), d), dot3(xyz(r2), d));
+    const float3 inv_d = safe_inverse(ld);
+    const float4 *mp = reinterpret_cast<const float4 *>(S.mesh_roots + meta.x);
+    const float4 m0 = sload4(mp), m1 = sload4(mp + 1);
+    const int lane = lane_id();
+    const bool in = (mask >> lane) & 1ull;
+    const uint64_t root_mask = ballot(in && box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, tmax) < NO_HIT);
+    if (root_mask == 0) return;
+    const int root_ref = __float_as_int(m0.w);
+    if (root_ref < 0) {
+        leaf_triangles(S, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h);
+        return;
+    }
+    WaveStack<1> st;
+    st.init();
+    int cur = root_ref;
+    uint64_t cm = root_mask;
+    while (true) {
+        const Wide w = load_wide(S.blas_wide, cur);
+        const bool on = (cm >> lane) & 1ull;
+        const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], lo, inv_d, tmax) : NO_HIT;
+        const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], lo, inv_d, tmax) : NO_HIT;
+        const uint64_t ml = ballot(dl < NO_HIT), mr = ballot(dr < NO_HIT);
+        int next_ref, next_count;
+        uint64_t next_mask;
+        if (ml != 0 && mr != 0) {
+            const uint64_t near_left = ballot(dl < NO_HIT && dl <= dr);
+            const bool left_first = 2 * popc64(near_left) >= popc64(ml | mr);
+            // a leaf child is pushed as its parent and side, and its run read back when popped
+            if (left_first) {
+                st.push(w.right >= 0 ? w.right : ~(2 * cur + 1), mr);
+                next_ref = w.left; next_count = w.lcount; next_mask = ml;
+            } else {
+                st.push(w.left >= 0 ? w.left : ~(2 * cur), ml);
+                next_ref = w.right; next_count = w.rcount; next_mask = mr;
+            }
+        } else if (ml != 0) {
+            next_ref = w.left; next_count = w.lcount; next_mask = ml;
+        } else if (mr != 0) {
+            next_ref = w.right; next_count = w.rcount; next_mask = mr;
+        } else {
+            next_ref = 0; next_count = 0; next_mask = 0;
+        }
+        // Leaves are tested where they are reached; then the walk resumes from the stack.
+        while (next_mask == 0 || next_ref < 0) {
+            if (next_mask != 0) leaf_triangles(S, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h);
+            if (st.sp == 0) return;
+            int r;
+            st.pop(r, next_mask);
+            if (r >= 0) {
+                next_ref = r;
+            } else {
+                const int parent = (~r) >> 1, side = (~r) & 1;
+                const Wide pw = load_wide(S.blas_wide, parent);
+                next_ref = side ? pw.right : pw.left;
+                next_count = side ? pw.rcount : pw.lcount;
+            }
+        }
+        cur = next_ref;
+        cm = next_mask;
+    }
+}
+
+// The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
+// distance to beat on entry and its closest distance on return.
+__device__ void trace_world(const TraceScene &S, const int world, const float3 o, const float3 d, const uint64_t live,
+                            float &tmax, LaneHit &h) {
+    if (live == 0) return;
+    const int root = uniform(S.roots[world]);
+    if (root < 0) return;
+    if (root >= S.total - 1) {
+        visit_instance(S, root - (S.total - 1), live, o, d, tmax, h);
+        return;
+    }
+    const float3 inv_d = safe_inverse(d);
+    const int lane = lane_id();
+    WaveStack<2> st;
+    st.init();
+    int cur = root;
+    uint64_t cm = live;
+    while (true) {
+        const Wide w = load_wide(S.tlas_wide, cur);
+        const bool on = (cm >> lane) & 1ull;
+        const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], o, inv_d, tmax) : NO_HIT;
+        const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], o, inv_d, tmax) : NO_HIT;
+        const uint64_t ml = ballot(dl < NO_HIT), mr = ballot(dr < NO_HIT);
+        int next_ref;
+        uint64_t next_mask;
+        if (ml != 0 && mr != 0) {
+            const uint64_t near_left = ballot(dl < NO_HIT && dl <= dr);
+            const bool left_first = 2 * popc64(near_left) >= popc64(ml | mr);
+            if (left_first) {
+                st.push(w.right, mr);
+                next_ref = w.left; next_mask = ml;
+            } else {
+                st.push(w.left, ml);
+                next_ref = w.right; next_mask = mr;
+            }
+        } else if (ml != 0) {
+            next_ref = w.left; next_mask = ml;
+        } else if (mr != 0) {
+            next_ref = w.right; next_mask = mr;
+        } else {
+            next_ref = 0; next_mask = 0;
+        }
+        while (next_mask == 0 || next_ref < 0) {
+            if (next_mask != 0) visit_instance(S, ~next_ref, next_mask, o, d, tmax, h);
+            if (st.sp == 0) return;
+            st.pop(next_ref, next_mask);
+        }
+        cur = next_ref;
+        cm = next_mask;
+    }
+}
+
+// Spheres, then the floor, then the meshes with the best so far as their limit (render_core.h:78-112).
+// Returns the object, or OBJECT_NONE; t and h describe the hit.
+__device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, const float3 o, const float3 d,
+                                         const uint64_t live, float &best, LaneHit &h, const float start = 1e30f) {
+    best = start;
+    int best_sphere = -1;
+    for (int i = 0; i < S.sphere_count; ++i) {
+        const float4 sp = sload4(S.spheres + i);
+        const float3 L = sub3(xyz(sp), o);
+        const float tca = dot3(L, d);
+        const float r2 = sp.w * sp.w;
+        const float d2 = dot3(L, L) - tca * tca;
+        if (d2 > r2) continue;
+        const float thc = sqrtf(r2 - d2);
+        float t = tca - thc;
+        if (t < 0.0f) t = tca + thc;
+        if (t >= 0.0f && t < best) {
+            best = t;
+            best_sphere = i;
+        }
+    }
+    bool hit_floor = false;
+    if (fabsf(d.y) > 1e-3f) {
+        const float dist = (S.floor_y - o.y) / d.y;
+        if (dist > 0.0f && dist < best) {
+            const float3 pt = add3(o, mul3(d, dist));
+            if (fabsf(pt.x) < S.floor_hx && pt.z > S.floor_zmin && pt.z < S.floor_zmax) {
+                best = dist;
+                hit_floor = true;
+            }
+        }
+    }
+    h.inst = -1;
+    h.tri = -1;
+    trace_world(S, world, o, d, live, best, h);
+    if (h.inst >= 0) return OBJECT_INSTANCE;
+    if (hit_floor) return OBJECT_FLOOR;
+    if (best_sphere >= 0) return OBJECT_SPHERE + best_sphere;
+    return OBJECT_NONE;
+}
+
+// Normal and object ID of a hit (render_core.h:119-141, mesh_dev.h:238-262).
+__device__ __forceinline__ void surface(const TraceScene &S, const int kind, const float3 o, const float3 d, const float t,
+                                        const LaneHit &h, float3 &normal, int &object) {
+    if (kind == OBJECT_INSTANCE) {
+        const InstanceRec &rec = S.inst[h.inst];
+        const float3 r0 = xyz(rec.inv[0]), r1 = xyz(rec.inv[1]), r2 = xyz(rec.inv[2]);
+        const float4 e1 = S.tris[3 * (size_t)h.tri + 1], e2 = S.tris[3 * (size_t)h.tri + 2];
+        const float3 nl = cross3(xyz(e1), xyz(e2));
+        float3 geometric = normalize3(add3(add3(mul3(r0, nl.x), mul3(r1, nl.y)), mul3(r2, nl.z)));
+        if (dot3(geometric, d) > 0.0f) geometric = neg3(geometric);
+        normal = geometric;
+        if (S.attrs != nullptr) {
+            const float4 *ap = S.attrs + 4 * (size_t)h.tri;
+            const float4 a0 = ap[0], a1 = ap[1], a2 = ap[2], a3 = ap[3];
+            if (__float_as_int(a3.w) & 1) {
+                const float w0 = 1.0f - h.u - h.v;
+                const float3 n = add3(add3(mul3(xyz(a0), w0), mul3(xyz(a1), h.u)), mul3(xyz(a2), h.v));
+                if (dot3(n, n) > 1e-12f) {
+                    const float3 smooth = normalize3(add3(add3(mul3(r0, n.x), mul3(r1, n.y)), mul3(r2, n.z)));
+                    normal = dot3(smooth, geometric) < 0.0f ? neg3(smooth) : smooth;
+                }
+            }
+        }
+        object = OBJECT_INSTANCE + rec.tmpl;
+    } else if (kind == OBJECT_FLOOR) {
+        normal = mk3(0.0f, 1.0f, 0.0f);
+        object = OBJECT_FLOOR;
+    } else {
+        const float4 sp = S.spheres[kind - OBJECT_SPHERE];
+        normal = normalize3(sub3(add3(o, mul3(d, t)), xyz(sp)));
+        object = kind;
+    }
+}
+
+struct Cam {
+    float3 pos, right, up, fwd;
+};
+
+// camera_from_matrix (render_core.h:264-273): the matrix's columns, renormalized.
+__device__ __forceinline__ Cam load_camera(const float *m12) {
+    const float4 *p = reinterpret_cast<const float4 *>(m12);
+    const float4 a = sload4(p), b = sload4(p + 1), c = sload4(p + 2);
+    Cam cam;
+    cam.pos = mk3(a.w, b.w, c.w);
+    cam.right = normalize3(mk3(a.x, b.x, c.x));
+    cam.up = normalize3(mk3(a.y, b.y, c.y));
+    cam.fwd = normalize3(mk3(-a.z, -b.z, -c.z));
+    return cam;
+}
+
+// camera_ray (device_scene.h:114-119) through a pixel's centre.
+__device__ __forceinline__ float3 pixel_ray(const Cam &c, const float fov_scale, const float aspect, const int x, const int y,
+                                            const int width, const int height) {
+    const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+    const float sx = (2.0f * px / (float)width - 1.0f) * fov_scale * aspect;
+    const float sy = -(2.0f * py / (float)height - 1.0f) * fov_scale;
+    return normalize3(add3(add3(mul3(c.right, sx), mul3(c.up, sy)), c.fwd));
+}
+
+// The tile of a wave: tiles are numbered view-major, and with xcd_remap consecutive workgroups, which
+// the dispatcher deals round-robin over the eight XCDs, are renumbered so each XCD works through one
+// contiguous range of tiles (the same views, so the same worlds' trees in its L2).
+__device__ __forceinline__ long long wave_tile(const int xcd_remap, const long long waves_total) {
+    const long long blocks = (long long)gridDim.x;
+    long long b = blockIdx.x;
+    if (xcd_remap && blocks >= 8) {
+        const long long per = blocks / 8, rem = blocks % 8, xcd = b % 8, k = b / 8;
+        // XCD x receives blocks x, x+8, ...: give it the x-th contiguous range
+        b = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+    }
+    (void)waves_total;
+    return b * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
+}
+
+__global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
+    const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
+    if (tile >= A.total_tiles) return;
+    const int lane = lane_id();
+    const int view = (int)(tile / A.tiles_per_view);
+    const int t = (int)(tile % A.tiles_per_view);
+    const int tx = t % A.tiles_x, ty = t / A.tiles_x;
+    const int x = tx * A.tile_w + lane % A.tile_w;
+    const int y = ty * A.tile_h + lane / A.tile_w;
+    const bool valid = x < A.width && y < A.height;
+    const int world = view / A.cameras_per_world;
+    const Cam cam = load_camera(A.cams + 12 * (size_t)view);
+    const float3 d = pixel_ray(cam, A.fov_scale, A.aspect, x, y, A.width, A.height);
+    float best;
+    LaneHit h;
+    const int kind = scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h);
+    if (!valid) return;
+    const size_t idx = ((size_t)view * A.height + y) * A.width + x;
+    const bool found = kind != OBJECT_NONE;
+    float3 normal = mk3(0.0f, 0.0f, 0.0f);
+    int object = OBJECT_NONE;
+    if (found) surface(A.s, kind, cam.pos, d, best, h, normal, object);
+    if (A.depth) A.depth[idx] = found ? best * dot3(d, cam.fwd) : __int_as_float(0x7f800000);
+    if (A.normal) {
+        A.normal[3 * idx] = normal.x;
+        A.normal[3 * idx + 1] = normal.y;
+        A.normal[3 * idx + 2] = normal.z;
+    }
+    if (A.object_id) A.object_id[idx] = object;
+}
+
+// Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
+// worlds walks each world's rays as its own packet.
+__global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
+    const long long base = ((long long)blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6)) * 64;
+    if (base >= A.count) return;
+    const int lane = lane_id();
+    const long long i = base + lane;
+    const bool valid = i < A.count;
+    const int world = valid ? (int)(i / A.rays_per_world) : -1;
+    float3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
+    bool ok = false;
+    if (valid) {
+        o = mk3(A.origins[3 * i], A.origins[3 * i + 1], A.origins[3 * i + 2]);
+        d = mk3(A.dirs[3 * i], A.dirs[3 * i + 1], A.dirs[3 * i + 2]);
+        const float len2 = dot3(d, d);
+        ok = len2 > 1e-30f && len2 < 1e30f;
+        if (ok) d = mul3(d, 1.0f / sqrtf(len2));
+    }
+    float best = 1e30f;
+    LaneHit h;
+    h.inst = -1;
+    h.tri = -1;
+    int kind = OBJECT_NONE;
+    uint64_t pending = ballot(ok);
+    while (pending != 0) {
+        const int w = __builtin_amdgcn_readlane(world, first_lane(pending));
+        const uint64_t mine = ballot(ok && world == w);
+        float b;
+        LaneHit hh;
+        const int k = scene_hit(A.s, w, o, d, mine, b, hh);
+        if ((mine >> lane) & 1ull) {
+            best = b;
+            h = hh;
+            kind = k;
+        }
+        pending &= ~mine;
+    }
+    if (!valid) return;
+    const bool found = ok && kind != OBJECT_NONE && best <= A.max_distance;
+    float3 normal = mk3(0.0f, 0.0f, 0.0f);
+    int object = OBJECT_NONE;
+    if (found) surface(A.s, kind, o, d, best, h, normal, object);
+    A.distance[i] = found ? best : __int_as_float(0x7f800000);
+    A.normal[3 * i] = normal.x;
+    A.normal[3 * i + 1] = normal.y;
+    A.normal[3 * i + 2] = normal.z;
+    A.object_id[i] = object;
+}
+
+// ------------------------------------------------------------------------------ backward --
+//
+// d(depth)/d(pose) through the fixed hit triangle: a pose change that moves the material point the ray
+// hit by dX moves the hit distance by n.dX / n.d, n being the triangle's world normal (any scale). With
+// X = Rb y + pb (y the point in its body's frame), dL/dpb = a and dL/dq_unit = a^T (dR/dq_unit) y, where
+// a = dL/dt * n / n.d; the projection onto the raw quaternion is applied once per body afterwards.
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// The 7 gradient terms of one lane's hit: returns false when the hit does not depend on a body pose.
+__device__ __forceinline__ bool hit_pose_terms(const TraceScene &S, const int world, const int bodies_per_world,
+                                               const int *body_of_instance, const float *local_rot_scale, const float *body_poses,
+                                               const float3 o, const float3 d, const LaneHit &h, const float dl_dt,
+                                               int &body, float g[7]) {
+    const InstanceRec &rec = S.inst[h.inst];
+    body = body_of_instance[rec.tmpl];
+    if (body < 0) return false;
+    const float3 r0 = xyz(rec.inv[0]), r1 = xyz(rec.inv[1]), r2 = xyz(rec.inv[2]);
+    const float4 v0 = S.tris[3 * (size_t)h.tri], e1 = S.tris[3 * (size_t)h.tri + 1], e2 = S.tris[3 * (size_t)h.tri + 2];
+    const float3 nl = cross3(xyz(e1), xyz(e2));
+    const float3 n = add3(add3(mul3(r0, nl.x), mul3(r1, nl.y)), mul3(r2, nl.z));
+    const float nd = dot3(n, d);
+    if (nd == 0.0f) return false;
+    const float s = dl_dt / nd;
+    const float3 a = mul3(n, s);
+    // the hit's point in mesh space, then in the body's frame: y = Rl diag(s) x + pl
+    const float3 x = add3(add3(xyz(v0), mul3(xyz(e1), h.u)), mul3(xyz(e2), h.v));
+    const float *L = local_rot_scale + 12 * (size_t)rec.tmpl;
+    const float3 y = mk3(L[0] * x.x + L[1] * x.y + L[2] * x.z + L[3], L[4] * x.x + L[5] * x.y + L[6] * x.z + L[7],
+                         L[8] * x.x + L[9] * x.y + L[10] * x.z + L[11]);
+    const float *bp = body_poses + 7 * ((size_t)world * bodies_per_world + body);
+    const float q0 = bp[3], q1 = bp[4], q2 = bp[5], q3 = bp[6];
+    const float qn = sqrtf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+    const float qx = q0 / qn, qy = q1 / qn, qz = q2 / qn, qw = q3 / qn;
+    g[0] = a.x;
+    g[1] = a.y;
+    g[2] = a.z;
+    // a^T (dR/dq_k) y for the unit quaternion's components, R as in instances.cpp:130-132
+    const float ax = a.x, ay = a.y, az = a.z, yx = y.x, yy = y.y, yz = y.z;
+    g[3] = ax * (2 * qy * yy + 2 * qz * yz) + ay * (2 * qy * yx - 4 * qx * yy - 2 * qw * yz) +
+           az * (2 * qz * yx + 2 * qw * yy - 4 * qx * yz);
+    g[4] = ax * (-4 * qy * yx + 2 * qx * yy + 2 * qw * yz) + ay * (2 * qx * yx + 2 * qz * yz) +
+           az * (-2 * qw * yx + 2 * qz * yy - 4 * qy * yz);
+    g[5] = ax * (-4 * qz * yx - 2 * qw * yy + 2 * qx * yz) + ay * (2 * qw * yx - 4 * qz * yy + 2 * qy * yz) +
+           az * (2 * qx * yx + 2 * qy * yy);
+    g[6] = ax * (-2 * qz * yy + 2 * qy * yz) + ay * (2 * qz * yx - 2 * qx * yz) + az * (-2 * qy * yx + 2 * qx * yy);
+    return true;
+}
+
+// Adds each lane's 7 terms into acc[body][7] (this wave's own slice of LDS), one body at a time, with a
+// fixed-order wave sum, so the result does not depend on scheduling.
+__device__ __forceinline__ void wave_accumulate(const bool has, const int body, const float g[7], float *acc) {
+    const int lane = lane_id();
+    uint64_t pending = ballot(has);
+    while (pending != 0) {
+        const int b0 = __builtin_amdgcn_readlane(body, first_lane(pending));
+        const bool mine = has && body == b0;
+        const uint64_t m = ballot(mine);
+        for (int k = 0; k < 7; ++k) {
+            const float s = wave_sum(mine ? g[k] : 0.0f);
+            if (lane == 0) acc[7 * b0 + k] += s;
+        }
+        pending &= ~m;
+    }
+}
+
+constexpr int BWD_WAVES = 4;
+constexpr int MAX_BODIES = 256;
+
+__global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const DepthBackwardArgs A) {
+    extern __shared__ float acc_all[];  // [BWD_WAVES][bodies][7]
+    const int view = blockIdx.x;
+    const int wave = uniform(threadIdx.x >> 6), lane = lane_id();
+    const int B = A.bodies_per_world;
+    float *acc = acc_all + (size_t)wave * B * 7;
+    for (int i = lane; i < B * 7; i += 64) acc[i] = 0.0f;
+    const int world = view / A.cameras_per_world;
+    const Cam cam = load_camera(A.cams + 12 * (size_t)view);
+    const int tiles_x = (A.width + 7) / 8, tiles_y = (A.height + 7) / 8;
+    for (int t = wave; t < tiles_x * tiles_y; t += BWD_WAVES) {
+        const int x = (t % tiles_x) * 8 + (lane & 7), y = (t / tiles_x) * 8 + (lane >> 3);
+        const bool valid = x < A.width && y < A.height;
+        const size_t idx = ((size_t)view * A.height + (valid ? y : 0)) * A.width + (valid ? x : 0);
+        const float3 d = pixel_ray(cam, A.fov_scale, A.aspect, x, y, A.width, A.height);
+        const float depth = valid ? A.depth[idx] : 0.0f;
+        const float gd = valid ? A.grad_depth[idx] : 0.0f;
+        const float cosf_ = dot3(d, cam.fwd);
+        // re-trace only rays whose forward hit something, bounded just beyond the forward's distance
+        const bool want = valid && gd != 0.0f && depth < __int_as_float(0x7f800000);
+        const float bound = want ? (depth / cosf_) * (1.0f + 1e-5f) + 1e-6f : 0.0f;
+        float best;
+        LaneHit h;
+        const int kind = scene_hit(A.s, world, cam.pos, d, ballot(want), best, h, bound);
+        int body = -1;
+        float g[7] = {0, 0, 0, 0, 0, 0, 0};
+        bool has = false;
+        if (want && kind == OBJECT_INSTANCE)
+            has = hit_pose_terms(A.s, world, B, A.body_of_instance, A.local_rot_scale, A.body_poses, cam.pos, d, h, gd * cosf_, body, g);
+        wave_accumulate(has, body, g, acc);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < B * 7; i += blockDim.x) {
+        float s = 0.0f;
+        for (int wv = 0; wv < BWD_WAVES; ++wv) s += acc_all[(size_t)wv * B * 7 + i];
+        A.partial[(size_t)view * B * 7 + i] = s;
+    }
+}
+
+__global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const RangeBackwardArgs A) {
+    extern __shared__ float acc_all[];
+    const int world = blockIdx.x / A.chunks, chunk = blockIdx.x % A.chunks;
+    const int wave = uniform(threadIdx.x >> 6), lane = lane_id();
+    const int B = A.bodies_per_world;
+    float *acc = acc_all + (size_t)wave * B * 7;
+    for (int i = lane; i < B * 7; i += 64) acc[i] = 0.0f;
+    const int R = A.rays_per_world;
+    const int per = (R + A.chunks - 1) / A.chunks;
+    const int r0 = chunk * per, r1 = min(R, r0 + per);
+    for (int r = r0 + wave * 64; r < r1; r += 64 * BWD_WAVES) {
+        const int ri = r + lane;
+        const bool valid = ri < r1;
+        const size_t i = (size_t)world * R + (valid ? ri : r0);
+        float3 o = mk3(A.origins[3 * i], A.origins[3 * i + 1], A.origins[3 * i + 2]);
+        float3 d = mk3(A.dirs[3 * i], A.dirs[3 * i + 1], A.dirs[3 * i + 2]);
+        const float len2 = dot3(d, d);
+        const bool ok = valid && len2 > 1e-30f && len2 < 1e30f;
+        if (ok) d = mul3(d, 1.0f / sqrtf(len2));
+        const float dist = valid ? A.distance[i] : 0.0f;
+        const float gd = valid ? A.grad_distance[i] : 0.0f;
+        const bool want = ok && gd != 0.0f && dist < __int_as_float(0x7f800000);
+        const float bound = want ? dist * (1.0f + 1e-5f) + 1e-6f : 0.0f;
+        float best;
+        LaneHit h;
+        const int kind = scene_hit(A.s, world, o, d, ballot(want), best, h, bound);
+        int body = -1;
+        float g[7] = {0, 0, 0, 0, 0, 0, 0};
+        bool has = false;
+        if (want && kind == OBJECT_INSTANCE)
+            has = hit_pose_terms(A.s, world, B, A.body_of_instance, A.local_rot_scale, A.body_poses, o, d, h, gd, body, g);
+        wave_accumulate(has, body, g, acc);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < B * 7; i += blockDim.x) {
+        float s = 0.0f;
+        for (int wv = 0; wv < BWD_WAVES; ++wv) s += acc_all[(size_t)wv * B * 7 + i];
+        A.partial[((size_t)world * A.chunks + chunk) * B * 7 + i] = s;
+    }
+}
+
+// Sums the parts of each world in order, then projects the unit-quaternion gradient onto the raw
+// quaternion: d unit / d q = (I - u u^T) / |q|.
+__global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.worlds * A.bodies_per_world) return;
+    const int w = i / A.bodies_per_world, b = i % A.bodies_per_world;
+    float g[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int p = 0; p < A.parts; ++p) {
+        const float *src = A.partial + (((size_t)w * A.parts + p) * A.bodies_per_world + b) * 7;
+        for (int k = 0; k < 7; ++k) g[k] += src[k];
+    }
+    const float *bp = A.body_poses + 7 * (size_t)i;
+    const float q[4] = {bp[3], bp[4], bp[5], bp[6]};
+    const float qn = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const float u[4] = {q[0] / qn, q[1] / qn, q[2] / qn, q[3] / qn};
+    const float ug = u[0] * g[3] + u[1] * g[4] + u[2] * g[5] + u[3] * g[6];
+    float *out = A.grad + 7 * (size_t)i;
+    out[0] = g[0];
+    out[1] = g[1];
+    out[2] = g[2];
+    for (int k = 0; k < 4; ++k) out[3 + k] = (g[3 + k] - u[k] * ug) / qn;
+}
+
+}  // namespace
+
+hipError_t launch_render(const RenderArgs &a, hipStream_t stream) {
+    if (a.total_tiles <= 0) return hipSuccess;
+    const long long blocks = (a.total_tiles + 3) / 4;
+    render_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
+    if (a.count <= 0) return hipSuccess;
+    const long long waves = (a.count + 63) / 64;
+    const long long blocks = (waves + 3) / 4;
+    ray_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream) {
+    if (a.views <= 0) return hipSuccess;
+    if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
+    depth_backward_kernel<<<a.views, 64 * BWD_WAVES, lds, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream) {
+    if (a.worlds <= 0 || a.chunks <= 0) return hipSuccess;
+    if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
+    range_backward_kernel<<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pose_grad_finish(const PoseGradFinishArgs &a, hipStream_t stream) {
+    const int n = a.worlds * a.bodies_per_world;
+    if (n <= 0) return hipSuccess;
+    pose_grad_finish_kernel<<<(n + 255) / 256, 256, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace ngx

@@ -1,0 +1,6 @@
+"""NumPy views of the 32-byte node layouts (the reference's src/bvh.h:30-49)."""
+import numpy as np
+
+TLAS_NODE = np.dtype([("lo", "<f4", (3,)), ("left", "<i4"), ("hi", "<f4", (3,)), ("right", "<i4")])
+BLAS_NODE = np.dtype([("lo", "<f4", (3,)), ("a", "<i4"), ("hi", "<f4", (3,)), ("count", "<i4")])
+assert TLAS_NODE.itemsize == 32 and BLAS_NODE.itemsize == 32

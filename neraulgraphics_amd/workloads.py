@@ -1,0 +1,147 @@
+"""Seeded synthetic workloads: the benchmark configurations of this backend, at full size or scaled down.
+
+  0  one world, 64x64 depth+ID, 16 icospheres (320 triangles) at U[-5,5]^3 + (0,0,-10), random unit
+     quaternion, uniform scale U[0.5,1.5]; camera at the origin looking down -z; fov 1.0.
+  1  4096 worlds, 64x64 depth+ID; 32 instances per world over three shared meshes (icosphere 320, box 12,
+     displaced grid 2000), placed as in 0 per world; one camera per world at U[-2,2]^3 looking at the
+     world's instance centroid; new instance matrices every step.
+  2  4096 worlds, 128x128 depth+ID; a 256-instance template over the three meshes, U{8..256} instances per
+     world; instances carried by 64 bodies (body i mod 64, local offset U[-0.5,0.5]^3 and random rotation,
+     per-axis scale U[0.5,1.5]); body poses at U[-5,5]^3 + (0,0,-10) with random rotations; new body poses
+     every step.
+  3  LiDAR: 1024 worlds of 65,536 rays (64 beams over elevations [-15, 15] degrees x 1024 azimuths over
+     [0, 360)) from an origin U[-1,1]^3; 100,000 instances over all worlds (672 worlds of 98, 352 of 97) of
+     one 138,000-triangle displaced grid at U[-10,10]^3; max distance 100, miss = inf.
+  4  differentiable depth: 1024 worlds, 64x64 depth, 32 instances as in 1, each on its own body; the target
+     is rendered from poses perturbed by N(0, 0.01) in translation and 1 degree about a random axis.
+
+Seed = 1000 + config. Quaternions are Gaussian four-vectors normalized. Everything is float32."""
+import numpy as np
+
+from .geometry import Instance, Transform, look_at, make_box, make_grid, make_icosphere, make_transforms
+
+FOV = 1.0
+
+
+def _quat(rng, n):
+    q = rng.normal(size=(n, 4))
+    return (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+
+
+def three_meshes(seed=7):
+    return [make_icosphere(1.0, 2), make_box((0.5, 0.5, 0.5)), make_grid(40, 25, 2.0, 1.25, 0.05, seed=seed)]
+
+
+def _placed_worlds(rng, worlds, per_world, offset=(0.0, 0.0, -10.0), spread=5.0):
+    """Per world: positions, quaternions, uniform scales and the float32 matrices they make."""
+    pos = (rng.uniform(-spread, spread, size=(worlds, per_world, 3)) + np.asarray(offset)).astype(np.float32)
+    quat = _quat(rng, worlds * per_world).reshape(worlds, per_world, 4)
+    scale = rng.uniform(0.5, 1.5, size=(worlds, per_world)).astype(np.float32)
+    return pos, quat, scale, make_transforms(pos, quat, scale)
+
+
+def _cameras_at(rng, targets):
+    cams = np.empty((len(targets), 12), np.float32)
+    for v, t in enumerate(targets):
+        eye = rng.uniform(-2.0, 2.0, size=3)
+        cams[v] = look_at(eye, t).reshape(12)
+    return cams
+
+
+def config(index, worlds=None, rays_per_world=None, seed=None):
+    """The workload as a dict (see the module docstring); `worlds` and `rays_per_world` scale it down."""
+    rng = np.random.default_rng(1000 + index if seed is None else seed)
+    if index == 0:
+        mesh = make_icosphere(1.0, 2)
+        pos, quat, scale, mats = _placed_worlds(rng, 1, 16)
+        cams = np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]], np.float32)
+        return dict(width=64, height=64, worlds=1, cameras_per_world=1, meshes=[mesh],
+                    instances=[Instance(0, Transform(mats[0, i].reshape(3, 4))) for i in range(16)], counts=None,
+                    mode="matrices", matrices=mats, cameras=cams, fov=FOV)
+    if index in (1, 4):
+        W = worlds or (4096 if index == 1 else 1024)
+        P = 32
+        meshes = three_meshes()
+        pos, quat, scale, mats = _placed_worlds(rng, W, P)
+        cams = _cameras_at(rng, pos.mean(axis=1))
+        insts = [Instance(i % 3) for i in range(P)]
+        out = dict(width=64, height=64, worlds=W, cameras_per_world=1, meshes=meshes, instances=insts, counts=None,
+                   cameras=cams, fov=FOV)
+        if index == 1:
+            out.update(mode="matrices", matrices=mats, positions=pos, quaternions=quat, scales=scale)
+            return out
+        # each instance on its own body: the body pose is the instance's position and rotation
+        poses = np.concatenate([pos, quat], axis=-1).astype(np.float32)
+        local = np.tile(np.array([0, 0, 0, 0, 0, 0, 1], np.float32), (P, 1))
+        # scales are per template instance in a binding: use world 0's
+        sc = np.repeat(scale[0][:, None], 3, axis=1).astype(np.float32)
+        axis = rng.normal(size=(W, P, 3))
+        axis /= np.linalg.norm(axis, axis=-1, keepdims=True)
+        half = np.deg2rad(1.0) / 2
+        dq = np.concatenate([axis * np.sin(half), np.full((W, P, 1), np.cos(half))], axis=-1)
+        target = poses.copy()
+        target[..., :3] += rng.normal(0.0, 0.01, size=(W, P, 3))
+        target[..., 3:] = _qmul(dq, poses[..., 3:].astype(np.float64))
+        out.update(mode="bodies", bodies=P, body_of_instance=np.arange(P, dtype=np.int32), local_pose=local, scale=sc,
+                   body_poses=poses, target_poses=target.astype(np.float32))
+        return out
+    if index == 2:
+        W = worlds or 4096
+        P, B = 256, 64
+        meshes = three_meshes()
+        counts = rng.integers(8, 257, size=W).astype(np.int32)
+        boi = (np.arange(P) % B).astype(np.int32)
+        local = np.concatenate([rng.uniform(-0.5, 0.5, size=(P, 3)), _quat(rng, P)], axis=1).astype(np.float32)
+        sc = rng.uniform(0.5, 1.5, size=(P, 3)).astype(np.float32)
+        bpos = (rng.uniform(-5, 5, size=(W, B, 3)) + np.array([0, 0, -10.0])).astype(np.float32)
+        poses = np.concatenate([bpos, _quat(rng, W * B).reshape(W, B, 4)], axis=-1).astype(np.float32)
+        cams = _cameras_at(rng, bpos.mean(axis=1))
+        insts = [Instance(i % 3) for i in range(P)]
+        return dict(width=128, height=128, worlds=W, cameras_per_world=1, meshes=meshes, instances=insts, counts=counts,
+                    mode="bodies", bodies=B, body_of_instance=boi, local_pose=local, scale=sc, body_poses=poses,
+                    cameras=cams, fov=FOV)
+    if index == 3:
+        W = worlds or 1024
+        R = rays_per_world or 65536
+        mesh = make_grid(460, 150, 4.0, 4.0 * 150 / 460, 0.05, seed=11)
+        full = 100000
+        if W == 1024:
+            counts = np.array([98] * 672 + [97] * 352, np.int32)
+        else:
+            counts = np.full(W, min(98, max(1, full // 1024)), np.int32)
+        P = int(counts.max())
+        pos, quat, scale, mats = _placed_worlds(rng, W, P, offset=(0.0, 0.0, 0.0), spread=10.0)
+        origin = rng.uniform(-1, 1, size=(W, 3)).astype(np.float32)
+        beams = 64 if R >= 64 else R
+        az_n = R // beams
+        el = np.deg2rad(np.linspace(-15.0, 15.0, beams))
+        az = 2 * np.pi * np.arange(az_n) / az_n
+        e, a = np.meshgrid(el, az, indexing="ij")
+        d = np.stack([np.cos(e) * np.sin(a), np.sin(e), -np.cos(e) * np.cos(a)], axis=-1).reshape(-1, 3).astype(np.float32)
+        dirs = np.broadcast_to(d, (W, beams * az_n, 3)).copy()
+        origins = np.broadcast_to(origin[:, None, :], (W, beams * az_n, 3)).copy()
+        insts = [Instance(0) for _ in range(P)]
+        return dict(width=64, height=64, worlds=W, cameras_per_world=1, meshes=[mesh], instances=insts, counts=counts,
+                    mode="matrices", matrices=mats, origins=origins, directions=dirs, max_distance=100.0)
+    raise ValueError(f"no configuration {index}")
+
+
+def _qmul(a, b):
+    """Hamilton product of quaternions stored x, y, z, w."""
+    ax, ay, az, aw = np.moveaxis(a, -1, 0)
+    bx, by, bz, bw = np.moveaxis(b, -1, 0)
+    return np.stack([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz], axis=-1)
+
+
+def make_renderer(cfg, **kw):
+    """A BatchRenderer for a workload dict, its poses applied (host arrays)."""
+    from .batch import BatchRenderer
+    r = BatchRenderer(cfg["width"], cfg["height"], cfg["worlds"], cfg["cameras_per_world"], meshes=cfg["meshes"],
+                      instances=cfg["instances"], instances_in_world=cfg["counts"], **kw)
+    if cfg["mode"] == "bodies":
+        r.bind_bodies(cfg["body_of_instance"], cfg["local_pose"], cfg["scale"], cfg["bodies"])
+        r.set_body_poses(cfg["body_poses"])
+    else:
+        r.set_instance_transforms(cfg["matrices"])
+    return r
