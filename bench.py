@@ -1,0 +1,278 @@
+"""Throughput of the batched sensor-rendering path on one MI355X.
+
+A step is one pass of the hot path over one batch: new body poses for every world (one kernel composes them
+in FP64 and rebuilds every world's top-level BVH), then depth + object-ID images of every view (one trace
+kernel). The workload is 4096 worlds at 128x128 with U{8..256} instances per world out of a 256-instance
+template carried by 64 bodies (neraulgraphics_amd.workloads, configuration 2); four resident pose sets are
+cycled so every step has new poses. Inputs are resident in HBM before timing; the step is replayed from a
+HIP graph.
+
+Also reported, on the same JSON line: the forward + pose-backward pass of configuration 4 (1024 worlds,
+64x64 depth, 32 bodies) and its ratio to the forward alone; the render kernel against the HBM roofline;
+and the reference CPU algorithm (this repository's C restatement of the reference CPU backend: FP64 body
+poses, per-world SAH top level with refit, the same traversal, rows of views in chunks of 4 over threads)
+timed on this host on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph]"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "images/sec @4096 worlds 128x128 depth+ID; fwd+bwd images/sec; % HBM BW"
+HBM_PEAK_GBS = 8000.0
+
+
+def render_bytes(views, width, height, instances, outputs_bytes_per_pixel=8):
+    """Algorithmic bytes of one render launch: the outputs (4 B depth + 4 B ID per pixel), the cameras
+    (48 B per view), and each world's top-level tree and instance records read at least once
+    (64 B wide node + 64 B record per instance)."""
+    return outputs_bytes_per_pixel * views * width * height + 48 * views + 128 * instances
+
+
+def build_bytes(worlds, bodies, instances):
+    """Algorithmic bytes of one top-level build launch: body poses in (28 B per body), per instance the
+    placed matrix, the 64 B instance record, the reference-layout leaf and interior node (2 x 32 B) and the
+    64 B wide node out."""
+    return 28 * worlds * bodies + instances * (48 + 64 + 64 + 64)
+
+
+def timed_steps(step, steps, warmup, torch):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_config2(args, torch):
+    from neraulgraphics_amd import workloads
+    cfg = workloads.config(2)
+    r = workloads.make_renderer(cfg)
+    if args.tile_w:
+        r.set_trace_config(args.tile_w, not args.no_xcd)
+    rng = np.random.default_rng(7)
+    pose_sets = [torch.from_numpy(cfg["body_poses"]).cuda()]
+    for _ in range(3):
+        p = cfg["body_poses"].copy()
+        p[..., :3] += rng.normal(0, 0.05, p[..., :3].shape).astype(np.float32)
+        pose_sets.append(torch.from_numpy(p).cuda())
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    fov = cfg["fov"]
+    k = [0]
+
+    def step_direct():
+        r.set_body_poses(pose_sets[k[0] % 4])
+        r.render(cams, fov, depth=True, object_ids=True, zero_copy=True)
+        k[0] += 1
+
+    # per-kernel times with events on the stream the kernels are launched on
+    for _ in range(3):
+        step_direct()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    for i in range(args.steps):
+        ev[i][0].record()
+        r.set_body_poses(pose_sets[i % 4])
+        ev[i][1].record()
+        r.render(cams, fov, depth=True, object_ids=True, zero_copy=True)
+        ev[i][2].record()
+    torch.cuda.synchronize()
+    build_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    render_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+
+    graphs = None
+    if not args.no_graph:
+        try:
+            graphs = []
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                for _ in range(2):
+                    step_direct()
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            for i in range(4):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    r.set_body_poses(pose_sets[i])
+                    r.render(cams, fov, depth=True, object_ids=True, zero_copy=True)
+                graphs.append(g)
+            torch.cuda.synchronize()
+        except Exception as e:  # graph capture is an optimization; direct launches are the same work
+            print(f"# hip graph capture unavailable ({e}); timing direct launches", file=sys.stderr, flush=True)
+            graphs = None
+
+    if graphs:
+        def step():
+            graphs[k[0] % 4].replay()
+            k[0] += 1
+    else:
+        step = step_direct
+    sec = timed_steps(step, args.steps, args.warmup, torch)
+    views = cfg["worlds"] * cfg["cameras_per_world"]
+    inst = int(np.sum(cfg["counts"]))
+    # correctness guard on the timed buffers: the last step's images are the ones of its poses
+    out = r.render(cams, fov, depth=True, object_ids=True, zero_copy=True)
+    torch.cuda.synchronize()
+    hit_frac = float(torch.isfinite(out["depth"]).float().mean())
+    return dict(cfg=cfg, sec=sec, build_ms=build_ms, render_ms=render_ms, views=views, instances=inst, graph=bool(graphs),
+                hit_frac=hit_frac, renderer=r)
+
+
+def bench_config4(args, torch):
+    from neraulgraphics_amd import workloads
+    from neraulgraphics_amd.autograd import render_depth
+    cfg = workloads.config(4)
+    r = workloads.make_renderer(cfg)
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    r.set_body_poses(cfg["target_poses"])
+    target = r.render(cams, cfg["fov"], depth=True, zero_copy=True)["depth"].clone()
+    poses = torch.from_numpy(cfg["body_poses"]).cuda()
+
+    def fwd():
+        with torch.no_grad():
+            render_depth(r, poses, cams, cfg["fov"])
+
+    def fwd_bwd():
+        p = poses.detach().requires_grad_(True)
+        d = render_depth(r, p, cams, cfg["fov"])
+        both = torch.isfinite(d) & torch.isfinite(target)
+        diff = torch.where(both, d - target, torch.zeros_like(d))
+        (0.5 * (diff * diff).sum()).backward()
+        return p.grad
+
+    steps = max(5, args.steps // 2)
+    t_f = timed_steps(fwd, steps, max(2, args.warmup), torch)
+    t_fb = timed_steps(fwd_bwd, steps, max(2, args.warmup), torch)
+    return dict(fwd_images_per_s=cfg["worlds"] / t_f, fwd_bwd_images_per_s=cfg["worlds"] / t_fb, ratio=t_f / t_fb,
+                worlds=cfg["worlds"])
+
+
+def cpu_baseline(seconds=12.0):
+    """The reference CPU algorithm on this host, on a sample of configuration 2's worlds."""
+    from neraulgraphics_amd import workloads
+    from oracle.oracle import Oracle, SahTlas
+    o = Oracle()
+    threads = min(16, os.cpu_count() or 1)
+    sample = 64
+    cfg = workloads.config(2, worlds=sample)
+    counts = cfg["counts"]
+    world_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    meshes = o.build_meshes(cfg["meshes"])
+    tmpl_mesh = np.array([i.mesh for i in cfg["instances"]], np.int32)
+    local_bounds = meshes["bounds"][tmpl_mesh]
+    tmpl_of = np.concatenate([np.arange(c) for c in counts]).astype(np.int32)
+    blas_root = meshes["roots"][tmpl_mesh][tmpl_of]
+    tlas = SahTlas(o, world_off)
+    rng = np.random.default_rng(3)
+    images, t0 = 0, time.perf_counter()
+    while True:
+        poses = cfg["body_poses"].copy()
+        poses[..., :3] += rng.normal(0, 0.05, poses[..., :3].shape).astype(np.float32)
+        mats = o.body_poses_to_matrices(world_off, cfg["bodies"], cfg["body_of_instance"], cfg["local_pose"], cfg["scale"], poses)
+        nodes, roots, inv = tlas.update(mats, local_bounds)
+        o.render(nodes, roots, inv, blas_root, meshes["nodes"], meshes["tris"], meshes["attrs"], world_off, cfg["cameras"],
+                 cfg["fov"], cfg["width"], cfg["height"], threads=threads, outputs=("depth", "id"))
+        images += sample
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+    return dict(value=images / el, unit="images/s", cores=threads, kind="port",
+                sample=f"{images} images of configuration 2 ({sample} worlds a step, 128x128 depth+ID, body poses, "
+                       f"per-world SAH top level with refit) in {el:.1f} s on {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-bwd", action="store_true")
+    ap.add_argument("--tile-w", type=int, default=0)
+    ap.add_argument("--no-xcd", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 512 MiB three times with a plain copy kernel, to scale HBM counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": f"this benchmark runs on one GPU; --gpus {args.gpus} is not supported: the path shards "
+                                   "by world with no exchange, so N GPUs run N independent batches"}), flush=True)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU"}), flush=True)
+        sys.exit(1)
+    if args.copy_calibration:
+        src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# copy calibration: 3 copies of 536870912 bytes", file=sys.stderr, flush=True)
+        del src, dst
+    print("# config 2: 4096 worlds 128x128 depth+ID", file=sys.stderr, flush=True)
+    c2 = bench_config2(args, torch)
+    sec = c2["sec"]
+    rb = render_bytes(c2["views"], 128, 128, c2["instances"])
+    achieved = rb / (c2["render_ms"] * 1e-3) / 1e9
+    print(f"# step {sec * 1e3:.3f} ms (build {c2['build_ms']:.3f} ms, render {c2['render_ms']:.3f} ms)", file=sys.stderr, flush=True)
+    c4 = None
+    if not args.no_bwd:
+        print("# config 4: forward and forward+backward", file=sys.stderr, flush=True)
+        c4 = bench_config4(args, torch)
+    cpu = None
+    if not args.no_cpu:
+        print("# CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_seconds)
+    prof = os.path.join(ROOT, "profiles", "render_kernel_config2_traffic.json")
+    traffic = None
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": c2["views"] / sec,
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "configuration 2: 4096 worlds, 128x128 depth+object-ID, U{8..256} of a 256-instance "
+                               "template per world, 64 body poses per world, new poses every step",
+                   "worlds": 4096, "width": 128, "height": 128, "instances": c2["instances"], "outputs": "depth+ID",
+                   "hip_graph": c2["graph"]},
+        "rays_per_s": c2["views"] * 128 * 128 / sec,
+        "build_ms": c2["build_ms"],
+        "render_ms": c2["render_ms"],
+        "hit_fraction": c2["hit_frac"],
+        "roofline": {"bound": "hbm", "kernel": "render_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": rb},
+        "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
+    }
+    if c4:
+        line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s_config4=c4["fwd_images_per_s"],
+                    fwd_bwd_over_fwd=c4["ratio"])
+    line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -64,5 +64,5 @@ def capsule(pointer, shape, dtype, device_id, owner):
 
 
 def to_torch(pointer, shape, dtype, device_id, owner):
-    import torch
-    return torch.from_dlpack(capsule(pointer, shape, dtype, device_id, owner))
+    from torch.utils.dlpack import from_dlpack
+    return from_dlpack(capsule(pointer, shape, dtype, device_id, owner))

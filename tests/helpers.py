@@ -1,0 +1,55 @@
+"""Comparison rules shared by the parity tests.
+
+Object IDs and hit/miss are compared exactly, except at rays where two surfaces are hit at the same
+distance (to 1e-5 relative): there the reference keeps whichever it met first, which depends on the order
+a walk meets them. Depth, normal and range are compared to 1e-5 relative;
+in practice the backend and the oracle compute the same IEEE singles, and the tests report how many
+values are bit-identical."""
+import numpy as np
+
+REL = 1e-5
+
+
+def compare_images(gpu, ref, what=""):
+    """gpu, ref: dicts with depth (V,H,W), object_id (V,H,W), optionally normal (V,H,W,3)."""
+    d, e = np.asarray(gpu["depth"]), np.asarray(ref["depth"])
+    gi, ri = np.asarray(gpu["object_id"]), np.asarray(ref["object_id"])
+    fin_g, fin_r = np.isfinite(d), np.isfinite(e)
+    both = fin_g & fin_r
+    close = np.zeros_like(both)
+    close[both] = np.abs(d[both] - e[both]) <= REL * np.abs(e[both])
+    id_diff = gi != ri
+    # an ID may differ only where both saw a surface at the same distance
+    bad_ids = id_diff & ~(both & close)
+    bad_hit = fin_g != fin_r
+    bad_depth = both & ~close
+    report = dict(pixels=int(d.size), hits=int(fin_r.sum()), id_ties=int((id_diff & both & close).sum()),
+                  bad_ids=int(bad_ids.sum()), bad_hit=int(bad_hit.sum()), bad_depth=int(bad_depth.sum()),
+                  depth_bitexact=float(np.mean(d.view(np.uint32)[both] == e.view(np.uint32)[both])) if both.any() else 1.0)
+    if "normal" in ref and ref["normal"] is not None and gpu.get("normal") is not None:
+        n, m = np.asarray(gpu["normal"]), np.asarray(ref["normal"])
+        same = ~id_diff
+        err = np.abs(n - m)[same]
+        report["bad_normal"] = int(np.sum(err > REL * np.maximum(np.abs(m[same]), 1.0)))
+    assert report["bad_ids"] == 0 and report["bad_hit"] == 0 and report["bad_depth"] == 0 and report.get("bad_normal", 0) == 0, \
+        f"{what}: {report}"
+    # ties are rare: a handful per image at most
+    assert report["id_ties"] <= max(2, report["pixels"] // 5000), f"{what}: too many tied IDs {report}"
+    return report
+
+
+def compare_rays(gpu, ref, what=""):
+    g = dict(depth=gpu["distance"], object_id=gpu["object_id"], normal=gpu["normal"])
+    r = dict(depth=ref["distance"], object_id=ref["object_id"], normal=ref["normal"])
+    return compare_images(g, r, what)
+
+
+def world_subtree(nodes, root, n_total):
+    """Every node index reachable from root (interior nodes index < n_total - 1; leaves >= n_total - 1)."""
+    out, stack = [], [root]
+    while stack:
+        v = stack.pop()
+        out.append(v)
+        if v < n_total - 1:
+            stack += [int(nodes[v]["left"]), int(nodes[v]["right"])]
+    return np.array(sorted(out), np.int64)

@@ -1,0 +1,248 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same seeded inputs.
+
+Top-level build: Morton keys, the stable sort, and every world's subtree (node indices, children, boxes,
+roots) bit-exact against the restatement of the reference's GPU build; instance inverses and body-pose
+matrices bit-exact. Images and rays: see helpers.compare_images."""
+import numpy as np
+import pytest
+
+from neraulgraphics_amd import OBJECT_INSTANCE, workloads
+from oracle.oracle import forward
+
+from .helpers import compare_images, compare_rays, world_subtree
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _check_build(r, ref, world_off):
+    b = r.read_build(intermediates=True)
+    st = ref["state"]
+    N = int(world_off[-1])
+    assert np.array_equal(b["matrices"].view(np.uint32), ref["matrices"].view(np.uint32)), "placed matrices differ"
+    assert np.array_equal(b["inv"].view(np.uint32), st["inv"].view(np.uint32)), "inverses differ"
+    assert np.array_equal(b["keys"], st["keys"]), "Morton keys differ"
+    assert np.array_equal(b["sorted_keys"], st["sorted_keys"]), "sorted keys differ"
+    assert np.array_equal(b["sorted_ids"], st["sorted_ids"]), "sort permutation differs"
+    assert np.array_equal(b["aabb"], st["aabb"]), "instance boxes differ"
+    nonempty = np.diff(world_off) > 0
+    assert np.array_equal(b["world_bounds"][nonempty], st["world_bounds"][nonempty]), "world centre bounds differ"
+    assert np.array_equal(b["roots"], st["roots"]), "world roots differ"
+    nodes_g, nodes_r = b["nodes"], st["nodes"]
+    for w in np.flatnonzero(nonempty):
+        idx = world_subtree(nodes_r, int(st["roots"][w]), N)
+        g, e = nodes_g[idx], nodes_r[idx]
+        assert np.array_equal(g["left"], e["left"]) and np.array_equal(g["right"], e["right"]), f"world {w} topology differs"
+        assert np.array_equal(g["lo"], e["lo"]) and np.array_equal(g["hi"], e["hi"]), f"world {w} boxes differ"
+
+
+@pytest.mark.parametrize("index,worlds", [(1, 64), (2, 64), (3, 16)])
+def test_top_level_build_matches_the_reference_algorithm(gpu, oracle, index, worlds):
+    _say(f"build parity: config {index}, {worlds} worlds")
+    cfg = workloads.config(index, worlds=worlds, rays_per_world=64)
+    r = workloads.make_renderer(cfg)
+    r.keep_build_intermediates(True)
+    if cfg["mode"] == "bodies":
+        r.set_body_poses(cfg["body_poses"])
+    else:
+        r.set_instance_transforms(cfg["matrices"])
+    ref = forward(cfg, oracle, outputs=())
+    _check_build(r, ref, ref["world_off"])
+
+
+@pytest.mark.parametrize("index,worlds", [(0, None), (1, 32), (2, 16), (4, 16)])
+def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds):
+    _say(f"render parity: config {index}, worlds {worlds}")
+    cfg = workloads.config(index, worlds=worlds)
+    r = workloads.make_renderer(cfg)
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    ref = forward(cfg, oracle)
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, f"config {index}")
+    _say(rep)
+    assert rep["hits"] > 0.05 * rep["pixels"]
+
+
+def test_ray_queries_match_the_oracle(gpu, oracle):
+    _say("ray parity: config 3, 8 worlds x 4096 rays")
+    cfg = workloads.config(3, worlds=8, rays_per_world=4096)
+    r = workloads.make_renderer(cfg)
+    out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], cfg["origins"], cfg["directions"], cfg["max_distance"])
+    rep = compare_rays(out, ref, "config 3")
+    _say(rep)
+    assert rep["hits"] > 0
+
+
+def test_rays_of_uneven_worlds_per_wave(gpu, oracle):
+    _say("ray parity: 37 rays per world, so waves straddle worlds")
+    cfg = workloads.config(3, worlds=8, rays_per_world=37 * 64)
+    o = cfg["origins"][:, :37]
+    d = cfg["directions"][:, ::64][:, :37]
+    r = workloads.make_renderer(cfg)
+    out = r.cast_rays(o, d, 100.0)
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], o, d, 100.0)
+    compare_rays(out, ref, "37 rays per world")
+
+
+def test_worlds_do_not_see_one_another(gpu):
+    """tests/test_batch.cpp:185-214: boxes at z=-8 and z=-3 in two worlds, seen from the origin."""
+    import neraulgraphics_amd as ng
+    W, H = 96, 54
+    b = ng.BatchRenderer(W, H, 2, meshes=[ng.make_box((1, 1, 1))], instances=[ng.Instance(0)])
+    poses = np.stack([ng.make_transform((0, 0, z)).matrix for z in (-8.0, -3.0)]).reshape(2, 1, 3, 4)
+    b.set_instance_transforms(poses)
+    cam = ng.look_at((0, 0, 0), (0, 0, -1)).reshape(12)
+    out = b.render(np.stack([cam, cam]), 1.5708, depth=True, object_ids=True)
+    centre = out["depth"][:, H // 2, W // 2]
+    assert np.allclose(centre, [7.0, 2.0], atol=0.02), centre
+    hits = b.cast_rays(np.zeros((2, 1, 3), np.float32), np.tile(np.array([0, 0, -1], np.float32), (2, 1, 1)), 100.0)
+    assert np.allclose(hits["distance"][:, 0], [7.0, 2.0], atol=1e-3) and (hits["object_id"] == OBJECT_INSTANCE).all()
+
+
+def test_single_instance_worlds_and_a_pile_of_coincident_boxes(gpu):
+    """tests/test_batch.cpp:311-331: one-leaf trees at 3 + w; 1,000 identical boxes whose front face is 4.8 away."""
+    import neraulgraphics_amd as ng
+    W, H = 96, 54
+    cam = ng.look_at((0, 0, 0), (0, 0, -8)).reshape(12)
+    single = ng.BatchRenderer(W, H, 4, meshes=[ng.make_box((1, 1, 1))], instances=[ng.Instance(0)])
+    single.set_instance_transforms(np.stack([ng.make_transform((0, 0, -4.0 - w)).matrix for w in range(4)]).reshape(4, 1, 3, 4))
+    d = single.render(np.tile(cam, (4, 1)), 1.5708, depth=True)["depth"][:, H // 2, W // 2]
+    assert np.allclose(d, 3.0 + np.arange(4), atol=0.02), d
+    pile = ng.BatchRenderer(W, H, 1, meshes=[ng.make_box((0.2, 0.2, 0.2))], instances=[ng.Instance(0)] * 1000)
+    pile.set_instance_transforms(np.tile(ng.make_transform((0, 0, -5)).matrix, (1, 1000, 1, 1)))
+    d = pile.render(cam[None], 1.5708, depth=True)["depth"][0, H // 2, W // 2]
+    assert abs(d - 4.8) < 0.02, d
+
+
+def test_spheres_and_floor(gpu, oracle):
+    """Analytic spheres and the floor before the meshes (render_core.h:78-112); a mesh must be strictly closer."""
+    import neraulgraphics_amd as ng
+    cfg = workloads.config(1, worlds=4)
+    spheres = [ng.Sphere((0.5, -1.0, -9.0), 1.0), ng.Sphere((-2.0, 1.0, -7.0), 0.7)]
+    floor = ng.Plane(y=-2.0, half_extent_x=30.0, z_min=-60.0, z_max=5.0)
+    r = workloads.make_renderer(cfg, spheres=spheres, floor=floor)
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.render(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                        st["meshes"]["attrs"], st["world_off"], cfg["cameras"], cfg["fov"], cfg["width"], cfg["height"],
+                        spheres=np.array([[0.5, -1.0, -9.0, 1.0], [-2.0, 1.0, -7.0, 0.7]]),
+                        floor=np.array([-2.0, 30.0, -60.0, 5.0]))
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "spheres+floor")
+    _say(rep)
+    ids = np.unique(out["object_ids"])
+    assert 33 in ids or 1 in ids or 2 in ids
+
+
+def test_zero_copy_views_are_overwritten_by_the_next_render(gpu):
+    import torch
+    cfg = workloads.config(1, worlds=4)
+    r = workloads.make_renderer(cfg)
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    first = r.render(cams, cfg["fov"], depth=True, zero_copy=True)["depth"]
+    kept = first.clone()
+    moved = cams.clone()
+    moved[:, 3] += 0.5
+    second = r.render(moved, cfg["fov"], depth=True, zero_copy=True)["depth"]
+    torch.cuda.synchronize()
+    assert first.data_ptr() == second.data_ptr() and not torch.equal(first, kept)
+
+
+def test_device_poses_equal_host_poses(gpu):
+    import torch
+    cfg = workloads.config(2, worlds=8)
+    r = workloads.make_renderer(cfg)
+    host = r.render(cfg["cameras"], cfg["fov"], depth=True, object_ids=True)
+    r.set_body_poses(torch.from_numpy(cfg["body_poses"]).cuda())
+    dev = r.render(torch.from_numpy(cfg["cameras"]).cuda(), cfg["fov"], depth=True, object_ids=True, zero_copy=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(dev["depth"].cpu().numpy().view(np.uint32), host["depth"].view(np.uint32))
+    assert np.array_equal(dev["object_ids"].cpu().numpy(), host["object_ids"])
+
+
+def test_bad_input_is_rejected_and_changes_nothing(gpu):
+    cfg = workloads.config(1, worlds=2)
+    r = workloads.make_renderer(cfg)
+    before = r.render(cfg["cameras"], cfg["fov"], depth=True)["depth"]
+    bad = cfg["matrices"].copy()
+    bad[1, 4] = 0.0
+    with pytest.raises(ValueError, match="singular"):
+        r.set_instance_transforms(bad)
+    bad = cfg["matrices"].copy()
+    bad[0, 0, 7] = np.nan
+    with pytest.raises(ValueError, match="NaN"):
+        r.set_instance_transforms(bad)
+    with pytest.raises(ValueError, match="at least one output"):
+        r.render(cfg["cameras"], cfg["fov"])
+    with pytest.raises(ValueError, match="fov"):
+        r.render(cfg["cameras"], 4.0, depth=True)
+    with pytest.raises(ValueError, match="samples_per_pixel"):
+        r.render(cfg["cameras"], cfg["fov"], 0, depth=True)
+    cams = cfg["cameras"].copy()
+    cams[0, 3] = np.nan
+    with pytest.raises(ValueError, match="camera"):
+        r.render(cams, cfg["fov"], depth=True)
+    with pytest.raises(ValueError, match="bind_bodies"):
+        r.set_body_poses(np.zeros((2, 1, 7), np.float32))
+    with pytest.raises(ValueError, match="body"):
+        r.bind_bodies(np.array([0] * 31 + [5], np.int32), np.tile([0, 0, 0, 0, 0, 0, 1], (32, 1)), np.ones((32, 3)), 1)
+    after = r.render(cfg["cameras"], cfg["fov"], depth=True)["depth"]
+    assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
+
+
+def test_pose_gradient_matches_fp64_closed_form_and_differences(gpu, oracle):
+    import torch
+    from neraulgraphics_amd.autograd import render_depth
+    _say("backward: config 4, 8 worlds")
+    cfg = workloads.config(4, worlds=8)
+    r = workloads.make_renderer(cfg)
+    target = torch.from_numpy(render_target(r, cfg)).cuda()
+    poses = torch.from_numpy(cfg["body_poses"]).cuda().requires_grad_(True)
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    depth = render_depth(r, poses, cams, cfg["fov"])
+    both = torch.isfinite(depth) & torch.isfinite(target)
+    diff = torch.where(both, depth - target, torch.zeros_like(depth))
+    loss = 0.5 * (diff * diff).sum()
+    loss.backward()
+    g_gpu = poses.grad.cpu().numpy().astype(np.float64)
+    # FP64 references on the oracle: the closed form and central differences, step 1e-4, per hit pixel
+    weight = diff.detach().cpu().numpy().reshape(-1).astype(np.float64)
+    ref = forward(cfg, oracle)
+    hit_inst = ref["hit_inst"].reshape(-1)
+    hit_tri = ref["hit_tri"].reshape(-1)
+    V, H, W = ref["depth"].shape
+    origins = np.zeros((V * H * W, 3))
+    dirs = np.zeros((V * H * W, 3))
+    cosv = np.zeros(V * H * W)
+    for v in range(V):
+        for y in range(H):
+            for x in range(W):
+                o, d, f = oracle.camera_ray(cfg["cameras"][v], cfg["fov"], W, H, x, y)
+                k = (v * H + y) * W + x
+                origins[k], dirs[k], cosv[k] = o, d, float(np.dot(d.astype(np.float64), f.astype(np.float64)))
+    ray_world = np.repeat(np.arange(V), H * W).astype(np.int32)
+    args = (origins, dirs, weight, cosv, hit_inst, hit_tri, ray_world, ref["world_off"], cfg["bodies"], cfg["body_of_instance"],
+            cfg["local_pose"], cfg["scale"], cfg["body_poses"], ref["meshes"]["tris"])
+    g_cf = oracle.pose_grad(0, 0.0, *args)
+    g_fd = oracle.pose_grad(1, 1e-4, *args)
+    g_fine = oracle.pose_grad(1, 1e-6, *args)
+    pin = np.linalg.norm(g_cf - g_fine) / np.linalg.norm(g_fine)
+    rel = np.linalg.norm(g_gpu - g_fd) / np.linalg.norm(g_fd)
+    rel_cf = np.linalg.norm(g_gpu - g_cf) / np.linalg.norm(g_cf)
+    _say(f"|closed form - FD(1e-6)| / |FD| = {pin:.2e}; |GPU - FD(1e-4)| / |FD| = {rel:.2e}; |GPU - closed form| = {rel_cf:.2e}")
+    assert pin < 1e-7  # the FP64 closed form is the derivative (a wrong sign or term would be O(1))
+    assert rel < 1e-3  # the north-star check: FP32 GPU gradient against FP64 central differences, step 1e-4
+    assert rel_cf < 1e-4
+
+
+def render_target(r, cfg):
+    r.set_body_poses(cfg["target_poses"])
+    d = r.render(cfg["cameras"], cfg["fov"], depth=True)["depth"]
+    r.set_body_poses(cfg["body_poses"])
+    return d
