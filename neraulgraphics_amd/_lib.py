@@ -6,7 +6,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libngx.so")
+# NGX_LIBRARY may name an instrumented build of the same sources (see build.py's extra_flags).
+LIB_PATH = os.environ.get("NGX_LIBRARY", os.path.join(_HERE, "libngx.so"))
 
 NGX_OK, NGX_INVALID_ARGUMENT, NGX_DEVICE_ERROR, NGX_OUT_OF_MEMORY, NGX_BAD_STATE, NGX_UNSUPPORTED = range(6)
 NGX_MEM_HOST, NGX_MEM_DEVICE = 0, 1

@@ -18,28 +18,31 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-I" + os.path.join(ROOT, "include")]
 
 
-def _compile(src):
-    obj = os.path.join(CSRC, "_obj", os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src, extra=(), objdir="_obj"):
+    obj = os.path.join(CSRC, objdir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compiling {src} failed:\n{r.stderr}")
     return obj
 
 
-def build(verbose=False):
-    os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
+def build(verbose=False, extra_flags=(), out=LIB):
+    """Compiles every source and links `out`. extra_flags (e.g. -DNGX_TRACE_STATS for the instrumented
+    traversal) go to every compile and use their own object directory."""
+    objdir = "_obj" if not extra_flags else "_obj_" + "_".join(f.strip("-").replace("=", "_") for f in extra_flags)
+    os.makedirs(os.path.join(CSRC, objdir), exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=len(SOURCES)) as pool:
-        objs = list(pool.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(pool.map(lambda src: _compile(src, extra_flags, objdir), SOURCES))
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"linking libngx.so failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

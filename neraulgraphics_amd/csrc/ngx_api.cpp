@@ -208,6 +208,25 @@ struct ngx_ctx {
 
     void set_device() const { HIP_CHECK(hipSetDevice(device)); }
 
+    ngx::RenderArgs render_args(const float *cams, const float fov) const {
+        ngx::RenderArgs a{};
+        a.s = trace_scene();
+        a.cams = cams;
+        a.fov_scale = std::tan(fov * 0.5f);
+        a.aspect = (float)width / (float)height;
+        a.width = width;
+        a.height = height;
+        a.cameras_per_world = cpw;
+        a.views = views;
+        a.tile_w = tile_w;
+        a.tile_h = 64 / tile_w;
+        a.tiles_x = (width + a.tile_w - 1) / a.tile_w;
+        a.tiles_per_view = a.tiles_x * ((height + a.tile_h - 1) / a.tile_h);
+        a.total_tiles = (long long)a.tiles_per_view * views;
+        a.xcd_remap = xcd_remap;
+        return a;
+    }
+
     bool capturing(hipStream_t s) const {
         hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
@@ -502,9 +521,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
             HIP_CHECK(hipMemcpyAsync(ctx->d_cam_stage, cams12, 12 * sizeof(float) * ctx->views, hipMemcpyHostToDevice, s));
             cams = ctx->d_cam_stage;
         }
-        ngx::RenderArgs a{};
-        a.s = ctx->trace_scene();
-        a.cams = cams;
+        ngx::RenderArgs a = ctx->render_args(cams, fov);
         a.fov_scale = std::tan(fov * 0.5f);
         a.aspect = (float)ctx->width / (float)ctx->height;
         a.width = ctx->width;
@@ -595,7 +612,10 @@ ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const 
         require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
         ctx->set_device();
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        const size_t need = (size_t)ctx->views * ctx->bodies * 7;
+        // enough workgroups to fill the GPU: each view's tiles are split over `parts` workgroups of 4 waves
+        const int tiles = ((ctx->width + 7) / 8) * ((ctx->height + 7) / 8);
+        const int parts = std::max(1, std::min(16, tiles / 8));
+        const size_t need = (size_t)ctx->views * parts * ctx->bodies * 7;
         if (need > ctx->partial_cap) {
             HIP_CHECK(hipStreamSynchronize(s));
             device_alloc(ctx->d_partial, need);
@@ -617,9 +637,10 @@ ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const 
         a.body_of_instance = ctx->d_body_of_instance;
         a.local_rot_scale = ctx->d_local_rot_scale;
         a.body_poses = body_poses;
+        a.parts = parts;
         a.partial = ctx->d_partial;
         HIP_CHECK(ngx::launch_depth_backward(a, s));
-        ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw, ctx->d_partial, body_poses, grad_out};
+        ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw * parts, ctx->d_partial, body_poses, grad_out};
         HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
         ctx->last_stream = s;
     });
@@ -786,5 +807,25 @@ ngx_status ngx_build_blas_host(const float *vertices, int32_t vertex_count, cons
         }
     });
 }
+
+#ifdef NGX_TRACE_STATS
+// Instrumented builds only: the render kernel's traversal counters summed over all waves (device cameras).
+ngx_status ngx_debug_render_stats(ngx_ctx *ctx, const float *cams_device, float fov, long long *sums) {
+    return guarded([&] {
+        require(ctx != nullptr && cams_device && sums, "arguments must not be null");
+        ctx->set_device();
+        const ngx::RenderArgs a = ctx->render_args(cams_device, fov);
+        int *d = nullptr;
+        device_alloc(d, (size_t)a.total_tiles * ngx::TRACE_STAT_COUNT);
+        HIP_CHECK(ngx::launch_render_stats(a, d, nullptr));
+        std::vector<int> h((size_t)a.total_tiles * ngx::TRACE_STAT_COUNT);
+        HIP_CHECK(hipMemcpy(h.data(), d, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+        (void)hipFree(d);
+        for (int k = 0; k < ngx::TRACE_STAT_COUNT; ++k) sums[k] = 0;
+        for (size_t i = 0; i < h.size(); ++i) sums[i % ngx::TRACE_STAT_COUNT] += h[i];
+        sums[ngx::TRACE_STAT_COUNT] = a.total_tiles;
+    });
+}
+#endif
 
 }  // extern "C"

@@ -93,7 +93,8 @@ struct DepthBackwardArgs {
     const int *body_of_instance;     // [P]
     const float *local_rot_scale;    // [P][12]: Rl diag(s) | pl, the instance in its body's frame
     const float *body_poses;   // [worlds][bodies][7], the poses the tree was built from
-    float *partial;            // [views][bodies][7]
+    int parts;                 // workgroups per view, each taking every parts-th group of tiles
+    float *partial;            // [views][parts][bodies][7]
 };
 
 struct RangeBackwardArgs {
@@ -125,6 +126,10 @@ hipError_t launch_rays(const RayArgs &a, hipStream_t stream);
 hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream);
 hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream);
 hipError_t launch_pose_grad_finish(const PoseGradFinishArgs &a, hipStream_t stream);
+#ifdef NGX_TRACE_STATS
+constexpr int TRACE_STAT_COUNT = 11;
+hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream);
+#endif
 
 // The largest world the one-workgroup-per-world build takes.
 constexpr int MAX_WORLD_INSTANCES = 1024;

@@ -24,6 +24,18 @@ struct LaneHit {
     float u, v;
 };
 
+// Traversal counters of one wavefront, wave-uniform. NoStats compiles them away; WaveStats is only used by the
+// instrumented kernel built with NGX_TRACE_STATS.
+enum StatIndex { ST_TLAS_NODES, ST_TLAS_LANES, ST_INSTANCES, ST_INSTANCE_LANES, ST_INSTANCE_ROOT_LANES, ST_BLAS_NODES,
+                 ST_BLAS_LANES, ST_TRIANGLES, ST_TRIANGLE_LANES, ST_POPS, ST_VALID, ST_COUNT };
+struct NoStats {
+    __device__ __forceinline__ void add(int, int) {}
+};
+struct WaveStats {
+    int c[ST_COUNT];
+    __device__ __forceinline__ void add(const int k, const int v) { c[k] += v; }
+};
+
 // A wave-uniform stack of (node reference, lane mask) entries, entry k held in lane k & 63 of register
 // set k >> 6: a push is a per-lane select, a pop a readlane.
 template <int SETS>
@@ -83,9 +95,13 @@ __device__ __forceinline__ Wide load_wide(const WideNode *nodes, const int index
 }
 
 // Tests the triangles first .. first+count-1 for the lanes in mask (mesh_dev.h:186-199).
+template <class Stats>
 __device__ __forceinline__ void leaf_triangles(const TraceScene &S, const int first, const int count, const uint64_t mask,
-                                               const int inst, const float3 lo, const float3 ld, float &tmax, LaneHit &h) {
+                                               const int inst, const float3 lo, const float3 ld, float &tmax, LaneHit &h,
+                                               Stats &stats) {
     const bool in = (mask >> lane_id()) & 1ull;
+    stats.add(ST_TRIANGLES, count);
+    stats.add(ST_TRIANGLE_LANES, count * popc64(mask));
     for (int j = first; j < first + count; ++j) {
         const float4 *tp = S.tris + 3 * (size_t)j;
         const float3 v0 = xyz(sload4(tp)), e1 = xyz(sload4(tp + 1)), e2 = xyz(sload4(tp + 2));
@@ -103,8 +119,11 @@ __device__ __forceinline__ void leaf_triangles(const TraceScene &S, const int fi
 
 // One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
 // renormalized so t is shared), then the mesh's own tree.
+template <class Stats>
 __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t mask, const float3 o, const float3 d,
-                               float &tmax, LaneHit &h) {
+                               float &tmax, LaneHit &h, Stats &stats) {
+    stats.add(ST_INSTANCES, 1);
+    stats.add(ST_INSTANCE_LANES, popc64(mask));
     const float4 *rp = reinterpret_cast<const float4 *>(S.inst + g);
     const float4 r0 = sload4(rp), r1 = sload4(rp + 1), r2 = sload4(rp + 2);
     const int4 meta = sload4i(reinterpret_cast<const int4 *>(rp + 3));
@@ -117,9 +136,10 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
     const bool in = (mask >> lane) & 1ull;
     const uint64_t root_mask = ballot(in && box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, tmax) < NO_HIT);
     if (root_mask == 0) return;
+    stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     const int root_ref = __float_as_int(m0.w);
     if (root_ref < 0) {
-        leaf_triangles(S, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h);
+        leaf_triangles(S, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h, stats);
         return;
     }
     WaveStack<1> st;
@@ -128,6 +148,8 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
     uint64_t cm = root_mask;
     while (true) {
         const Wide w = load_wide(S.blas_wide, cur);
+        stats.add(ST_BLAS_NODES, 1);
+        stats.add(ST_BLAS_LANES, popc64(cm));
         const bool on = (cm >> lane) & 1ull;
         const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], lo, inv_d, tmax) : NO_HIT;
         const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], lo, inv_d, tmax) : NO_HIT;
@@ -154,7 +176,7 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
         }
         // Leaves are tested where they are reached; then the walk resumes from the stack.
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) leaf_triangles(S, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h);
+            if (next_mask != 0) leaf_triangles(S, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
             if (st.sp == 0) return;
             int r;
             st.pop(r, next_mask);
@@ -174,13 +196,14 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
 // distance to beat on entry and its closest distance on return.
+template <class Stats>
 __device__ void trace_world(const TraceScene &S, const int world, const float3 o, const float3 d, const uint64_t live,
-                            float &tmax, LaneHit &h) {
+                            float &tmax, LaneHit &h, Stats &stats) {
     if (live == 0) return;
     const int root = uniform(S.roots[world]);
     if (root < 0) return;
     if (root >= S.total - 1) {
-        visit_instance(S, root - (S.total - 1), live, o, d, tmax, h);
+        visit_instance(S, root - (S.total - 1), live, o, d, tmax, h, stats);
         return;
     }
     const float3 inv_d = safe_inverse(d);
@@ -191,6 +214,8 @@ __device__ void trace_world(const TraceScene &S, const int world, const float3 o
     uint64_t cm = live;
     while (true) {
         const Wide w = load_wide(S.tlas_wide, cur);
+        stats.add(ST_TLAS_NODES, 1);
+        stats.add(ST_TLAS_LANES, popc64(cm));
         const bool on = (cm >> lane) & 1ull;
         const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], o, inv_d, tmax) : NO_HIT;
         const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], o, inv_d, tmax) : NO_HIT;
@@ -215,7 +240,7 @@ __device__ void trace_world(const TraceScene &S, const int world, const float3 o
             next_ref = 0; next_mask = 0;
         }
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) visit_instance(S, ~next_ref, next_mask, o, d, tmax, h);
+            if (next_mask != 0) visit_instance(S, ~next_ref, next_mask, o, d, tmax, h, stats);
             if (st.sp == 0) return;
             st.pop(next_ref, next_mask);
         }
@@ -226,8 +251,9 @@ __device__ void trace_world(const TraceScene &S, const int world, const float3 o
 
 // Spheres, then the floor, then the meshes with the best so far as their limit (render_core.h:78-112).
 // Returns the object, or OBJECT_NONE; t and h describe the hit.
+template <class Stats>
 __device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, const float3 o, const float3 d,
-                                         const uint64_t live, float &best, LaneHit &h, const float start = 1e30f) {
+                                         const uint64_t live, float &best, LaneHit &h, Stats &stats, const float start = 1e30f) {
     best = start;
     int best_sphere = -1;
     for (int i = 0; i < S.sphere_count; ++i) {
@@ -258,7 +284,7 @@ __device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, c
     }
     h.inst = -1;
     h.tri = -1;
-    trace_world(S, world, o, d, live, best, h);
+    trace_world(S, world, o, d, live, best, h, stats);
     if (h.inst >= 0) return OBJECT_INSTANCE;
     if (hit_floor) return OBJECT_FLOOR;
     if (best_sphere >= 0) return OBJECT_SPHERE + best_sphere;
@@ -354,7 +380,8 @@ __global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
     const float3 d = pixel_ray(cam, A.fov_scale, A.aspect, x, y, A.width, A.height);
     float best;
     LaneHit h;
-    const int kind = scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h);
+    NoStats stats;
+    const int kind = scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats);
     if (!valid) return;
     const size_t idx = ((size_t)view * A.height + y) * A.width + x;
     const bool found = kind != OBJECT_NONE;
@@ -369,6 +396,34 @@ __global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
     }
     if (A.object_id) A.object_id[idx] = object;
 }
+
+#ifdef NGX_TRACE_STATS
+// The render kernel's traversal with its counters: each wave writes ST_COUNT ints to its own slot, lane k
+// storing counter k, so no atomics are involved.
+__global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, int *out) {
+    const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
+    if (tile >= A.total_tiles) return;
+    const int lane = lane_id();
+    const int view = (int)(tile / A.tiles_per_view);
+    const int t = (int)(tile % A.tiles_per_view);
+    const int tx = t % A.tiles_x, ty = t / A.tiles_x;
+    const int x = tx * A.tile_w + lane % A.tile_w;
+    const int y = ty * A.tile_h + lane / A.tile_w;
+    const bool valid = x < A.width && y < A.height;
+    const int world = view / A.cameras_per_world;
+    const Cam cam = load_camera(A.cams + 12 * (size_t)view);
+    const float3 d = pixel_ray(cam, A.fov_scale, A.aspect, x, y, A.width, A.height);
+    float best;
+    LaneHit h;
+    WaveStats stats;
+    for (int k = 0; k < ST_COUNT; ++k) stats.c[k] = 0;
+    stats.add(ST_VALID, popc64(ballot(valid)));
+    scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats);
+    int mine = 0;
+    for (int k = 0; k < ST_COUNT; ++k) mine = lane == k ? stats.c[k] : mine;
+    if (lane < ST_COUNT) out[(size_t)tile * ST_COUNT + lane] = mine;
+}
+#endif
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
 // worlds walks each world's rays as its own packet.
@@ -399,7 +454,8 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
         const uint64_t mine = ballot(ok && world == w);
         float b;
         LaneHit hh;
-        const int k = scene_hit(A.s, w, o, d, mine, b, hh);
+        NoStats stats;
+        const int k = scene_hit(A.s, w, o, d, mine, b, hh, stats);
         if ((mine >> lane) & 1ull) {
             best = b;
             h = hh;
@@ -493,7 +549,7 @@ constexpr int MAX_BODIES = 256;
 
 __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const DepthBackwardArgs A) {
     extern __shared__ float acc_all[];  // [BWD_WAVES][bodies][7]
-    const int view = blockIdx.x;
+    const int view = blockIdx.x / A.parts, part = blockIdx.x % A.parts;
     const int wave = uniform(threadIdx.x >> 6), lane = lane_id();
     const int B = A.bodies_per_world;
     float *acc = acc_all + (size_t)wave * B * 7;
@@ -501,7 +557,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
     const int world = view / A.cameras_per_world;
     const Cam cam = load_camera(A.cams + 12 * (size_t)view);
     const int tiles_x = (A.width + 7) / 8, tiles_y = (A.height + 7) / 8;
-    for (int t = wave; t < tiles_x * tiles_y; t += BWD_WAVES) {
+    for (int t = part * BWD_WAVES + wave; t < tiles_x * tiles_y; t += A.parts * BWD_WAVES) {
         const int x = (t % tiles_x) * 8 + (lane & 7), y = (t / tiles_x) * 8 + (lane >> 3);
         const bool valid = x < A.width && y < A.height;
         const size_t idx = ((size_t)view * A.height + (valid ? y : 0)) * A.width + (valid ? x : 0);
@@ -514,7 +570,8 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
         const float bound = want ? (depth / cosf_) * (1.0f + 1e-5f) + 1e-6f : 0.0f;
         float best;
         LaneHit h;
-        const int kind = scene_hit(A.s, world, cam.pos, d, ballot(want), best, h, bound);
+        NoStats stats;
+        const int kind = scene_hit(A.s, world, cam.pos, d, ballot(want), best, h, stats, bound);
         int body = -1;
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         bool has = false;
@@ -526,7 +583,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
     for (int i = threadIdx.x; i < B * 7; i += blockDim.x) {
         float s = 0.0f;
         for (int wv = 0; wv < BWD_WAVES; ++wv) s += acc_all[(size_t)wv * B * 7 + i];
-        A.partial[(size_t)view * B * 7 + i] = s;
+        A.partial[((size_t)view * A.parts + part) * B * 7 + i] = s;
     }
 }
 
@@ -555,7 +612,8 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const Ra
         const float bound = want ? dist * (1.0f + 1e-5f) + 1e-6f : 0.0f;
         float best;
         LaneHit h;
-        const int kind = scene_hit(A.s, world, o, d, ballot(want), best, h, bound);
+        NoStats stats;
+        const int kind = scene_hit(A.s, world, o, d, ballot(want), best, h, stats, bound);
         int body = -1;
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         bool has = false;
@@ -603,6 +661,15 @@ hipError_t launch_render(const RenderArgs &a, hipStream_t stream) {
     return hipGetLastError();
 }
 
+#ifdef NGX_TRACE_STATS
+hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream) {
+    if (a.total_tiles <= 0) return hipSuccess;
+    const long long blocks = (a.total_tiles + 3) / 4;
+    render_stats_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a, out);
+    return hipGetLastError();
+}
+#endif
+
 hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
     if (a.count <= 0) return hipSuccess;
     const long long waves = (a.count + 63) / 64;
@@ -615,7 +682,7 @@ hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream)
     if (a.views <= 0) return hipSuccess;
     if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
-    depth_backward_kernel<<<a.views, 64 * BWD_WAVES, lds, stream>>>(a);
+    depth_backward_kernel<<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a);
     return hipGetLastError();
 }
 
