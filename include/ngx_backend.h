@@ -166,8 +166,9 @@ int32_t ngx_device_id(const ngx_ctx *ctx);
 const char *ngx_last_error(void);
 
 /* Tuning of the trace kernels: tile_w x tile_h pixels per wavefront (64 pixels: 8x8, 16x4, 32x2 or 64x1),
- * and whether consecutive workgroups are kept on one XCD. */
-ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap);
+ * whether consecutive workgroups are kept on one XCD, and the walk: 1 the wavefront walks its 64 rays as a
+ * packet (the default), 0 each lane walks its own ray. */
+ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap, int32_t packet);
 
 /* ---- build intermediates, for checking the top-level build against the reference algorithm ---- */
 

@@ -68,7 +68,7 @@ _SIGNATURES = {
     "ngx_instance_count": (ctypes.c_int32, [_VP]),
     "ngx_device_id": (ctypes.c_int32, [_VP]),
     "ngx_last_error": (ctypes.c_char_p, []),
-    "ngx_set_trace_config": (ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.c_int32]),
+    "ngx_set_trace_config": (ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "ngx_debug_keep_build": (ctypes.c_int32, [_VP, ctypes.c_int32]),
     "ngx_debug_read_build": (ctypes.c_int32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "ngx_build_blas_host": (ctypes.c_int32, [_VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, c_int32_p, _VP, _VP]),

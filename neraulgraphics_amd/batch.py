@@ -147,8 +147,10 @@ class BatchRenderer:
     def synchronize(self):
         _lib.check(self._lib.ngx_synchronize(self._h))
 
-    def set_trace_config(self, tile_w=8, xcd_remap=True):
-        _lib.check(self._lib.ngx_set_trace_config(self._h, int(tile_w), int(bool(xcd_remap))))
+    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True):
+        """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, and packet (True, the
+        default) or per-lane (False) walks."""
+        _lib.check(self._lib.ngx_set_trace_config(self._h, int(tile_w), int(bool(xcd_remap)), int(bool(packet))))
 
     # ---- poses
     def set_instance_transforms(self, matrices):

@@ -100,6 +100,7 @@ struct ngx_ctx {
     int *d_tmpl_mesh = nullptr;
     float *d_mesh_bounds = nullptr;
     ngx::WideNode *d_blas_wide = nullptr;
+    int *d_blas_parent = nullptr;
     ngx::MeshRoot *d_mesh_roots = nullptr;
     float4 *d_tris = nullptr;
     float4 *d_attrs = nullptr;
@@ -111,6 +112,7 @@ struct ngx_ctx {
     ngx::InstanceRec *d_inst = nullptr;
     ngx::TlasNode *d_nodes = nullptr;
     ngx::WideNode *d_wide = nullptr;
+    int *d_wide_parent = nullptr;
     int *d_roots = nullptr;
 
     // bodies
@@ -143,10 +145,10 @@ struct ngx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     hipStream_t last_stream = nullptr;
-    int tile_w = 8, xcd_remap = 1;
+    int tile_w = 8, xcd_remap = 1, packet = 1;
 
     ~ngx_ctx() {
-        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids};
+        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_wide_parent};
         for (int *p : ints) if (p) (void)hipFree(p);
         float *floats[] = {d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
                            d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
@@ -162,6 +164,8 @@ struct ngx_ctx {
     ngx::TraceScene trace_scene() const {
         ngx::TraceScene s{};
         s.tlas_wide = d_wide;
+        s.tlas_parent = d_wide_parent;
+        s.blas_parent = d_blas_parent;
         s.roots = d_roots;
         s.total = total;
         s.inst = d_inst;
@@ -195,6 +199,7 @@ struct ngx_ctx {
         a.inst = d_inst;
         a.nodes = d_nodes;
         a.wide = d_wide;
+        a.wide_parent = d_wide_parent;
         a.roots = d_roots;
         if (keep_build) {
             a.dbg_keys = d_dbg_keys;
@@ -224,6 +229,7 @@ struct ngx_ctx {
         a.tiles_per_view = a.tiles_x * ((height + a.tile_h - 1) / a.tile_h);
         a.total_tiles = (long long)a.tiles_per_view * views;
         a.xcd_remap = xcd_remap;
+        a.packet = packet;
         return a;
     }
 
@@ -346,6 +352,8 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             ngx::host::build_meshes(mi, built);
             upload(c.d_blas_wide, built.wide.data(), built.wide.size() ? built.wide.size() : 0);
             if (!c.d_blas_wide) device_alloc(c.d_blas_wide, 1);
+            upload(c.d_blas_parent, built.wide_parent.data(), built.wide_parent.size());
+            if (!c.d_blas_parent) device_alloc(c.d_blas_parent, 1);
             upload(c.d_mesh_roots, built.mesh_roots.data(), built.mesh_roots.size());
             upload(c.d_tris, reinterpret_cast<const float4 *>(built.triangles.data()), built.triangles.size() / 4);
             if (!built.attributes.empty())
@@ -377,6 +385,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             device_alloc(c.d_inst, (size_t)c.total);
             device_alloc(c.d_nodes, 2 * (size_t)c.total - 1);
             device_alloc(c.d_wide, std::max<size_t>(1, (size_t)c.total - 1));
+            device_alloc(c.d_wide_parent, std::max<size_t>(1, (size_t)c.total - 1));
             // nodes that join worlds are never written by a build: give them a fixed value once
             HIP_CHECK(hipMemset(c.d_nodes, 0, sizeof(ngx::TlasNode) * (2 * (size_t)c.total - 1)));
             HIP_CHECK(hipMemset(c.d_wide, 0, sizeof(ngx::WideNode) * std::max<size_t>(1, (size_t)c.total - 1)));
@@ -537,6 +546,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         a.normal = (outputs & NGX_OUTPUT_NORMAL) ? ctx->d_normal : nullptr;
         a.object_id = (outputs & NGX_OUTPUT_OBJECT_ID) ? ctx->d_id : nullptr;
         a.xcd_remap = ctx->xcd_remap;
+        a.packet = ctx->packet;
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
         HIP_CHECK(ngx::launch_render(a, s));
@@ -593,6 +603,7 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
         a.distance = ctx->d_hit_dist;
         a.normal = ctx->d_hit_normal;
         a.object_id = ctx->d_hit_id;
+        a.packet = ctx->packet;
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
         HIP_CHECK(ngx::launch_rays(a, s));
@@ -718,12 +729,13 @@ int32_t ngx_instances_per_world(const ngx_ctx *ctx) { return ctx ? ctx->per_worl
 int32_t ngx_instance_count(const ngx_ctx *ctx) { return ctx ? ctx->total : 0; }
 int32_t ngx_device_id(const ngx_ctx *ctx) { return ctx ? ctx->device : 0; }
 
-ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap) {
+ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap, int32_t packet) {
     return guarded([&] {
         require(ctx != nullptr, "context is null");
         require(tile_w == 8 || tile_w == 16 || tile_w == 32 || tile_w == 64, "tile_w must be 8, 16, 32 or 64");
         ctx->tile_w = tile_w;
         ctx->xcd_remap = xcd_remap ? 1 : 0;
+        ctx->packet = packet ? 1 : 0;
     });
 }
 

@@ -229,6 +229,11 @@ void build_meshes(const MeshInput &in, BuiltMeshes &out) {
         child_ref(l, n.a, w.left, w.left_count);
         child_ref(r, n.a + 1, w.right, w.right_count);
     }
+    out.wide_parent.assign(out.wide.size(), -1);
+    for (size_t i = 0; i < out.wide.size(); ++i) {
+        if (out.wide[i].left >= 0) out.wide_parent[out.wide[i].left] = (int)i;
+        if (out.wide[i].right >= 0) out.wide_parent[out.wide[i].right] = (int)i;
+    }
     for (size_t m = 0; m < out.roots.size(); ++m) {
         const BlasNode &root = out.nodes[out.roots[m]];
         MeshRoot mr{};

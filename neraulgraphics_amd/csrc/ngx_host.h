@@ -28,6 +28,7 @@ struct BuiltMeshes {
     std::vector<int> roots;           // per mesh, in nodes
     std::vector<Box> bounds;          // per mesh, in its own space
     std::vector<WideNode> wide;       // the traversal layout of nodes' interior nodes
+    std::vector<int> wide_parent;     // per wide node: its parent wide node, -1 at a mesh's root
     std::vector<MeshRoot> mesh_roots; // per mesh, for the tracer
 };
 

@@ -31,6 +31,7 @@ struct TlasBuildArgs {
     InstanceRec *inst;         // [N]
     TlasNode *nodes;           // [2N-1], the reference layout
     WideNode *wide;            // [N-1], the traversal layout
+    int *wide_parent;          // [N-1]: the wide node whose child each one is, -1 for a world's root
     int *roots;                // [worlds]: root in `nodes` (a leaf index >= N-1 for a one-instance world), -1 if empty
 
     // Optional build intermediates, for checking the build against the reference algorithm (null: skipped).
@@ -43,6 +44,8 @@ struct TlasBuildArgs {
 
 struct TraceScene {
     const WideNode *tlas_wide;
+    const int *tlas_parent;    // per wide node, -1 at a world's root
+    const int *blas_parent;    // per bottom-level wide node, -1 at a mesh's root
     const int *roots;
     int total;
     const InstanceRec *inst;
@@ -67,6 +70,7 @@ struct RenderArgs {
     float *normal;             // [views][H][W][3] or null
     int32_t *object_id;        // [views][H][W] or null
     int xcd_remap;             // 1: consecutive tiles stay on one XCD
+    int packet;                // 1: wavefront-packet walk; 0: per-lane walk
 };
 
 struct RayArgs {
@@ -79,6 +83,7 @@ struct RayArgs {
     float *distance;
     float *normal;
     int32_t *object_id;
+    int packet;
 };
 
 // The pose backward of depth: dL/d(body pose) for every world, through each pixel's fixed hit triangle.

@@ -362,6 +362,8 @@ __global__ __launch_bounds__(MAXN) void tlas_build_kernel(const TlasBuildArgs A)
         wn.right = r >= n - 1 ? ~(a + (int)val[r - (n - 1)]) : a + r;
         wn.left_count = wn.right_count = 1;
         A.wide[gi] = wn;
+        const int pl = sh.parent[i];
+        A.wide_parent[gi] = pl < 0 ? -1 : (pl == 0 ? root_index : a + pl);
     }
 }
 

@@ -249,11 +249,182 @@ __device__ void trace_world(const TraceScene &S, const int world, const float3 o
     }
 }
 
+// ------------------------------------------------------------------------------ per-lane walk --
+//
+// Each lane walks its own ray through both levels, in exactly the reference's order (near child first,
+// mesh_dev.h:138-150, the far child visited only while its box is entered before the closest hit so far,
+// :155), so ties break as the reference breaks them. No stack: a trail of bits per level remembers, for
+// every node on the current path, whether its far child is still to visit and which side was near, and the
+// walk climbs back through parent pointers. The top and bottom levels share one node step (the operands
+// are selected per lane), so lanes working on different instances still advance together; leaves are
+// postponed until every lane of the wave has one, then instance entries and triangle runs are done
+// together (the "while-while" organisation of Aila and Laine, 2009).
+template <class Stats>
+__device__ void trace_world_lanes(const TraceScene &S, const int world, const float3 o, const float3 d, const bool live,
+                                  float &tmax, LaneHit &h, Stats &stats) {
+    constexpr int NODE = 0, LEAF = 1, BACK = 2, DONE = 3;
+    const float3 inv_d = safe_inverse(d);
+    int state = DONE, level = 0;
+    int cur_t = -1, cur_b = -1;
+    uint64_t bits_t = 0, side_t = 0, bits_b = 0, side_b = 0;
+    int leaf_ref = 0, leaf_count = 0, g = 0;
+    float3 lo = o, ld = d, inv_ld = inv_d;
+    if (live) {
+        const int root = S.roots[world];
+        if (root >= S.total - 1) {
+            state = LEAF;
+            leaf_ref = ~(root - (S.total - 1));
+        } else if (root >= 0) {
+            state = NODE;
+            cur_t = root;
+        }
+    }
+    while (ballot(state != DONE)) {
+        // ---- node steps until every lane waits at a leaf or is done
+        while (true) {
+            const bool on_node = state == NODE, on_back = state == BACK;
+            const uint64_t active = ballot(on_node || on_back);
+            if (active == 0) break;
+            stats.add(ST_TLAS_NODES, 1);
+            stats.add(ST_TLAS_LANES, popc64(active));
+            if (on_node || on_back) {
+                const bool bottom = level == 1;
+                int cur = bottom ? cur_b : cur_t;
+                uint64_t bits = bottom ? bits_b : bits_t;
+                uint64_t side = bottom ? side_b : side_t;
+                const bool pending = on_back && (bits & 1ull);
+                if (on_node || pending) {
+                    const float4 *np = reinterpret_cast<const float4 *>((bottom ? S.blas_wide : S.tlas_wide) + cur);
+                    const float4 f0 = np[0], f1 = np[1], f2 = np[2];
+                    const int4 f3 = *reinterpret_cast<const int4 *>(np + 3);
+                    const float3 ro = bottom ? lo : o, ri = bottom ? inv_ld : inv_d;
+                    const float dl = box_entry(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, ro, ri, tmax);
+                    const float dr = box_entry(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, ro, ri, tmax);
+                    const bool hl = dl < NO_HIT, hr = dr < NO_HIT;
+                    int child = 0, child_count = 0;
+                    bool go = false;
+                    if (on_node) {
+                        if (hl && hr) {
+                            const bool left_first = dl <= dr;
+                            bits |= 1ull;
+                            side = left_first ? (side & ~1ull) : (side | 1ull);
+                            child = left_first ? f3.x : f3.y;
+                            child_count = left_first ? f3.z : f3.w;
+                            go = true;
+                        } else if (hl || hr) {
+                            child = hl ? f3.x : f3.y;
+                            child_count = hl ? f3.z : f3.w;
+                            go = true;
+                        } else {
+                            state = BACK;
+                        }
+                    } else {
+                        bits &= ~1ull;
+                        const bool far_left = (side & 1ull) != 0;  // the near child was the right one
+                        if (far_left ? hl : hr) {
+                            child = far_left ? f3.x : f3.y;
+                            child_count = far_left ? f3.z : f3.w;
+                            go = true;
+                        }
+                    }
+                    if (go) {
+                        if (child >= 0) {
+                            cur = child;
+                            bits <<= 1;
+                            side <<= 1;
+                            state = NODE;
+                        } else {
+                            state = LEAF;
+                            leaf_ref = child;
+                            leaf_count = child_count;
+                        }
+                    }
+                } else {
+                    // climb one level; a level whose root is finished hands back to the level above
+                    const int parent = cur < 0 ? -1 : (bottom ? S.blas_parent : S.tlas_parent)[cur];
+                    if (parent < 0) {
+                        if (bottom) level = 0;  // the instance is done: resume the top level where it left off
+                        else state = DONE;
+                    } else {
+                        cur = parent;
+                        bits >>= 1;
+                        side >>= 1;
+                    }
+                }
+                if (bottom) {
+                    cur_b = cur;
+                    bits_b = bits;
+                    side_b = side;
+                } else {
+                    cur_t = cur;
+                    bits_t = bits;
+                    side_t = side;
+                }
+            }
+        }
+        // ---- instance entries: the ray moved into mesh space, then the mesh's root box
+        const bool enter = state == LEAF && level == 0;
+        if (ballot(enter)) {
+            stats.add(ST_INSTANCES, 1);
+            stats.add(ST_INSTANCE_LANES, popc64(ballot(enter)));
+            if (enter) {
+                g = ~leaf_ref;
+                const float4 *rp = reinterpret_cast<const float4 *>(S.inst + g);
+                const float4 r0 = rp[0], r1 = rp[1], r2 = rp[2];
+                const int mesh = reinterpret_cast<const int4 *>(rp + 3)->x;
+                lo = mk3(dot3(xyz(r0), o) + r0.w, dot3(xyz(r1), o) + r1.w, dot3(xyz(r2), o) + r2.w);
+                ld = mk3(dot3(xyz(r0), d), dot3(xyz(r1), d), dot3(xyz(r2), d));
+                inv_ld = safe_inverse(ld);
+                const float4 *mp = reinterpret_cast<const float4 *>(S.mesh_roots + mesh);
+                const float4 m0 = mp[0], m1 = mp[1];
+                if (box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_ld, tmax) < NO_HIT) {
+                    level = 1;
+                    bits_b = side_b = 0;
+                    const int root_ref = __float_as_int(m0.w);
+                    if (root_ref >= 0) {
+                        cur_b = root_ref;
+                        state = NODE;
+                    } else {
+                        cur_b = -1;
+                        leaf_ref = root_ref;
+                        leaf_count = __float_as_int(m1.w);
+                    }
+                } else {
+                    state = BACK;
+                }
+            }
+        }
+        // ---- triangle runs
+        const bool tris = state == LEAF && level == 1;
+        if (ballot(tris)) {
+            stats.add(ST_TRIANGLES, 1);
+            stats.add(ST_TRIANGLE_LANES, popc64(ballot(tris)));
+            if (tris) {
+                const int first = ~leaf_ref;
+                for (int j = first; j < first + leaf_count; ++j) {
+                    const float4 *tp = S.tris + 3 * (size_t)j;
+                    float t, bu, bv;
+                    if (triangle_hit(xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), lo, ld, tmax, t, bu, bv)) {
+                        tmax = t;
+                        h.t = t;
+                        h.inst = g;
+                        h.tri = j;
+                        h.u = bu;
+                        h.v = bv;
+                    }
+                }
+                state = BACK;
+            }
+        }
+    }
+}
+
 // Spheres, then the floor, then the meshes with the best so far as their limit (render_core.h:78-112).
 // Returns the object, or OBJECT_NONE; t and h describe the hit.
 template <class Stats>
 __device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, const float3 o, const float3 d,
-                                         const uint64_t live, float &best, LaneHit &h, Stats &stats, const float start = 1e30f) {
+                                         const uint64_t live, float &best, LaneHit &h, Stats &stats, const float start = 1e30f,
+                                         const bool packet = false) {
     best = start;
     int best_sphere = -1;
     for (int i = 0; i < S.sphere_count; ++i) {
@@ -284,7 +455,8 @@ __device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, c
     }
     h.inst = -1;
     h.tri = -1;
-    trace_world(S, world, o, d, live, best, h, stats);
+    if (packet) trace_world(S, world, o, d, live, best, h, stats);
+    else trace_world_lanes(S, world, o, d, ((live >> lane_id()) & 1ull) != 0, best, h, stats);
     if (h.inst >= 0) return OBJECT_INSTANCE;
     if (hit_floor) return OBJECT_FLOOR;
     if (best_sphere >= 0) return OBJECT_SPHERE + best_sphere;
@@ -381,7 +553,7 @@ __global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
     float best;
     LaneHit h;
     NoStats stats;
-    const int kind = scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats);
+    const int kind = scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats, 1e30f, A.packet != 0);
     if (!valid) return;
     const size_t idx = ((size_t)view * A.height + y) * A.width + x;
     const bool found = kind != OBJECT_NONE;
@@ -418,7 +590,7 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
     WaveStats stats;
     for (int k = 0; k < ST_COUNT; ++k) stats.c[k] = 0;
     stats.add(ST_VALID, popc64(ballot(valid)));
-    scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats);
+    scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats, 1e30f, A.packet != 0);
     int mine = 0;
     for (int k = 0; k < ST_COUNT; ++k) mine = lane == k ? stats.c[k] : mine;
     if (lane < ST_COUNT) out[(size_t)tile * ST_COUNT + lane] = mine;
@@ -448,20 +620,26 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     h.inst = -1;
     h.tri = -1;
     int kind = OBJECT_NONE;
-    uint64_t pending = ballot(ok);
-    while (pending != 0) {
-        const int w = __builtin_amdgcn_readlane(world, first_lane(pending));
-        const uint64_t mine = ballot(ok && world == w);
-        float b;
-        LaneHit hh;
-        NoStats stats;
-        const int k = scene_hit(A.s, w, o, d, mine, b, hh, stats);
-        if ((mine >> lane) & 1ull) {
-            best = b;
-            h = hh;
-            kind = k;
+    if (A.packet) {
+        uint64_t pending = ballot(ok);
+        while (pending != 0) {
+            const int w = __builtin_amdgcn_readlane(world, first_lane(pending));
+            const uint64_t mine = ballot(ok && world == w);
+            float b;
+            LaneHit hh;
+            NoStats stats;
+            const int k = scene_hit(A.s, w, o, d, mine, b, hh, stats, 1e30f, true);
+            if ((mine >> lane) & 1ull) {
+                best = b;
+                h = hh;
+                kind = k;
+            }
+            pending &= ~mine;
         }
-        pending &= ~mine;
+    } else {
+        // each lane walks its own world
+        NoStats stats;
+        kind = scene_hit(A.s, world, o, d, ballot(ok), best, h, stats, 1e30f, false);
     }
     if (!valid) return;
     const bool found = ok && kind != OBJECT_NONE && best <= A.max_distance;

@@ -53,11 +53,13 @@ def test_top_level_build_matches_the_reference_algorithm(gpu, oracle, index, wor
     _check_build(r, ref, ref["world_off"])
 
 
+@pytest.mark.parametrize("walk", ["packet", "lanes"])
 @pytest.mark.parametrize("index,worlds", [(0, None), (1, 32), (2, 16), (4, 16)])
-def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds):
-    _say(f"render parity: config {index}, worlds {worlds}")
+def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds, walk):
+    _say(f"render parity: config {index}, worlds {worlds}, {walk} walk")
     cfg = workloads.config(index, worlds=worlds)
     r = workloads.make_renderer(cfg)
+    r.set_trace_config(packet=walk == "packet")
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
     rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, f"config {index}")
@@ -65,10 +67,12 @@ def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds):
     assert rep["hits"] > 0.05 * rep["pixels"]
 
 
-def test_ray_queries_match_the_oracle(gpu, oracle):
-    _say("ray parity: config 3, 8 worlds x 4096 rays")
+@pytest.mark.parametrize("walk", ["packet", "lanes"])
+def test_ray_queries_match_the_oracle(gpu, oracle, walk):
+    _say(f"ray parity: config 3, 8 worlds x 4096 rays, {walk} walk")
     cfg = workloads.config(3, worlds=8, rays_per_world=4096)
     r = workloads.make_renderer(cfg)
+    r.set_trace_config(packet=walk == "packet")
     out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
     st = forward(cfg, oracle, outputs=())
     ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
@@ -78,12 +82,14 @@ def test_ray_queries_match_the_oracle(gpu, oracle):
     assert rep["hits"] > 0
 
 
-def test_rays_of_uneven_worlds_per_wave(gpu, oracle):
-    _say("ray parity: 37 rays per world, so waves straddle worlds")
+@pytest.mark.parametrize("walk", ["packet", "lanes"])
+def test_rays_of_uneven_worlds_per_wave(gpu, oracle, walk):
+    _say(f"ray parity: 37 rays per world, so waves straddle worlds, {walk} walk")
     cfg = workloads.config(3, worlds=8, rays_per_world=37 * 64)
     o = cfg["origins"][:, :37]
     d = cfg["directions"][:, ::64][:, :37]
     r = workloads.make_renderer(cfg)
+    r.set_trace_config(packet=walk == "packet")
     out = r.cast_rays(o, d, 100.0)
     st = forward(cfg, oracle, outputs=())
     ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
