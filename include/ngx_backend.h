@@ -182,6 +182,13 @@ ngx_status ngx_debug_keep_build(ngx_ctx *ctx, int32_t keep);
 ngx_status ngx_debug_read_build(ngx_ctx *ctx, void *nodes, int32_t *roots, float *inv, float *matrices, uint64_t *keys,
                                 uint64_t *sorted_keys, int32_t *sorted_ids, float *aabb, float *world_bounds);
 
+/* Checks, on device device_id, the reciprocal the tracer uses for 1/x (the hardware estimate and one
+ * fused Newton step) against IEEE division for all 2^32 float bit patterns: mismatches among
+ * 2^-126 <= |x| < 2^126 (where the tracer uses it), among all finite nonzero x, and one mismatching
+ * pattern of the first kind (0 if none). About a second of device time. */
+ngx_status ngx_debug_check_reciprocal(int32_t device_id, uint64_t *mismatches_in_range, uint64_t *mismatches_anywhere,
+                                      uint32_t *example);
+
 /* The host-side bottom-level build of one mesh (src/mesh_build.cpp:91-157 with src/bvh.cpp:46-174), no device
  * needed: nodes (capacity 2 x triangles) x 32 bytes in the reference's Node layout, triangles
  * (triangles x 12 floats, v0 e1 e2 padded), order (triangles). Returns the node count in *node_count. */

@@ -650,6 +650,7 @@ ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const 
         a.body_poses = body_poses;
         a.parts = parts;
         a.partial = ctx->d_partial;
+        a.packet = ctx->packet;
         HIP_CHECK(ngx::launch_depth_backward(a, s));
         ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw * parts, ctx->d_partial, body_poses, grad_out};
         HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
@@ -681,6 +682,7 @@ ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const 
         a.dirs = directions;
         a.rays_per_world = rays_per_world;
         a.chunks = chunks;
+        a.packet = ctx->packet;
         a.max_distance = max_distance;
         a.distance = distance;
         a.grad_distance = grad_distance;

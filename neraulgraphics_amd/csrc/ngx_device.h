@@ -77,11 +77,23 @@ __device__ __forceinline__ float3 cross3(const float3 a, const float3 b) {
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ float3 neg3(const float3 a) { return mk3(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ float3 normalize3(const float3 v) { return mul3(v, 1.0f / sqrtf(dot3(v, v))); }
 __device__ __forceinline__ float3 xyz(const float4 v) { return mk3(v.x, v.y, v.z); }
 
+// 1/x rounded to nearest, as IEEE division rounds it, in three instructions: the hardware reciprocal
+// estimate and one Newton step with fused multiply-adds. Exhaustively equal to 1.0f / x for every
+// 2^-126 <= |x| < 2^126 (checked on the device by ngx_debug_check_reciprocal); outside that range the
+// division is done.
+__device__ __forceinline__ float rcp_rn(const float x) {
+    if (__builtin_expect(!(fabsf(x) >= 0x1p-126f && fabsf(x) < 0x1p126f), 0)) return 1.0f / x;
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+}
+
+__device__ __forceinline__ float3 normalize3(const float3 v) { return mul3(v, rcp_rn(sqrtf(dot3(v, v)))); }
+
 // 1/x that never gives infinity (mesh_dev.h:58-61).
-__device__ __forceinline__ float safe_rcp(const float x) { return 1.0f / (fabsf(x) < 1e-12f ? copysignf(1e-12f, x) : x); }
+__device__ __forceinline__ float safe_rcp(const float x) { return rcp_rn(fabsf(x) < 1e-12f ? copysignf(1e-12f, x) : x); }
 __device__ __forceinline__ float3 safe_inverse(const float3 d) { return mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)); }
 
 // Slab test (mesh_dev.h:66-74): entry distance, or NO_HIT when missed or beyond tmax.
@@ -101,7 +113,7 @@ __device__ __forceinline__ bool triangle_hit(const float3 v0, const float3 e1, c
     const float3 p = cross3(d, e2);
     const float det = dot3(e1, p);
     if (fabsf(det) < 1e-14f) return false;
-    const float inv_det = 1.0f / det;
+    const float inv_det = rcp_rn(det);
     const float3 s = sub3(o, v0);
     const float u = dot3(s, p) * inv_det;
     if (u < 0.0f || u > 1.0f) return false;
@@ -124,6 +136,8 @@ __device__ __forceinline__ float uniform_f(const float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 __device__ __forceinline__ int popc64(const uint64_t m) { return __popcll(m); }
+// Whether this lane's bit is set in a wave-uniform mask: the mask becomes the execution mask, no vector work.
+__device__ __forceinline__ bool in_mask(const uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 __device__ __forceinline__ int first_lane(const uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
 
 // Loads through the scalar data cache: the address must be the same in every lane of the wave. The

@@ -100,6 +100,7 @@ struct DepthBackwardArgs {
     const float *body_poses;   // [worlds][bodies][7], the poses the tree was built from
     int parts;                 // workgroups per view, each taking every parts-th group of tiles
     float *partial;            // [views][parts][bodies][7]
+    int packet;                // the re-trace's walk, as the forward's
 };
 
 struct RangeBackwardArgs {
@@ -116,6 +117,7 @@ struct RangeBackwardArgs {
     const float *body_poses;
     int worlds;
     float *partial;            // [worlds][chunks][bodies][7]
+    int packet;
 };
 
 struct PoseGradFinishArgs {

@@ -18,7 +18,6 @@ namespace ngx {
 namespace {
 
 struct LaneHit {
-    float t;
     int inst;
     int tri;
     float u, v;
@@ -94,34 +93,101 @@ __device__ __forceinline__ Wide load_wide(const WideNode *nodes, const int index
     return w;
 }
 
-// Tests the triangles first .. first+count-1 for the lanes in mask (mesh_dev.h:186-199).
+// ---------------------------------------------------------------- batched triangle tests --
+//
+// Below an instance the packet thins out: a leaf is typically reached by a handful of the wave's rays, so
+// testing its triangles one after the other for the whole wave leaves most lanes idle. Instead a leaf
+// reached by m rays adds m x count (ray, triangle) pairs to a table in LDS, and once 64 pairs are queued,
+// or the instance is finished, every lane tests one pair. A ray's pairs are numbered in the order the walk
+// met its triangles, and the ray keeps the closest hit, the lowest pair number among equal distances, with
+// one 64-bit LDS minimum per pair (distance bits above pair number; distances are positive, so their bits
+// order as the floats do). That is what testing the triangles one by one with a strict "closer than"
+// keeps (mesh_dev.h:98, :186-199). A ray's limit is not lowered until its batch is tested, so a few
+// more boxes may be entered; that costs time, not correctness.
+struct PairLds {
+    int pair[64];                  // ray lane | triangle << 6
+    unsigned long long best[64];   // per ray lane: distance bits << 32 | pair number
+    float2 uv[64];                 // per pair: barycentrics of its hit
+    float2 hit_uv[64];             // per ray lane: barycentrics of its closest hit so far
+};
+
+// Orders a wavefront's LDS accesses: a wavefront's LDS operations are performed in order, so a
+// compiler barrier is all that is needed between one lane's write and another lane's read.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int rank_in(const uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Tests the queued pairs of instance inst, whose mesh-space ray is (lo, ld) in each ray's lane.
 template <class Stats>
-__device__ __forceinline__ void leaf_triangles(const TraceScene &S, const int first, const int count, const uint64_t mask,
-                                               const int inst, const float3 lo, const float3 ld, float &tmax, LaneHit &h,
-                                               Stats &stats) {
-    const bool in = (mask >> lane_id()) & 1ull;
-    stats.add(ST_TRIANGLES, count);
-    stats.add(ST_TRIANGLE_LANES, count * popc64(mask));
-    for (int j = first; j < first + count; ++j) {
-        const float4 *tp = S.tris + 3 * (size_t)j;
-        const float3 v0 = xyz(sload4(tp)), e1 = xyz(sload4(tp + 1)), e2 = xyz(sload4(tp + 2));
+__device__ __forceinline__ void flush_pairs(const TraceScene &S, PairLds *L, int &pairs, const int inst, const float3 lo,
+                                            const float3 ld, float &tmax, LaneHit &h, Stats &stats) {
+    if (pairs == 0) return;
+    stats.add(ST_TRIANGLES, 1);
+    stats.add(ST_TRIANGLE_LANES, pairs);
+    const int lane = lane_id();
+    L->best[lane] = ((unsigned long long)__float_as_uint(tmax) << 32) | 0xffffffffull;
+    wave_lds_order();
+    const bool act = lane < pairs;
+    const int e = act ? L->pair[lane] : 0;
+    const int r = e & 63;
+    const float3 po = mk3(__shfl(lo.x, r, 64), __shfl(lo.y, r, 64), __shfl(lo.z, r, 64));
+    const float3 pd = mk3(__shfl(ld.x, r, 64), __shfl(ld.y, r, 64), __shfl(ld.z, r, 64));
+    const float pt = __shfl(tmax, r, 64);
+    if (act) {
+        const float4 *tp = S.tris + 3 * (size_t)(e >> 6);
         float t, bu, bv;
-        if (triangle_hit(v0, e1, e2, lo, ld, tmax, t, bu, bv) && in) {
-            tmax = t;
-            h.t = t;
-            h.inst = inst;
-            h.tri = j;
-            h.u = bu;
-            h.v = bv;
+        if (triangle_hit(xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), po, pd, pt, t, bu, bv)) {
+            atomicMin(&L->best[r], ((unsigned long long)__float_as_uint(t) << 32) | (unsigned)lane);
+            L->uv[lane] = make_float2(bu, bv);
         }
+    }
+    wave_lds_order();
+    const unsigned long long b = L->best[lane];
+    const unsigned p = (unsigned)b;
+    if (p != 0xffffffffu) {
+        tmax = __uint_as_float((unsigned)(b >> 32));
+        h.inst = inst;
+        h.tri = L->pair[p] >> 6;
+        L->hit_uv[lane] = L->uv[p];  // kept in LDS, not registers, until the walk is over
+    }
+    wave_lds_order();
+    pairs = 0;
+}
+
+// Queues the triangles first .. first+count-1 for the rays in mask, testing the queue first whenever it
+// would overflow.
+template <class Stats>
+__device__ __forceinline__ void add_leaf(const TraceScene &S, PairLds *L, int &pairs, const int first, const int count,
+                                         const uint64_t mask, const int inst, const float3 lo, const float3 ld, float &tmax,
+                                         LaneHit &h, Stats &stats) {
+    const int m = popc64(mask);
+    const int lane = lane_id();
+    const bool in = in_mask(mask);
+    const int rank = rank_in(mask);
+    const int fit = 64 / m;
+    for (int done = 0; done < count;) {
+        const int c = min(count - done, fit);
+        if (pairs + m * c > 64) flush_pairs(S, L, pairs, inst, lo, ld, tmax, h, stats);
+        if (in) {
+            const int base = pairs + rank * c;
+            for (int k = 0; k < c; ++k) L->pair[base + k] = lane | ((first + done + k) << 6);
+        }
+        pairs += m * c;
+        done += c;
     }
 }
 
 // One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
 // renormalized so t is shared), then the mesh's own tree.
 template <class Stats>
-__device__ void visit_instance(const TraceScene &S, const int g, const uint64_t mask, const float3 o, const float3 d,
-                               float &tmax, LaneHit &h, Stats &stats) {
+__device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, const uint64_t mask, const float3 o,
+                               const float3 d, float &tmax, LaneHit &h, Stats &stats) {
     stats.add(ST_INSTANCES, 1);
     stats.add(ST_INSTANCE_LANES, popc64(mask));
     const float4 *rp = reinterpret_cast<const float4 *>(S.inst + g);
@@ -133,13 +199,15 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
     const float4 *mp = reinterpret_cast<const float4 *>(S.mesh_roots + meta.x);
     const float4 m0 = sload4(mp), m1 = sload4(mp + 1);
     const int lane = lane_id();
-    const bool in = (mask >> lane) & 1ull;
+    const bool in = in_mask(mask);
     const uint64_t root_mask = ballot(in && box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, tmax) < NO_HIT);
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     const int root_ref = __float_as_int(m0.w);
+    int pairs = 0;
     if (root_ref < 0) {
-        leaf_triangles(S, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h, stats);
+        add_leaf(S, L, pairs, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h, stats);
+        flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
         return;
     }
     WaveStack<1> st;
@@ -150,7 +218,7 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
         const Wide w = load_wide(S.blas_wide, cur);
         stats.add(ST_BLAS_NODES, 1);
         stats.add(ST_BLAS_LANES, popc64(cm));
-        const bool on = (cm >> lane) & 1ull;
+        const bool on = in_mask(cm);
         const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], lo, inv_d, tmax) : NO_HIT;
         const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], lo, inv_d, tmax) : NO_HIT;
         const uint64_t ml = ballot(dl < NO_HIT), mr = ballot(dr < NO_HIT);
@@ -174,10 +242,14 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
         } else {
             next_ref = 0; next_count = 0; next_mask = 0;
         }
-        // Leaves are tested where they are reached; then the walk resumes from the stack.
+        // Leaves are queued where they are reached; then the walk resumes from the stack.
+        bool finished = false;
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) leaf_triangles(S, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
-            if (st.sp == 0) return;
+            if (next_mask != 0) add_leaf(S, L, pairs, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
+            if (st.sp == 0) {
+                finished = true;
+                break;
+            }
             int r;
             st.pop(r, next_mask);
             if (r >= 0) {
@@ -189,26 +261,31 @@ __device__ void visit_instance(const TraceScene &S, const int g, const uint64_t 
                 next_count = side ? pw.rcount : pw.lcount;
             }
         }
+        if (finished) break;
         cur = next_ref;
         cm = next_mask;
     }
+    flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
 }
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
 // distance to beat on entry and its closest distance on return.
 template <class Stats>
-__device__ void trace_world(const TraceScene &S, const int world, const float3 o, const float3 d, const uint64_t live,
-                            float &tmax, LaneHit &h, Stats &stats) {
+__device__ void trace_world(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
+                            const uint64_t live, float &tmax, LaneHit &h, Stats &stats) {
     if (live == 0) return;
     const int root = uniform(S.roots[world]);
     if (root < 0) return;
     if (root >= S.total - 1) {
-        visit_instance(S, root - (S.total - 1), live, o, d, tmax, h, stats);
+        visit_instance(S, L, root - (S.total - 1), live, o, d, tmax, h, stats);
         return;
     }
     const float3 inv_d = safe_inverse(d);
     const int lane = lane_id();
-    WaveStack<2> st;
+    // One entry is pushed per level. A world's tree holds at most 1024 instances, and below its root every
+    // level fixes at least one more bit of a Morton code (30 bits) or, among equal codes, of the leaves'
+    // positions (at most 11 bits over 1024 consecutive positions): at most 42 levels, so 64 entries do.
+    WaveStack<1> st;
     st.init();
     int cur = root;
     uint64_t cm = live;
@@ -216,7 +293,7 @@ __device__ void trace_world(const TraceScene &S, const int world, const float3 o
         const Wide w = load_wide(S.tlas_wide, cur);
         stats.add(ST_TLAS_NODES, 1);
         stats.add(ST_TLAS_LANES, popc64(cm));
-        const bool on = (cm >> lane) & 1ull;
+        const bool on = in_mask(cm);
         const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], o, inv_d, tmax) : NO_HIT;
         const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], o, inv_d, tmax) : NO_HIT;
         const uint64_t ml = ballot(dl < NO_HIT), mr = ballot(dr < NO_HIT);
@@ -240,7 +317,7 @@ __device__ void trace_world(const TraceScene &S, const int world, const float3 o
             next_ref = 0; next_mask = 0;
         }
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) visit_instance(S, ~next_ref, next_mask, o, d, tmax, h, stats);
+            if (next_mask != 0) visit_instance(S, L, ~next_ref, next_mask, o, d, tmax, h, stats);
             if (st.sp == 0) return;
             st.pop(next_ref, next_mask);
         }
@@ -406,7 +483,6 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
                     float t, bu, bv;
                     if (triangle_hit(xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), lo, ld, tmax, t, bu, bv)) {
                         tmax = t;
-                        h.t = t;
                         h.inst = g;
                         h.tri = j;
                         h.u = bu;
@@ -421,10 +497,9 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
 
 // Spheres, then the floor, then the meshes with the best so far as their limit (render_core.h:78-112).
 // Returns the object, or OBJECT_NONE; t and h describe the hit.
-template <class Stats>
-__device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, const float3 o, const float3 d,
-                                         const uint64_t live, float &best, LaneHit &h, Stats &stats, const float start = 1e30f,
-                                         const bool packet = false) {
+template <bool PACKET, class Stats>
+__device__ __forceinline__ int scene_hit(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
+                                         const uint64_t live, float &best, LaneHit &h, Stats &stats, const float start = 1e30f) {
     best = start;
     int best_sphere = -1;
     for (int i = 0; i < S.sphere_count; ++i) {
@@ -455,8 +530,15 @@ __device__ __forceinline__ int scene_hit(const TraceScene &S, const int world, c
     }
     h.inst = -1;
     h.tri = -1;
-    if (packet) trace_world(S, world, o, d, live, best, h, stats);
-    else trace_world_lanes(S, world, o, d, ((live >> lane_id()) & 1ull) != 0, best, h, stats);
+    if (PACKET) {
+        trace_world(S, L, world, o, d, live, best, h, stats);
+        wave_lds_order();
+        if (h.inst >= 0) {
+            const float2 uv = L->hit_uv[lane_id()];
+            h.u = uv.x;
+            h.v = uv.y;
+        }
+    } else trace_world_lanes(S, world, o, d, ((live >> lane_id()) & 1ull) != 0, best, h, stats);
     if (h.inst >= 0) return OBJECT_INSTANCE;
     if (hit_floor) return OBJECT_FLOOR;
     if (best_sphere >= 0) return OBJECT_SPHERE + best_sphere;
@@ -537,7 +619,12 @@ __device__ __forceinline__ long long wave_tile(const int xcd_remap, const long l
     return b * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
 }
 
-__global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
+// LDS of the packet walk's triangle batches: one PairLds per wavefront of a 256-thread workgroup.
+#define NGX_PAIR_LDS(name) __shared__ PairLds name##_all[4]; PairLds *name = PACKET ? &name##_all[threadIdx.x >> 6] : nullptr
+
+template <bool PACKET>
+__global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_kernel(const RenderArgs A) {
+    NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
     if (tile >= A.total_tiles) return;
     const int lane = lane_id();
@@ -553,7 +640,7 @@ __global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
     float best;
     LaneHit h;
     NoStats stats;
-    const int kind = scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats, 1e30f, A.packet != 0);
+    const int kind = scene_hit<PACKET>(A.s, L, world, cam.pos, d, ballot(valid), best, h, stats);
     if (!valid) return;
     const size_t idx = ((size_t)view * A.height + y) * A.width + x;
     const bool found = kind != OBJECT_NONE;
@@ -572,7 +659,9 @@ __global__ __launch_bounds__(256) void render_kernel(const RenderArgs A) {
 #ifdef NGX_TRACE_STATS
 // The render kernel's traversal with its counters: each wave writes ST_COUNT ints to its own slot, lane k
 // storing counter k, so no atomics are involved.
+template <bool PACKET>
 __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, int *out) {
+    NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
     if (tile >= A.total_tiles) return;
     const int lane = lane_id();
@@ -590,7 +679,7 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
     WaveStats stats;
     for (int k = 0; k < ST_COUNT; ++k) stats.c[k] = 0;
     stats.add(ST_VALID, popc64(ballot(valid)));
-    scene_hit(A.s, world, cam.pos, d, ballot(valid), best, h, stats, 1e30f, A.packet != 0);
+    scene_hit<PACKET>(A.s, L, world, cam.pos, d, ballot(valid), best, h, stats);
     int mine = 0;
     for (int k = 0; k < ST_COUNT; ++k) mine = lane == k ? stats.c[k] : mine;
     if (lane < ST_COUNT) out[(size_t)tile * ST_COUNT + lane] = mine;
@@ -599,7 +688,9 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
 // worlds walks each world's rays as its own packet.
+template <bool PACKET>
 __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
+    NGX_PAIR_LDS(L);
     const long long base = ((long long)blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6)) * 64;
     if (base >= A.count) return;
     const int lane = lane_id();
@@ -613,14 +704,14 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
         d = mk3(A.dirs[3 * i], A.dirs[3 * i + 1], A.dirs[3 * i + 2]);
         const float len2 = dot3(d, d);
         ok = len2 > 1e-30f && len2 < 1e30f;
-        if (ok) d = mul3(d, 1.0f / sqrtf(len2));
+        if (ok) d = mul3(d, rcp_rn(sqrtf(len2)));
     }
     float best = 1e30f;
     LaneHit h;
     h.inst = -1;
     h.tri = -1;
     int kind = OBJECT_NONE;
-    if (A.packet) {
+    if (PACKET) {
         uint64_t pending = ballot(ok);
         while (pending != 0) {
             const int w = __builtin_amdgcn_readlane(world, first_lane(pending));
@@ -628,8 +719,8 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
             float b;
             LaneHit hh;
             NoStats stats;
-            const int k = scene_hit(A.s, w, o, d, mine, b, hh, stats, 1e30f, true);
-            if ((mine >> lane) & 1ull) {
+            const int k = scene_hit<true>(A.s, L, w, o, d, mine, b, hh, stats);
+            if (in_mask(mine)) {
                 best = b;
                 h = hh;
                 kind = k;
@@ -639,7 +730,7 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     } else {
         // each lane walks its own world
         NoStats stats;
-        kind = scene_hit(A.s, world, o, d, ballot(ok), best, h, stats, 1e30f, false);
+        kind = scene_hit<false>(A.s, L, world, o, d, ballot(ok), best, h, stats);
     }
     if (!valid) return;
     const bool found = ok && kind != OBJECT_NONE && best <= A.max_distance;
@@ -725,8 +816,10 @@ __device__ __forceinline__ void wave_accumulate(const bool has, const int body, 
 constexpr int BWD_WAVES = 4;
 constexpr int MAX_BODIES = 256;
 
+template <bool PACKET>
 __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const DepthBackwardArgs A) {
     extern __shared__ float acc_all[];  // [BWD_WAVES][bodies][7]
+    NGX_PAIR_LDS(L);
     const int view = blockIdx.x / A.parts, part = blockIdx.x % A.parts;
     const int wave = uniform(threadIdx.x >> 6), lane = lane_id();
     const int B = A.bodies_per_world;
@@ -749,7 +842,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
         float best;
         LaneHit h;
         NoStats stats;
-        const int kind = scene_hit(A.s, world, cam.pos, d, ballot(want), best, h, stats, bound);
+        const int kind = scene_hit<PACKET>(A.s, L, world, cam.pos, d, ballot(want), best, h, stats, bound);
         int body = -1;
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         bool has = false;
@@ -765,8 +858,10 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
     }
 }
 
+template <bool PACKET>
 __global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const RangeBackwardArgs A) {
     extern __shared__ float acc_all[];
+    NGX_PAIR_LDS(L);
     const int world = blockIdx.x / A.chunks, chunk = blockIdx.x % A.chunks;
     const int wave = uniform(threadIdx.x >> 6), lane = lane_id();
     const int B = A.bodies_per_world;
@@ -783,7 +878,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const Ra
         float3 d = mk3(A.dirs[3 * i], A.dirs[3 * i + 1], A.dirs[3 * i + 2]);
         const float len2 = dot3(d, d);
         const bool ok = valid && len2 > 1e-30f && len2 < 1e30f;
-        if (ok) d = mul3(d, 1.0f / sqrtf(len2));
+        if (ok) d = mul3(d, rcp_rn(sqrtf(len2)));
         const float dist = valid ? A.distance[i] : 0.0f;
         const float gd = valid ? A.grad_distance[i] : 0.0f;
         const bool want = ok && gd != 0.0f && dist < __int_as_float(0x7f800000);
@@ -791,7 +886,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const Ra
         float best;
         LaneHit h;
         NoStats stats;
-        const int kind = scene_hit(A.s, world, o, d, ballot(want), best, h, stats, bound);
+        const int kind = scene_hit<PACKET>(A.s, L, world, o, d, ballot(want), best, h, stats, bound);
         int body = -1;
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         bool has = false;
@@ -835,7 +930,8 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
 hipError_t launch_render(const RenderArgs &a, hipStream_t stream) {
     if (a.total_tiles <= 0) return hipSuccess;
     const long long blocks = (a.total_tiles + 3) / 4;
-    render_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a);
+    if (a.packet) render_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a);
+    else render_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a);
     return hipGetLastError();
 }
 
@@ -843,7 +939,8 @@ hipError_t launch_render(const RenderArgs &a, hipStream_t stream) {
 hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream) {
     if (a.total_tiles <= 0) return hipSuccess;
     const long long blocks = (a.total_tiles + 3) / 4;
-    render_stats_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a, out);
+    if (a.packet) render_stats_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, out);
+    else render_stats_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, out);
     return hipGetLastError();
 }
 #endif
@@ -852,7 +949,8 @@ hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
     if (a.count <= 0) return hipSuccess;
     const long long waves = (a.count + 63) / 64;
     const long long blocks = (waves + 3) / 4;
-    ray_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a);
+    if (a.packet) ray_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a);
+    else ray_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a);
     return hipGetLastError();
 }
 
@@ -860,7 +958,8 @@ hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream)
     if (a.views <= 0) return hipSuccess;
     if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
-    depth_backward_kernel<<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a);
+    if (a.packet) depth_backward_kernel<true><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a);
+    else depth_backward_kernel<false><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a);
     return hipGetLastError();
 }
 
@@ -868,7 +967,8 @@ hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream)
     if (a.worlds <= 0 || a.chunks <= 0) return hipSuccess;
     if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
-    range_backward_kernel<<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a);
+    if (a.packet) range_backward_kernel<true><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a);
+    else range_backward_kernel<false><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a);
     return hipGetLastError();
 }
 
