@@ -165,10 +165,26 @@ int32_t ngx_device_id(const ngx_ctx *ctx);
 /* Message of the calling thread's last failed call ("" if none). */
 const char *ngx_last_error(void);
 
-/* Tuning of the trace kernels: tile_w x tile_h pixels per wavefront (64 pixels: 8x8, 16x4, 32x2 or 64x1),
- * whether consecutive workgroups are kept on one XCD, and the walk: 1 the wavefront walks its 64 rays as a
- * packet (the default), 0 each lane walks its own ray. */
-ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap, int32_t packet);
+/* Tuning of the trace kernels (no counterpart in the reference, whose tile is fixed at 8x8 threads):
+ *  tile_w      a wavefront's pixels are a tile_w x (64 / tile_w) tile: 8, 16, 32 or 64;
+ *  xcd_remap   1: consecutive workgroups, which the dispatcher deals round-robin over the eight XCDs, are
+ *              renumbered so each XCD works through one contiguous range of tiles (one L2 per XCD);
+ *  packet      1: the wavefront walks its 64 rays as a packet (the default); 0: each lane walks its own ray;
+ *  pair_limit  packet walk: (ray, triangle) pairs queued before they are tested together, 1..64;
+ *  persistent  1: a fixed grid of 8 workgroups per compute unit takes tiles from per-XCD counters instead
+ *              of one wavefront per tile. */
+typedef struct {
+    int32_t tile_w, xcd_remap, packet, pair_limit, persistent;
+} ngx_trace_config;
+ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config);
+ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);
+
+/* Tries every trace configuration worth trying on this scene and these cameras (each rendered once to warm
+ * up, then `reps` times timed with events on `stream`), keeps the fastest, and returns it with its time per
+ * render. Synchronous; not while `stream` is being captured. The outputs are left as the last render wrote
+ * them. */
+ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
+                        void *stream, ngx_trace_config *best, float *best_ms);
 
 /* ---- build intermediates, for checking the top-level build against the reference algorithm ---- */
 

@@ -41,6 +41,14 @@ class BatchFrame(ctypes.Structure):
                 ("depth", ctypes.c_void_p), ("normal", ctypes.c_void_p), ("object_id", ctypes.c_void_p)]
 
 
+class TraceConfig(ctypes.Structure):
+    _fields_ = [("tile_w", ctypes.c_int32), ("xcd_remap", ctypes.c_int32), ("packet", ctypes.c_int32),
+                ("pair_limit", ctypes.c_int32), ("persistent", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class RayHits(ctypes.Structure):
     _fields_ = [("memory", ctypes.c_int32), ("device_id", ctypes.c_int32), ("count", ctypes.c_int32),
                 ("distance", ctypes.c_void_p), ("normal", ctypes.c_void_p), ("object_id", ctypes.c_void_p)]
@@ -68,7 +76,10 @@ _SIGNATURES = {
     "ngx_instance_count": (ctypes.c_int32, [_VP]),
     "ngx_device_id": (ctypes.c_int32, [_VP]),
     "ngx_last_error": (ctypes.c_char_p, []),
-    "ngx_set_trace_config": (ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "ngx_set_trace_config": (ctypes.c_int32, [_VP, ctypes.POINTER(TraceConfig)]),
+    "ngx_get_trace_config": (ctypes.c_int32, [_VP, ctypes.POINTER(TraceConfig)]),
+    "ngx_autotune": (ctypes.c_int32, [_VP, _VP, ctypes.c_int32, ctypes.c_float, ctypes.c_uint32, ctypes.c_int32, _VP,
+                                      ctypes.POINTER(TraceConfig), ctypes.POINTER(ctypes.c_float)]),
     "ngx_debug_keep_build": (ctypes.c_int32, [_VP, ctypes.c_int32]),
     "ngx_debug_read_build": (ctypes.c_int32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "ngx_build_blas_host": (ctypes.c_int32, [_VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, c_int32_p, _VP, _VP]),

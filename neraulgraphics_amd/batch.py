@@ -116,6 +116,7 @@ class BatchRenderer:
         self._width, self._height = width, height
         self._worlds, self._cpw = world_count, cameras_per_world
         self._per_world = len(instances)
+        self._mesh_signature = ",".join(str(len(m.indices) // 3 if hasattr(m, "indices") else 0) for m in meshes)
         self._bodies = None
         self._device_id = int(device_id)
         self._keep = []
@@ -147,10 +148,67 @@ class BatchRenderer:
     def synchronize(self):
         _lib.check(self._lib.ngx_synchronize(self._h))
 
-    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True):
-        """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, and packet (True, the
-        default) or per-lane (False) walks."""
-        _lib.check(self._lib.ngx_set_trace_config(self._h, int(tile_w), int(bool(xcd_remap)), int(bool(packet))))
+    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=False):
+        """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
+        default) or per-lane (False) walks, (ray, triangle) pairs batched before testing (1..64), and the
+        persistent fixed-grid launch."""
+        c = _lib.TraceConfig(int(tile_w), int(bool(xcd_remap)), int(bool(packet)), int(pair_limit), int(bool(persistent)))
+        _lib.check(self._lib.ngx_set_trace_config(self._h, ctypes.byref(c)))
+
+    @property
+    def trace_config(self):
+        c = _lib.TraceConfig()
+        _lib.check(self._lib.ngx_get_trace_config(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def autotune(self, cameras, fov=1.0, *, depth=True, normals=False, object_ids=True, reps=3, cache=True):
+        """Times every trace configuration worth trying on these cameras and keeps the fastest (see
+        ngx_autotune). With cache, the choice is remembered in a JSON file (NGX_TUNE_CACHE, default
+        ~/.cache/neraulgraphics_amd/autotune.json) under a key made of the device, the frame, the scene's sizes
+        and the outputs, and reused without timing. Returns the configuration with its time per render (ms;
+        None when it came from the cache)."""
+        import json
+        import os
+        outputs = (_lib.OUTPUT_DEPTH if depth else 0) | (_lib.OUTPUT_NORMAL if normals else 0) | \
+                  (_lib.OUTPUT_OBJECT_ID if object_ids else 0)
+        path = os.environ.get("NGX_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "neraulgraphics_amd", "autotune.json"))
+        key = None
+        if cache:
+            try:
+                device = _torch().cuda.get_device_name(self._device_id)
+            except Exception:
+                device = "device"
+            key = "|".join(str(x) for x in (device, self._width, self._height, self._worlds, self._cpw, self._per_world,
+                                            self.instance_count, self._mesh_signature, outputs))
+            try:
+                with open(path) as fh:
+                    known = json.load(fh)
+                if key in known:
+                    self.set_trace_config(**known[key])
+                    return dict(known[key], ms=None)
+            except (OSError, ValueError, TypeError):
+                pass
+        views = self.view_count
+        p, mem, keep = _as_input(cameras, f"cameras ({views} cameras of 3 x 4 numbers, camera-to-world)", views * 12)
+        best, ms = _lib.TraceConfig(), ctypes.c_float()
+        _lib.check(self._lib.ngx_autotune(self._h, p, mem, float(fov), outputs, int(reps), self._stream_for(mem),
+                                          ctypes.byref(best), ctypes.byref(ms)))
+        chosen = best.as_dict()
+        if cache:
+            try:
+                os.makedirs(os.path.dirname(path), exist_ok=True)
+                try:
+                    with open(path) as fh:
+                        known = json.load(fh)
+                except (OSError, ValueError):
+                    known = {}
+                known[key] = chosen
+                with open(path + ".tmp", "w") as fh:
+                    json.dump(known, fh, indent=1, sort_keys=True)
+                os.replace(path + ".tmp", path)
+            except OSError:
+                pass
+        return dict(chosen, ms=float(ms.value))
 
     # ---- poses
     def set_instance_transforms(self, matrices):

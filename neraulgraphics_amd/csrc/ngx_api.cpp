@@ -90,6 +90,9 @@ void release(T *&ptr) {
 struct ngx_ctx {
     int width = 0, height = 0, worlds = 0, cpw = 1, views = 0;
     int per_world = 0, total = 0, max_count = 0, device = 0;
+    int small_max = 0;                // the largest world built by one workgroup
+    std::vector<int> big_worlds;      // worlds built by the device-wide path
+    ngx::BigBuildScratch big{};
     std::vector<int> counts, world_off;
     int sphere_count = 0;
     float floor[4] = {0, 0, 0, 0};
@@ -145,17 +148,21 @@ struct ngx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     hipStream_t last_stream = nullptr;
-    int tile_w = 8, xcd_remap = 1, packet = 1;
+    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 0;
+    int persistent_blocks = 0;          // 8 workgroups of 4 wavefronts per compute unit
+    int *d_tile_counters = nullptr;
 
     ~ngx_ctx() {
-        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_wide_parent};
+        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_wide_parent,
+                       d_tile_counters};
         for (int *p : ints) if (p) (void)hipFree(p);
         float *floats[] = {d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
                            d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
                            d_dbg_world_bounds};
         for (float *p : floats) if (p) (void)hipFree(p);
         void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
-                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys};
+                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, big.key[0], big.key[1], big.val[0], big.val[1],
+                          big.box, big.node_box, big.left, big.right, big.parent, big.arrivals, big.bounds, big.tile_counts};
         for (void *p : others) if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -179,6 +186,7 @@ struct ngx_ctx {
         s.floor_hx = floor[1];
         s.floor_zmin = floor[2];
         s.floor_zmax = floor[3];
+        s.pair_limit = pair_limit;
         return s;
     }
 
@@ -230,6 +238,7 @@ struct ngx_ctx {
         a.total_tiles = (long long)a.tiles_per_view * views;
         a.xcd_remap = xcd_remap;
         a.packet = packet;
+        a.tile_counters = persistent ? d_tile_counters : nullptr;
         return a;
     }
 
@@ -239,12 +248,17 @@ struct ngx_ctx {
         return st != hipStreamCaptureStatusNone;
     }
 
+    void build(const ngx::TlasBuildArgs &a, hipStream_t stream) const {
+        HIP_CHECK(ngx::launch_tlas_build(a, small_max, stream));
+        for (const int w : big_worlds) HIP_CHECK(ngx::launch_tlas_build_big(a, big, w, world_off[w], counts[w], stream));
+    }
+
     void build_from_matrices(const float *matrices_wp12, hipStream_t stream) {
         if (total == 0) return;
         ngx::TlasBuildArgs a = build_args();
         a.from_bodies = 0;
         a.matrices_in = matrices_wp12;
-        HIP_CHECK(ngx::launch_tlas_build(a, max_count, stream));
+        build(a, stream);
         last_body_poses = nullptr;
     }
 
@@ -253,7 +267,7 @@ struct ngx_ctx {
         ngx::TlasBuildArgs a = build_args();
         a.from_bodies = 1;
         a.body_poses = poses;
-        HIP_CHECK(ngx::launch_tlas_build(a, max_count, stream));
+        build(a, stream);
         last_body_poses = poses;
     }
 };
@@ -321,9 +335,6 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             max_count = std::max(max_count, c);
         }
         require(total <= INT32_MAX / 4, "too many instances over all worlds: " + std::to_string(total));
-        if (max_count > ngx::MAX_WORLD_INSTANCES)
-            fail(NGX_UNSUPPORTED, "a world of " + std::to_string(max_count) + " instances: this backend builds worlds of up to " +
-                                      std::to_string(ngx::MAX_WORLD_INSTANCES));
 
         auto ctx = std::make_unique<ngx_ctx>();
         ngx_ctx &c = *ctx;
@@ -339,9 +350,20 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
         for (int w = 0; w < s.worlds; ++w) c.world_off[w + 1] = c.world_off[w] + counts[w];
         c.total = (int)total;
         c.max_count = max_count;
+        for (int w = 0; w < s.worlds; ++w) {
+            if (counts[w] > ngx::WORKGROUP_BUILD_MAX) c.big_worlds.push_back(w);
+            else c.small_max = std::max(c.small_max, counts[w]);
+        }
         c.sphere_count = s.sphere_count;
         std::memcpy(c.floor, fl, sizeof(fl));
         c.set_device();
+        {
+            hipDeviceProp_t prop;
+            HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
+            c.persistent_blocks = 8 * std::max(1, prop.multiProcessorCount);
+            c.persistent_blocks = (c.persistent_blocks + 7) / 8 * 8;
+        }
+        device_alloc(c.d_tile_counters, 8);
         HIP_CHECK(hipEventCreate(&c.ev0));
         HIP_CHECK(hipEventCreate(&c.ev1));
 
@@ -386,6 +408,24 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             device_alloc(c.d_nodes, 2 * (size_t)c.total - 1);
             device_alloc(c.d_wide, std::max<size_t>(1, (size_t)c.total - 1));
             device_alloc(c.d_wide_parent, std::max<size_t>(1, (size_t)c.total - 1));
+            if (!c.big_worlds.empty()) {
+                ngx::BigBuildScratch &b = c.big;
+                b.capacity = 0;
+                for (const int w : c.big_worlds) b.capacity = std::max(b.capacity, counts[w]);
+                const size_t cap = (size_t)b.capacity, tiles = (cap + 1023) / 1024;
+                for (int k = 0; k < 2; ++k) {
+                    device_alloc(b.key[k], cap);
+                    device_alloc(b.val[k], cap);
+                }
+                device_alloc(b.box, 6 * cap);
+                device_alloc(b.node_box, 6 * cap);
+                device_alloc(b.left, cap);
+                device_alloc(b.right, cap);
+                device_alloc(b.parent, 2 * cap);
+                device_alloc(b.arrivals, cap);
+                device_alloc(b.bounds, 6);
+                device_alloc(b.tile_counts, 16 * tiles);
+            }
             // nodes that join worlds are never written by a build: give them a fixed value once
             HIP_CHECK(hipMemset(c.d_nodes, 0, sizeof(ngx::TlasNode) * (2 * (size_t)c.total - 1)));
             HIP_CHECK(hipMemset(c.d_wide, 0, sizeof(ngx::WideNode) * std::max<size_t>(1, (size_t)c.total - 1)));
@@ -549,7 +589,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         a.packet = ctx->packet;
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
-        HIP_CHECK(ngx::launch_render(a, s));
+        HIP_CHECK(ngx::launch_render(a, s, ctx->persistent ? ctx->persistent_blocks : 0));
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
         ctx->timed = timed;
         ctx->last_stream = s;
@@ -731,14 +771,79 @@ int32_t ngx_instances_per_world(const ngx_ctx *ctx) { return ctx ? ctx->per_worl
 int32_t ngx_instance_count(const ngx_ctx *ctx) { return ctx ? ctx->total : 0; }
 int32_t ngx_device_id(const ngx_ctx *ctx) { return ctx ? ctx->device : 0; }
 
-ngx_status ngx_set_trace_config(ngx_ctx *ctx, int32_t tile_w, int32_t xcd_remap, int32_t packet) {
+ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
     return guarded([&] {
-        require(ctx != nullptr, "context is null");
-        require(tile_w == 8 || tile_w == 16 || tile_w == 32 || tile_w == 64, "tile_w must be 8, 16, 32 or 64");
-        ctx->tile_w = tile_w;
-        ctx->xcd_remap = xcd_remap ? 1 : 0;
-        ctx->packet = packet ? 1 : 0;
+        require(ctx != nullptr && config != nullptr, "context and config must not be null");
+        const ngx_trace_config &c = *config;
+        require(c.tile_w == 8 || c.tile_w == 16 || c.tile_w == 32 || c.tile_w == 64, "tile_w must be 8, 16, 32 or 64");
+        require(c.pair_limit >= 1 && c.pair_limit <= 64, "pair_limit must be 1 to 64");
+        ctx->tile_w = c.tile_w;
+        ctx->xcd_remap = c.xcd_remap ? 1 : 0;
+        ctx->packet = c.packet ? 1 : 0;
+        ctx->pair_limit = c.pair_limit;
+        ctx->persistent = c.persistent ? 1 : 0;
     });
+}
+
+ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config) {
+    return guarded([&] {
+        require(ctx != nullptr && config != nullptr, "context and config must not be null");
+        *config = ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent};
+    });
+}
+
+ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
+                        void *stream, ngx_trace_config *best, float *best_ms) {
+    const ngx_trace_config start = ctx ? ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent}
+                                       : ngx_trace_config{};
+    const ngx_status st = guarded([&] {
+        require(ctx != nullptr && cams12 != nullptr, "context and cameras must not be null");
+        require(reps >= 1 && reps <= 100, "reps must be 1 to 100");
+        require(!ctx->capturing(static_cast<hipStream_t>(stream)), "autotune cannot run while the stream is being captured");
+        std::vector<ngx_trace_config> cands;
+        for (const int tw : {8, 16, 32}) {
+            for (const int limit : {32, 64})
+                for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers});
+            if (tw <= 16) cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0});
+        }
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ctx->set_device();
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        float best_time = 0.0f;
+        ngx_trace_config winner = start;
+        ngx_status failure = NGX_OK;
+        for (const ngx_trace_config &c : cands) {
+            ngx_set_trace_config(ctx, &c);
+            ngx_batch_frame f;
+            failure = ngx_render_batch(ctx, cams12, memory, fov, 1, outputs, 0, stream, &f);  // warm-up
+            if (failure != NGX_OK) break;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps && failure == NGX_OK; ++r) failure = ngx_render_batch(ctx, cams12, memory, fov, 1, outputs, 0, stream, &f);
+            if (failure != NGX_OK) break;
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)reps;
+            if (best_time == 0.0f || ms < best_time) {
+                best_time = ms;
+                winner = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (failure != NGX_OK) {
+            const std::string message = g_last_error;
+            ngx_set_trace_config(ctx, &start);
+            fail(failure, "autotune: " + message);
+        }
+        ngx_set_trace_config(ctx, &winner);
+        if (best) *best = winner;
+        if (best_ms) *best_ms = best_time;
+    });
+    return st;
 }
 
 ngx_status ngx_debug_keep_build(ngx_ctx *ctx, int32_t keep) {

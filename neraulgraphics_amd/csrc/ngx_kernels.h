@@ -56,6 +56,7 @@ struct TraceScene {
     int sphere_count;
     const float4 *spheres;     // centre xyz, radius
     float floor_y, floor_hx, floor_zmin, floor_zmax;
+    int pair_limit;            // packet walk: (ray, triangle) pairs queued before they are tested, 1..64
 };
 
 struct RenderArgs {
@@ -71,6 +72,7 @@ struct RenderArgs {
     int32_t *object_id;        // [views][H][W] or null
     int xcd_remap;             // 1: consecutive tiles stay on one XCD
     int packet;                // 1: wavefront-packet walk; 0: per-lane walk
+    int *tile_counters;        // persistent mode (non-null): 8 counters, zeroed before the launch, one per XCD
 };
 
 struct RayArgs {
@@ -127,8 +129,28 @@ struct PoseGradFinishArgs {
     float *grad;               // [worlds][bodies][7]
 };
 
+// Builds every world of at most WORKGROUP_BUILD_MAX instances (max_count: the largest such world), one
+// workgroup per world; larger worlds are skipped.
 hipError_t launch_tlas_build(const TlasBuildArgs &a, int max_count, hipStream_t stream);
-hipError_t launch_render(const RenderArgs &a, hipStream_t stream);
+
+// Scratch of the device-wide build of one large world, sized for `capacity` instances.
+struct BigBuildScratch {
+    int capacity;
+    uint32_t *key[2], *val[2]; // [capacity] each
+    float *box;                // [6][capacity]: instance boxes by template index, lo xyz then hi xyz
+    float *node_box;           // [6][capacity]: internal node boxes by local index
+    int *left, *right;         // [capacity]
+    int *parent;               // [2 * capacity]
+    int *arrivals;             // [capacity]
+    int *bounds;               // [6]: centre bounds as ordered ints
+    int *tile_counts;          // [16 * ceil(capacity / 1024)]
+};
+
+// Builds world `world` (instances first .. first+n-1, n > 1) with grid-wide kernels.
+hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &s, int world, int first, int n, hipStream_t stream);
+// Renders every view; with a.tile_counters set, a fixed grid of `persistent_blocks` workgroups takes tiles
+// from per-XCD counters until none are left (the counters are zeroed by this call first).
+hipError_t launch_render(const RenderArgs &a, hipStream_t stream, int persistent_blocks = 0);
 hipError_t launch_rays(const RayArgs &a, hipStream_t stream);
 hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream);
 hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream);
@@ -139,6 +161,6 @@ hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream
 #endif
 
 // The largest world the one-workgroup-per-world build takes.
-constexpr int MAX_WORLD_INSTANCES = 1024;
+constexpr int WORKGROUP_BUILD_MAX = 1024;
 
 }  // namespace ngx

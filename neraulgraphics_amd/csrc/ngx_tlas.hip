@@ -119,6 +119,132 @@ __device__ void block_exclusive_scan(int *data, const int count, int *wave_tot) 
     __syncthreads();
 }
 
+// ---- pieces shared by the workgroup build and the device-wide build
+
+// Places instance i of world w (global index g): its matrix as given or composed from its body, the
+// inverse the tracer reads, and its world box (the 8 corners of the mesh box, cuda_tlas.cuh:69-96).
+__device__ void place_instance(const TlasBuildArgs &A, const int w, const int i, const int g, float lo[3], float hi[3]) {
+    float m[12];
+    if (A.from_bodies) {
+        const int b = A.body_of_instance[i];
+        const float *body = b < 0 ? nullptr : A.body_poses + 7 * ((size_t)w * A.bodies_per_world + b);
+        compose_body_pose(body, A.local_pose + 7 * i, A.scale + 3 * i, m);
+    } else {
+        const float4 *mi = reinterpret_cast<const float4 *>(A.matrices_in + 12 * ((size_t)w * A.matrices_stride + i));
+        const float4 r0 = mi[0], r1 = mi[1], r2 = mi[2];
+        m[0] = r0.x; m[1] = r0.y; m[2] = r0.z; m[3] = r0.w;
+        m[4] = r1.x; m[5] = r1.y; m[6] = r1.z; m[7] = r1.w;
+        m[8] = r2.x; m[9] = r2.y; m[10] = r2.z; m[11] = r2.w;
+    }
+    float4 *mo = reinterpret_cast<float4 *>(A.matrices_out + 12 * (size_t)g);
+    mo[0] = make_float4(m[0], m[1], m[2], m[3]);
+    mo[1] = make_float4(m[4], m[5], m[6], m[7]);
+    mo[2] = make_float4(m[8], m[9], m[10], m[11]);
+    float inv[12];
+    invert_affine(m, inv);
+    const int mesh = A.tmpl_mesh[i];
+    InstanceRec rec;
+    rec.inv[0] = make_float4(inv[0], inv[1], inv[2], inv[3]);
+    rec.inv[1] = make_float4(inv[4], inv[5], inv[6], inv[7]);
+    rec.inv[2] = make_float4(inv[8], inv[9], inv[10], inv[11]);
+    rec.mesh = mesh;
+    rec.tmpl = i;
+    rec.pad0 = rec.pad1 = 0;
+    A.inst[g] = rec;
+
+    const float *lb = A.mesh_bounds + 6 * mesh;
+    for (int ax = 0; ax < 3; ++ax) { lo[ax] = 1e30f; hi[ax] = -1e30f; }
+    for (int corner = 0; corner < 8; ++corner) {
+        const float p[3] = {(corner & 1) ? lb[3] : lb[0], (corner & 2) ? lb[4] : lb[1], (corner & 4) ? lb[5] : lb[2]};
+        for (int ax = 0; ax < 3; ++ax) {
+            const float v = m[4 * ax] * p[0] + m[4 * ax + 1] * p[1] + m[4 * ax + 2] * p[2] + m[4 * ax + 3];
+            lo[ax] = fminf(lo[ax], v);
+            hi[ax] = fmaxf(hi[ax], v);
+        }
+    }
+    if (A.dbg_aabb) {
+        float *o = A.dbg_aabb + 6 * (size_t)g;
+        for (int ax = 0; ax < 3; ++ax) { o[ax] = lo[ax]; o[3 + ax] = hi[ax]; }
+    }
+}
+
+// 30-bit Morton code of a box centre within the world's centre bounds (ordered ints), cuda_tlas.cuh:99-115.
+__device__ __forceinline__ uint32_t morton_code(const float lo[3], const float hi[3], const int *world_bounds) {
+    uint32_t q[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        const float blo = from_ordered(world_bounds[ax]), bhi = from_ordered(world_bounds[3 + ax]);
+        const float centre = 0.5f * (lo[ax] + hi[ax]);
+        const float extent = bhi - blo;
+        const float t = extent > 1e-20f ? (centre - blo) / extent : 0.0f;
+        q[ax] = (uint32_t)(fminf(fmaxf(t, 0.0f), 1.0f) * 1023.0f);
+    }
+    return (spread_bits(q[0]) << 2) | (spread_bits(q[1]) << 1) | spread_bits(q[2]);
+}
+
+// Karras 2012 for internal node i of a world's n sorted keys at global positions a..a+n-1 (cuda_tlas.cuh:
+// 119-159): equal keys are told apart by their global positions, keys of other worlds are out of range.
+// Children: local node index, or n-1+k for the k-th leaf.
+template <class Keys>
+__device__ __forceinline__ void karras_node(const int i, const int n, const int a, const Keys &key, int &l, int &r) {
+    auto delta = [&](const int p, const int q) -> int {
+        if (q < 0 || q >= n) return -1;
+        const uint32_t kp = key[p], kq = key[q];
+        if (kp == kq) return 64 + __clz((a + p) ^ (a + q));
+        return 32 + __clz(kp ^ kq);
+    };
+    const int d = delta(i, i + 1) > delta(i, i - 1) ? 1 : -1;
+    const int min_prefix = delta(i, i - d);
+    int max_length = 2;
+    while (delta(i, i + max_length * d) > min_prefix) max_length *= 2;
+    int length = 0;
+    for (int t = max_length / 2; t >= 1; t /= 2)
+        if (delta(i, i + (length + t) * d) > min_prefix) length += t;
+    const int j = i + length * d;
+    const int node_prefix = delta(i, j);
+    int s = 0;
+    for (int t = (length + 1) / 2;; t = (t + 1) / 2) {
+        if (delta(i, i + (s + t) * d) > node_prefix) s += t;
+        if (t == 1) break;
+    }
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    const int lo = min(i, j), hi = max(i, j);
+    l = (lo == gamma) ? n - 1 + gamma : gamma;
+    r = (hi == gamma + 1) ? n - 1 + gamma + 1 : gamma + 1;
+}
+
+// Writes internal node i of world w (runs at a, n instances) in both layouts: box(c, ax, &lo, &hi) gives
+// local child c's box, id(k) the template index of the k-th sorted leaf, parent its local parent (-1 at the
+// world's root).
+template <class BoxOf, class IdOf>
+__device__ __forceinline__ void write_interior(const TlasBuildArgs &A, const int i, const int n, const int a, const int root_index,
+                                               const int l, const int r, const int parent, const BoxOf &box, const IdOf &id) {
+    const int N = A.total;
+    const int gi = i == 0 ? root_index : a + i;
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        box(l, ax, llo[ax], lhi[ax]);
+        box(r, ax, rlo[ax], rhi[ax]);
+    }
+    TlasNode node;
+    for (int ax = 0; ax < 3; ++ax) {
+        node.lo[ax] = fminf(llo[ax], rlo[ax]);
+        node.hi[ax] = fmaxf(lhi[ax], rhi[ax]);
+    }
+    node.left = l >= n - 1 ? N - 1 + a + (l - (n - 1)) : a + l;
+    node.right = r >= n - 1 ? N - 1 + a + (r - (n - 1)) : a + r;
+    A.nodes[gi] = node;
+    WideNode wn;
+    for (int ax = 0; ax < 3; ++ax) {
+        wn.l_lo[ax] = llo[ax]; wn.l_hi[ax] = lhi[ax];
+        wn.r_lo[ax] = rlo[ax]; wn.r_hi[ax] = rhi[ax];
+    }
+    wn.left = l >= n - 1 ? ~(a + id(l - (n - 1))) : a + l;
+    wn.right = r >= n - 1 ? ~(a + id(r - (n - 1))) : a + r;
+    wn.left_count = wn.right_count = 1;
+    A.wide[gi] = wn;
+    A.wide_parent[gi] = parent < 0 ? -1 : (parent == 0 ? root_index : a + parent);
+}
+
 template <int MAXN>
 struct BuildShared {
     static constexpr int NW = MAXN / 64;
@@ -147,61 +273,21 @@ __global__ __launch_bounds__(MAXN) void tlas_build_kernel(const TlasBuildArgs A)
         if (tid == 0) A.roots[w] = -1;
         return;
     }
+    if (n > MAXN) return;  // built by the device-wide path
     const bool has = tid < n;
 
     // ---- place: matrix, inverse, box, vote on the centre bounds
     int cen[3] = {to_ordered(1e30f), to_ordered(1e30f), to_ordered(1e30f)};
     int cmax[3] = {to_ordered(-1e30f), to_ordered(-1e30f), to_ordered(-1e30f)};
     if (has) {
-        const int g = a + tid;
-        float m[12];
-        if (A.from_bodies) {
-            const int b = A.body_of_instance[tid];
-            const float *body = b < 0 ? nullptr : A.body_poses + 7 * ((size_t)w * A.bodies_per_world + b);
-            compose_body_pose(body, A.local_pose + 7 * tid, A.scale + 3 * tid, m);
-        } else {
-            const float4 *mi = reinterpret_cast<const float4 *>(A.matrices_in + 12 * ((size_t)w * A.matrices_stride + tid));
-            const float4 r0 = mi[0], r1 = mi[1], r2 = mi[2];
-            m[0] = r0.x; m[1] = r0.y; m[2] = r0.z; m[3] = r0.w;
-            m[4] = r1.x; m[5] = r1.y; m[6] = r1.z; m[7] = r1.w;
-            m[8] = r2.x; m[9] = r2.y; m[10] = r2.z; m[11] = r2.w;
-        }
-        float4 *mo = reinterpret_cast<float4 *>(A.matrices_out + 12 * (size_t)g);
-        mo[0] = make_float4(m[0], m[1], m[2], m[3]);
-        mo[1] = make_float4(m[4], m[5], m[6], m[7]);
-        mo[2] = make_float4(m[8], m[9], m[10], m[11]);
-        float inv[12];
-        invert_affine(m, inv);
-        const int mesh = A.tmpl_mesh[tid];
-        InstanceRec rec;
-        rec.inv[0] = make_float4(inv[0], inv[1], inv[2], inv[3]);
-        rec.inv[1] = make_float4(inv[4], inv[5], inv[6], inv[7]);
-        rec.inv[2] = make_float4(inv[8], inv[9], inv[10], inv[11]);
-        rec.mesh = mesh;
-        rec.tmpl = tid;
-        rec.pad0 = rec.pad1 = 0;
-        A.inst[g] = rec;
-
-        const float *lb = A.mesh_bounds + 6 * mesh;
-        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-        for (int corner = 0; corner < 8; ++corner) {
-            const float p[3] = {(corner & 1) ? lb[3] : lb[0], (corner & 2) ? lb[4] : lb[1], (corner & 4) ? lb[5] : lb[2]};
-            for (int ax = 0; ax < 3; ++ax) {
-                const float v = m[4 * ax] * p[0] + m[4 * ax + 1] * p[1] + m[4 * ax + 2] * p[2] + m[4 * ax + 3];
-                lo[ax] = fminf(lo[ax], v);
-                hi[ax] = fmaxf(hi[ax], v);
-            }
-        }
+        float lo[3], hi[3];
+        place_instance(A, w, tid, a + tid, lo, hi);
         for (int ax = 0; ax < 3; ++ax) {
             sh.box_lo[ax][tid] = lo[ax];
             sh.box_hi[ax][tid] = hi[ax];
             const int c = to_ordered(0.5f * (lo[ax] + hi[ax]));
             cen[ax] = c;
             cmax[ax] = c;
-        }
-        if (A.dbg_aabb) {
-            float *o = A.dbg_aabb + 6 * (size_t)g;
-            for (int ax = 0; ax < 3; ++ax) { o[ax] = lo[ax]; o[3 + ax] = hi[ax]; }
         }
     }
     for (int ax = 0; ax < 3; ++ax) {
@@ -219,15 +305,9 @@ __global__ __launch_bounds__(MAXN) void tlas_build_kernel(const TlasBuildArgs A)
 
     // ---- keys
     if (has) {
-        uint32_t q[3];
-        for (int ax = 0; ax < 3; ++ax) {
-            const float lo = from_ordered(sh.world_bounds[ax]), hi = from_ordered(sh.world_bounds[3 + ax]);
-            const float centre = 0.5f * (sh.box_lo[ax][tid] + sh.box_hi[ax][tid]);
-            const float extent = hi - lo;
-            const float t = extent > 1e-20f ? (centre - lo) / extent : 0.0f;
-            q[ax] = (uint32_t)(fminf(fmaxf(t, 0.0f), 1.0f) * 1023.0f);
-        }
-        const uint32_t morton = (spread_bits(q[0]) << 2) | (spread_bits(q[1]) << 1) | spread_bits(q[2]);
+        const float lo[3] = {sh.box_lo[0][tid], sh.box_lo[1][tid], sh.box_lo[2][tid]};
+        const float hi[3] = {sh.box_hi[0][tid], sh.box_hi[1][tid], sh.box_hi[2][tid]};
+        const uint32_t morton = morton_code(lo, hi, sh.world_bounds);
         sh.key[0][tid] = morton;
         sh.val[0][tid] = (uint32_t)tid;
         if (A.dbg_keys) A.dbg_keys[a + tid] = ((uint64_t)w << 32) | morton;
@@ -285,35 +365,12 @@ __global__ __launch_bounds__(MAXN) void tlas_build_kernel(const TlasBuildArgs A)
     if (tid < n - 1) sh.arrivals[tid] = 0;
     __syncthreads();
     if (tid < n - 1) {
-        const int i = tid;
-        auto delta = [&](const int p, const int q) -> int {
-            if (q < 0 || q >= n) return -1;
-            const uint32_t kp = key[p], kq = key[q];
-            if (kp == kq) return 64 + __clz((a + p) ^ (a + q));
-            return 32 + __clz(kp ^ kq);
-        };
-        const int d = delta(i, i + 1) > delta(i, i - 1) ? 1 : -1;
-        const int min_prefix = delta(i, i - d);
-        int max_length = 2;
-        while (delta(i, i + max_length * d) > min_prefix) max_length *= 2;
-        int length = 0;
-        for (int t = max_length / 2; t >= 1; t /= 2)
-            if (delta(i, i + (length + t) * d) > min_prefix) length += t;
-        const int j = i + length * d;
-        const int node_prefix = delta(i, j);
-        int s = 0;
-        for (int t = (length + 1) / 2;; t = (t + 1) / 2) {
-            if (delta(i, i + (s + t) * d) > node_prefix) s += t;
-            if (t == 1) break;
-        }
-        const int gamma = i + s * d + (d < 0 ? d : 0);
-        const int lo = min(i, j), hi = max(i, j);
-        const int l = (lo == gamma) ? n - 1 + gamma : gamma;
-        const int r = (hi == gamma + 1) ? n - 1 + gamma + 1 : gamma + 1;
-        sh.left[i] = l;
-        sh.right[i] = r;
-        sh.parent[l] = i;
-        sh.parent[r] = i;
+        int l, r;
+        karras_node(tid, n, a, key, l, r);
+        sh.left[tid] = l;
+        sh.right[tid] = r;
+        sh.parent[l] = tid;
+        sh.parent[r] = tid;
     }
     __syncthreads();
 
@@ -343,28 +400,242 @@ __global__ __launch_bounds__(MAXN) void tlas_build_kernel(const TlasBuildArgs A)
     const int root_index = A.root_at_end[w] ? a + n - 1 : a;
     if (tid == 0) A.roots[w] = root_index;
     if (tid < n - 1) {
-        const int i = tid;
-        const int gi = i == 0 ? root_index : a + i;
-        const int l = sh.left[i], r = sh.right[i];
-        TlasNode node;
-        for (int ax = 0; ax < 3; ++ax) { node.lo[ax] = sh.node_lo[ax][i]; node.hi[ax] = sh.node_hi[ax][i]; }
-        node.left = l >= n - 1 ? N - 1 + a + (l - (n - 1)) : a + l;
-        node.right = r >= n - 1 ? N - 1 + a + (r - (n - 1)) : a + r;
-        A.nodes[gi] = node;
-        WideNode wn;
-        for (int ax = 0; ax < 3; ++ax) {
-            wn.l_lo[ax] = l >= n - 1 ? sh.box_lo[ax][val[l - (n - 1)]] : sh.node_lo[ax][l];
-            wn.l_hi[ax] = l >= n - 1 ? sh.box_hi[ax][val[l - (n - 1)]] : sh.node_hi[ax][l];
-            wn.r_lo[ax] = r >= n - 1 ? sh.box_lo[ax][val[r - (n - 1)]] : sh.node_lo[ax][r];
-            wn.r_hi[ax] = r >= n - 1 ? sh.box_hi[ax][val[r - (n - 1)]] : sh.node_hi[ax][r];
-        }
-        wn.left = l >= n - 1 ? ~(a + (int)val[l - (n - 1)]) : a + l;
-        wn.right = r >= n - 1 ? ~(a + (int)val[r - (n - 1)]) : a + r;
-        wn.left_count = wn.right_count = 1;
-        A.wide[gi] = wn;
-        const int pl = sh.parent[i];
-        A.wide_parent[gi] = pl < 0 ? -1 : (pl == 0 ? root_index : a + pl);
+        auto box = [&](const int c, const int ax, float &lo, float &hi) {
+            lo = c >= n - 1 ? sh.box_lo[ax][val[c - (n - 1)]] : sh.node_lo[ax][c];
+            hi = c >= n - 1 ? sh.box_hi[ax][val[c - (n - 1)]] : sh.node_hi[ax][c];
+        };
+        auto id = [&](const int k) { return (int)val[k]; };
+        write_interior(A, tid, n, a, root_index, sh.left[tid], sh.right[tid], sh.parent[tid], box, id);
     }
+}
+
+// ------------------------------------------------------------------ device-wide build of a large world --
+//
+// A world with more instances than one workgroup sorts in LDS is built by a sequence of grid-wide kernels
+// with the same arithmetic: place (and the centre bounds by ordered-int atomics, cuda_tlas.cuh:69-96),
+// keys, a stable LSD radix sort of 4-bit digits over tiles of 1024 keys (per-tile digit counts, one scan
+// of the counts in digit-major order, a stable scatter that ranks each tile's keys with the same wave
+// ballots as the workgroup sort), Karras per node, the bottom-up union with arrival counters in global
+// memory, and the same node writes.
+
+constexpr int BIG_TILE = 1024;
+
+__global__ __launch_bounds__(256) void big_init_kernel(const BigBuildScratch S, const int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < 2 * n - 1) S.parent[i] = -1;
+    if (i < n - 1) S.arrivals[i] = 0;
+    if (i < 3) {
+        S.bounds[i] = to_ordered(1e30f);
+        S.bounds[3 + i] = to_ordered(-1e30f);
+    }
+}
+
+__global__ __launch_bounds__(256) void big_place_kernel(const TlasBuildArgs A, const BigBuildScratch S, const int w, const int a,
+                                                        const int n) {
+    __shared__ int part[6][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int cmin[3] = {to_ordered(1e30f), to_ordered(1e30f), to_ordered(1e30f)};
+    int cmax[3] = {to_ordered(-1e30f), to_ordered(-1e30f), to_ordered(-1e30f)};
+    if (i < n) {
+        float lo[3], hi[3];
+        place_instance(A, w, i, a + i, lo, hi);
+        for (int ax = 0; ax < 3; ++ax) {
+            S.box[(size_t)ax * S.capacity + i] = lo[ax];
+            S.box[(size_t)(3 + ax) * S.capacity + i] = hi[ax];
+            cmin[ax] = cmax[ax] = to_ordered(0.5f * (lo[ax] + hi[ax]));
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        const int mn = wave_min(cmin[ax]), mx = wave_max(cmax[ax]);
+        if (lane == 0) {
+            part[ax][wave] = mn;
+            part[3 + ax][wave] = mx;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int t = threadIdx.x;
+        int v = part[t][0];
+        for (int k = 1; k < 4; ++k) v = t < 3 ? min(v, part[t][k]) : max(v, part[t][k]);
+        if (t < 3) atomicMin(&S.bounds[t], v);
+        else atomicMax(&S.bounds[t], v);
+    }
+}
+
+__global__ __launch_bounds__(256) void big_keys_kernel(const TlasBuildArgs A, const BigBuildScratch S, const int w, const int a,
+                                                       const int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < 6 && A.dbg_world_bounds) A.dbg_world_bounds[6 * w + i] = from_ordered(S.bounds[i]);
+    if (i >= n) return;
+    float lo[3], hi[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        lo[ax] = S.box[(size_t)ax * S.capacity + i];
+        hi[ax] = S.box[(size_t)(3 + ax) * S.capacity + i];
+    }
+    const uint32_t morton = morton_code(lo, hi, S.bounds);
+    S.key[0][i] = morton;
+    S.val[0][i] = (uint32_t)i;
+    if (A.dbg_keys) A.dbg_keys[a + i] = ((uint64_t)w << 32) | morton;
+}
+
+// The digit of each key of a 1024-key tile, its rank among the equal digits of its wave, and how many
+// of its wave share the digit (valid for the rank-0 lane).
+__device__ __forceinline__ void tile_digit(const bool has, const uint32_t k, const int shift, uint32_t &digit, int &rank, int &count) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    digit = (k >> shift) & 15u;
+    uint64_t peers = ballot(has);
+    for (int bit = 0; bit < 4; ++bit) {
+        const bool set = (digit >> bit) & 1u;
+        const uint64_t vote = ballot(has && set);
+        peers &= set ? vote : ~vote;
+    }
+    rank = popc64(peers & below);
+    count = popc64(peers);
+}
+
+__global__ __launch_bounds__(BIG_TILE) void big_sort_count_kernel(const BigBuildScratch S, const int n, const int src, const int shift,
+                                                                  const int tiles) {
+    constexpr int NW = BIG_TILE / 64;
+    __shared__ int hist[16 * NW];
+    const int t = blockIdx.x, tid = threadIdx.x, wave = tid >> 6;
+    const int i = t * BIG_TILE + tid;
+    const bool has = i < n;
+    if (tid < 16 * NW) hist[tid] = 0;
+    __syncthreads();
+    uint32_t digit;
+    int rank, count;
+    tile_digit(has, has ? S.key[src][i] : 0u, shift, digit, rank, count);
+    if (has && rank == 0) hist[digit * NW + wave] = count;
+    __syncthreads();
+    if (tid < 16) {
+        int sum = 0;
+        for (int wv = 0; wv < NW; ++wv) sum += hist[tid * NW + wv];
+        S.tile_counts[(size_t)tid * tiles + t] = sum;
+    }
+}
+
+// Exclusive scan of tile_counts[0 .. total) in place by one workgroup, 1024 entries at a time.
+__global__ __launch_bounds__(1024) void big_sort_scan_kernel(const BigBuildScratch S, const int total) {
+    __shared__ int wave_tot[16];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < total; base += 1024) {
+        const int c = carry;
+        const int idx = base + tid;
+        const int x = idx < total ? S.tile_counts[idx] : 0;
+        int incl = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wave_tot[wave] = incl;
+        __syncthreads();
+        for (int wv = 0; wv < wave; ++wv) incl += wave_tot[wv];
+        __syncthreads();
+        if (idx < total) S.tile_counts[idx] = c + incl - x;
+        if (tid == 1023) carry = c + incl;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BIG_TILE) void big_sort_scatter_kernel(const BigBuildScratch S, const int n, const int src, const int shift,
+                                                                    const int tiles) {
+    constexpr int NW = BIG_TILE / 64;
+    __shared__ int hist[16 * NW];
+    __shared__ int wave_tot[NW];
+    const int t = blockIdx.x, tid = threadIdx.x, wave = tid >> 6;
+    const int i = t * BIG_TILE + tid;
+    const bool has = i < n;
+    const uint32_t k = has ? S.key[src][i] : 0u, v = has ? S.val[src][i] : 0u;
+    if (tid < 16 * NW) hist[tid] = 0;
+    __syncthreads();
+    uint32_t digit;
+    int rank, count;
+    tile_digit(has, k, shift, digit, rank, count);
+    if (has && rank == 0) hist[digit * NW + wave] = count;
+    __syncthreads();
+    block_exclusive_scan(hist, 16 * NW, wave_tot);
+    if (has) {
+        const int local = hist[digit * NW + wave] - hist[digit * NW] + rank;
+        const int dest = S.tile_counts[(size_t)digit * tiles + t] + local;
+        S.key[src ^ 1][dest] = k;
+        S.val[src ^ 1][dest] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void big_karras_kernel(const TlasBuildArgs A, const BigBuildScratch S, const int w, const int a,
+                                                         const int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t *key = S.key[0];
+    const int id = (int)S.val[0][i];
+    if (A.dbg_sorted_keys) A.dbg_sorted_keys[a + i] = ((uint64_t)w << 32) | key[i];
+    if (A.dbg_sorted_ids) A.dbg_sorted_ids[a + i] = a + id;
+    TlasNode leaf;
+    for (int ax = 0; ax < 3; ++ax) {
+        leaf.lo[ax] = S.box[(size_t)ax * S.capacity + id];
+        leaf.hi[ax] = S.box[(size_t)(3 + ax) * S.capacity + id];
+    }
+    leaf.left = ~(a + id);
+    leaf.right = -1;
+    A.nodes[(size_t)(A.total - 1) + a + i] = leaf;
+    if (i < n - 1) {
+        int l, r;
+        karras_node(i, n, a, key, l, r);
+        S.left[i] = l;
+        S.right[i] = r;
+        S.parent[l] = i;
+        S.parent[r] = i;
+    }
+}
+
+__device__ __forceinline__ void big_child_box(const BigBuildScratch &S, const int n, const int c, const int ax, float &lo, float &hi) {
+    if (c >= n - 1) {
+        const int id = (int)S.val[0][c - (n - 1)];
+        lo = S.box[(size_t)ax * S.capacity + id];
+        hi = S.box[(size_t)(3 + ax) * S.capacity + id];
+    } else {
+        lo = S.node_box[(size_t)ax * S.capacity + c];
+        hi = S.node_box[(size_t)(3 + ax) * S.capacity + c];
+    }
+}
+
+// Bottom up: the second of a node's two children to arrive computes its box. The arrival counter's
+// acquire-release at device scope orders the first child's box before the second child reads it.
+__global__ __launch_bounds__(256) void big_refit_kernel(const BigBuildScratch S, const int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int node = n - 1 + i;
+    while (true) {
+        const int p = S.parent[node];
+        if (p < 0) break;
+        const int prior = __hip_atomic_fetch_add(&S.arrivals[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0) break;
+        const int l = S.left[p], r = S.right[p];
+        for (int ax = 0; ax < 3; ++ax) {
+            float llo, lhi, rlo, rhi;
+            big_child_box(S, n, l, ax, llo, lhi);
+            big_child_box(S, n, r, ax, rlo, rhi);
+            S.node_box[(size_t)ax * S.capacity + p] = fminf(llo, rlo);
+            S.node_box[(size_t)(3 + ax) * S.capacity + p] = fmaxf(lhi, rhi);
+        }
+        node = p;
+    }
+}
+
+__global__ __launch_bounds__(256) void big_write_kernel(const TlasBuildArgs A, const BigBuildScratch S, const int w, const int a,
+                                                        const int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int root_index = A.root_at_end[w] ? a + n - 1 : a;
+    if (i == 0) A.roots[w] = root_index;
+    if (i >= n - 1) return;
+    auto box = [&](const int c, const int ax, float &lo, float &hi) { big_child_box(S, n, c, ax, lo, hi); };
+    auto id = [&](const int k) { return (int)S.val[0][k]; };
+    write_interior(A, i, n, a, root_index, S.left[i], S.right[i], S.parent[i], box, id);
 }
 
 }  // namespace
@@ -375,8 +646,27 @@ hipError_t launch_tlas_build(const TlasBuildArgs &a, const int max_count, hipStr
     else if (max_count <= 128) tlas_build_kernel<128><<<a.worlds, 128, 0, stream>>>(a);
     else if (max_count <= 256) tlas_build_kernel<256><<<a.worlds, 256, 0, stream>>>(a);
     else if (max_count <= 512) tlas_build_kernel<512><<<a.worlds, 512, 0, stream>>>(a);
-    else if (max_count <= MAX_WORLD_INSTANCES) tlas_build_kernel<1024><<<a.worlds, 1024, 0, stream>>>(a);
-    else return hipErrorInvalidValue;
+    else tlas_build_kernel<1024><<<a.worlds, 1024, 0, stream>>>(a);  // larger worlds return at once
+    return hipGetLastError();
+}
+
+hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &s, const int world, const int first, const int n,
+                                 hipStream_t stream) {
+    if (n <= 1 || n > s.capacity) return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)((n + 255) / 256), blocks2 = (unsigned)((2 * n - 1 + 255) / 256);
+    const int tiles = (n + BIG_TILE - 1) / BIG_TILE;
+    big_init_kernel<<<blocks2, 256, 0, stream>>>(s, n);
+    big_place_kernel<<<blocks, 256, 0, stream>>>(a, s, world, first, n);
+    big_keys_kernel<<<blocks, 256, 0, stream>>>(a, s, world, first, n);
+    for (int pass = 0; pass < 8; ++pass) {
+        const int src = pass & 1;
+        big_sort_count_kernel<<<tiles, BIG_TILE, 0, stream>>>(s, n, src, 4 * pass, tiles);
+        big_sort_scan_kernel<<<1, 1024, 0, stream>>>(s, 16 * tiles);
+        big_sort_scatter_kernel<<<tiles, BIG_TILE, 0, stream>>>(s, n, src, 4 * pass, tiles);
+    }
+    big_karras_kernel<<<blocks, 256, 0, stream>>>(a, s, world, first, n);
+    big_refit_kernel<<<blocks, 256, 0, stream>>>(s, n);
+    big_write_kernel<<<blocks, 256, 0, stream>>>(a, s, world, first, n);
     return hipGetLastError();
 }
 

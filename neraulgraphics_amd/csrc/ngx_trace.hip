@@ -173,7 +173,7 @@ __device__ __forceinline__ void add_leaf(const TraceScene &S, PairLds *L, int &p
     const int fit = 64 / m;
     for (int done = 0; done < count;) {
         const int c = min(count - done, fit);
-        if (pairs + m * c > 64) flush_pairs(S, L, pairs, inst, lo, ld, tmax, h, stats);
+        if (pairs + m * c > S.pair_limit) flush_pairs(S, L, pairs, inst, lo, ld, tmax, h, stats);
         if (in) {
             const int base = pairs + rank * c;
             for (int k = 0; k < c; ++k) L->pair[base + k] = lane | ((first + done + k) << 6);
@@ -282,9 +282,9 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
     }
     const float3 inv_d = safe_inverse(d);
     const int lane = lane_id();
-    // One entry is pushed per level. A world's tree holds at most 1024 instances, and below its root every
-    // level fixes at least one more bit of a Morton code (30 bits) or, among equal codes, of the leaves'
-    // positions (at most 11 bits over 1024 consecutive positions): at most 42 levels, so 64 entries do.
+    // One entry is pushed per level at most. Below a world's root every level fixes at least one more bit
+    // of a Morton code (30 bits) or, among equal codes, of the leaves' positions (at most ceil(log2 n) + 1
+    // bits over n consecutive positions): fewer than 64 levels for any n < 2^31, so 64 entries do.
     WaveStack<1> st;
     st.init();
     int cur = root;
@@ -622,11 +622,9 @@ __device__ __forceinline__ long long wave_tile(const int xcd_remap, const long l
 // LDS of the packet walk's triangle batches: one PairLds per wavefront of a 256-thread workgroup.
 #define NGX_PAIR_LDS(name) __shared__ PairLds name##_all[4]; PairLds *name = PACKET ? &name##_all[threadIdx.x >> 6] : nullptr
 
+// One 64-pixel tile of one view: camera ray per lane, closest hit, depth / normal / object ID.
 template <bool PACKET>
-__global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_kernel(const RenderArgs A) {
-    NGX_PAIR_LDS(L);
-    const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
-    if (tile >= A.total_tiles) return;
+__device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, const long long tile) {
     const int lane = lane_id();
     const int view = (int)(tile / A.tiles_per_view);
     const int t = (int)(tile % A.tiles_per_view);
@@ -654,6 +652,37 @@ __global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_kernel(const Rende
         A.normal[3 * idx + 2] = normal.z;
     }
     if (A.object_id) A.object_id[idx] = object;
+}
+
+// One tile per wavefront, tiles dealt by the dispatcher.
+template <bool PACKET>
+__global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_kernel(const RenderArgs A) {
+    NGX_PAIR_LDS(L);
+    const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
+    if (tile >= A.total_tiles) return;
+    render_tile<PACKET>(A, L, tile);
+}
+
+// Persistent: a fixed grid whose wavefronts take tiles until none are left. The tiles are cut into eight
+// contiguous ranges, one per XCD (the dispatcher deals workgroups round-robin over the XCDs, so workgroup b
+// runs on XCD b mod 8); each range has its own counter, so the worlds of a range stay in one XCD's L2.
+// Every wavefront leaves the loop once its range's counter passes the range's end.
+template <bool PACKET>
+__global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_persistent_kernel(const RenderArgs A) {
+    NGX_PAIR_LDS(L);
+    const int xcd = blockIdx.x & 7;
+    const long long first = A.total_tiles * xcd / 8, count = A.total_tiles * (xcd + 1) / 8 - first;
+    while (true) {
+        int t = 0;
+        if (lane_id() == 0) t = atomicAdd(&A.tile_counters[xcd], 1);
+        t = __builtin_amdgcn_readlane(t, 0);
+        if (t >= count) break;
+        render_tile<PACKET>(A, L, first + t);
+    }
+}
+
+__global__ void zero_counters_kernel(int *c) {
+    if (threadIdx.x < 8) c[threadIdx.x] = 0;
 }
 
 #ifdef NGX_TRACE_STATS
@@ -927,8 +956,15 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
 
 }  // namespace
 
-hipError_t launch_render(const RenderArgs &a, hipStream_t stream) {
+hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int persistent_blocks) {
     if (a.total_tiles <= 0) return hipSuccess;
+    if (a.tile_counters && persistent_blocks > 0) {
+        if (a.total_tiles >= INT32_MAX || persistent_blocks % 8 != 0) return hipErrorInvalidValue;
+        zero_counters_kernel<<<1, 64, 0, stream>>>(a.tile_counters);
+        if (a.packet) render_persistent_kernel<true><<<persistent_blocks, 256, 0, stream>>>(a);
+        else render_persistent_kernel<false><<<persistent_blocks, 256, 0, stream>>>(a);
+        return hipGetLastError();
+    }
     const long long blocks = (a.total_tiles + 3) / 4;
     if (a.packet) render_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a);
     else render_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a);

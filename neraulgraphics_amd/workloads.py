@@ -126,6 +126,43 @@ def config(index, worlds=None, rays_per_world=None, seed=None):
     raise ValueError(f"no configuration {index}")
 
 
+class _Lcg:
+    """A 32-bit linear congruential generator (multiplier 1664525, increment 1013904223, start 777) giving
+    floats in [0, 1) from its top 24 bits: the stream tests/test_batch.cpp:50-56 of the reference draws from."""
+
+    def __init__(self, state=777):
+        self.state = state
+
+    def next(self):
+        self.state = (self.state * 1664525 + 1013904223) & 0xFFFFFFFF
+        return np.float32(self.state >> 8) / np.float32(1 << 24)
+
+
+def scattered(worlds, per_world, counts=None, width=96, height=54, spread=None):
+    """Boxes (half size 0.4) and icospheres (radius 0.5, one subdivision), instance i using mesh i mod 2, scattered
+    through each world as in the reference's large-scene test (tests/test_batch.cpp:338-349): position
+    (s (r-0.5), s/2 (r-0.5), -10 - s r) with s = 60 for one world and 3 otherwise, rotation (r-0.5, r-0.5, r-0.5, 1)
+    normalized, scale 0.5 + r; every world seen from (0, 0, 5) looking at (0, 0, -30) with a 90 degree field of
+    view. `counts` (optional) gives each world's number of instances out of per_world; `spread` overrides s."""
+    meshes = [make_box((0.4, 0.4, 0.4)), make_icosphere(0.5, 1)]
+    insts = [Instance(i % 2) for i in range(per_world)]
+    rng = _Lcg()
+    spread = np.float32(spread if spread is not None else (60.0 if worlds == 1 else 3.0))
+    half = np.float32(0.5)
+    pos = np.empty((worlds * per_world, 3), np.float32)
+    rot = np.empty((worlds * per_world, 4), np.float32)
+    sc = np.empty(worlds * per_world, np.float32)
+    for g in range(worlds * per_world):
+        pos[g] = (spread * (rng.next() - half), spread * half * (rng.next() - half), np.float32(-10.0) - spread * rng.next())
+        rot[g] = (rng.next() - half, rng.next() - half, rng.next() - half, 1.0)
+        sc[g] = half + rng.next()
+    mats = make_transforms(pos, rot, sc).reshape(worlds, per_world, 12)
+    cam = look_at((0.0, 0.0, 5.0), (0.0, 0.0, -30.0)).reshape(12)
+    return dict(width=width, height=height, worlds=worlds, cameras_per_world=1, meshes=meshes, instances=insts,
+                counts=None if counts is None else np.asarray(counts, np.int32), mode="matrices", matrices=mats,
+                cameras=np.tile(cam, (worlds, 1)), fov=1.5708)
+
+
 def _qmul(a, b):
     """Hamilton product of quaternions stored x, y, z, w."""
     ax, ay, az, aw = np.moveaxis(a, -1, 0)

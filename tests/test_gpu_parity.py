@@ -53,6 +53,43 @@ def test_top_level_build_matches_the_reference_algorithm(gpu, oracle, index, wor
     _check_build(r, ref, ref["world_off"])
 
 
+def test_large_worlds_build_and_render_as_the_reference_algorithm(gpu, oracle):
+    """Worlds above 1024 instances take the device-wide build; 1024 and below the workgroup build. Both must give
+    the reference algorithm's keys, permutation and trees, and the same images."""
+    counts = [3000, 17, 1025, 1024, 0, 2]
+    _say(f"large-world build parity: counts {counts}")
+    cfg = workloads.scattered(len(counts), 3000, counts=counts, spread=30.0)
+    r = workloads.make_renderer(cfg)
+    r.keep_build_intermediates(True)
+    r.set_instance_transforms(cfg["matrices"])
+    ref = forward(cfg, oracle)
+    _check_build(r, ref, ref["world_off"])
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "large worlds")
+    _say(rep)
+    assert rep["hits"] > 1000
+
+
+@pytest.mark.parametrize("worlds,per", [(1, 20000), (300, 40)])
+def test_morton_trees_see_what_the_sah_trees_see(gpu, oracle, worlds, per):
+    """tests/test_batch.cpp:335-371: 20,000 instances in one world and 300 worlds of 40, rendered through this
+    backend's Morton-code trees and through the reference CPU backend's SAH trees (the oracle), agree: at most
+    1 pixel in 2000 differs in object ID, and where IDs agree depths are within 0.01."""
+    _say(f"LBVH vs SAH: {worlds} x {per}")
+    cfg = workloads.scattered(worlds, per)
+    r = workloads.make_renderer(cfg)
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, object_ids=True)
+    ref = forward(cfg, oracle, tlas="sah", outputs=("depth", "id"))
+    a_id, b_id = np.asarray(out["object_ids"]), ref["object_id"]
+    a_d, b_d = np.asarray(out["depth"]), ref["depth"]
+    diff = int((a_id != b_id).sum())
+    hits = int((a_id != 0).sum())
+    same = (a_id == b_id) & np.isfinite(a_d) & np.isfinite(b_d)
+    worst = float(np.abs(a_d[same] - b_d[same]).max()) if same.any() else 0.0
+    _say(f"{hits} of {a_id.size} pixels hit, {diff} differ in ID, worst depth difference {worst:.6f}")
+    assert hits > 1000 and diff <= a_id.size // 2000 and worst < 0.01
+
+
 @pytest.mark.parametrize("walk", ["packet", "lanes"])
 @pytest.mark.parametrize("index,worlds", [(0, None), (1, 32), (2, 16), (4, 16)])
 def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds, walk):
@@ -68,6 +105,34 @@ def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds, walk):
 
 
 @pytest.mark.parametrize("walk", ["packet", "lanes"])
+@pytest.mark.parametrize("tile_w,pair_limit,persistent", [(16, 16, True), (32, 1, False), (8, 64, True)])
+def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_limit, persistent):
+    _say(f"render parity: config 2, 16 worlds, tile_w {tile_w}, pair_limit {pair_limit}, persistent {persistent}")
+    cfg = workloads.config(2, worlds=16)
+    r = workloads.make_renderer(cfg)
+    r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
+    assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent))
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    ref = forward(cfg, oracle)
+    _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 2"))
+
+
+def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, monkeypatch):
+    monkeypatch.setenv("NGX_TUNE_CACHE", str(tmp_path / "tune.json"))
+    cfg = workloads.config(1, worlds=64)
+    r = workloads.make_renderer(cfg)
+    _say("autotune: config 1, 64 worlds")
+    got = r.autotune(cfg["cameras"], cfg["fov"], reps=2)
+    _say(got)
+    assert got["ms"] > 0 and got["tile_w"] in (8, 16, 32, 64)
+    assert {k: got[k] for k in r.trace_config} == r.trace_config
+    again = r.autotune(cfg["cameras"], cfg["fov"])
+    assert again["ms"] is None and {k: again[k] for k in r.trace_config} == r.trace_config
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    ref = forward(cfg, oracle)
+    compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 1 tuned")
+
+
 def test_ray_queries_match_the_oracle(gpu, oracle, walk):
     _say(f"ray parity: config 3, 8 worlds x 4096 rays, {walk} walk")
     cfg = workloads.config(3, worlds=8, rays_per_world=4096)
