@@ -5,15 +5,17 @@ in FP64 and rebuilds every world's top-level BVH), then depth + object-ID images
 kernel). The workload is 4096 worlds at 128x128 with U{8..256} instances per world out of a 256-instance
 template carried by 64 bodies (neraulgraphics_amd.workloads, configuration 2); four resident pose sets are
 cycled so every step has new poses. Inputs are resident in HBM before timing; the step is replayed from a
-HIP graph.
+HIP graph. Before timing, the trace configuration is autotuned on the benchmark's own cameras (--no-tune keeps
+the defaults).
 
-Also reported, on the same JSON line: the forward + pose-backward pass of configuration 4 (1024 worlds,
+Also reported, on the same JSON line: LiDAR ray queries of configuration 3 (1024 worlds x 65,536 rays,
+rays and results resident on the device); the forward + pose-backward pass of configuration 4 (1024 worlds,
 64x64 depth, 32 bodies) and its ratio to the forward alone; the render kernel against the HBM roofline;
 and the reference CPU algorithm (this repository's C restatement of the reference CPU backend: FP64 body
 poses, per-world SAH top level with refit, the same traversal, rows of views in chunks of 4 over threads)
 timed on this host on a bounded sample.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph]"""
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph] [--no-tune] [--no-lidar] [--no-bwd]"""
 import argparse
 import json
 import os
@@ -58,15 +60,20 @@ def bench_config2(args, torch):
     from neraulgraphics_amd import workloads
     cfg = workloads.config(2)
     r = workloads.make_renderer(cfg)
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    tuned = None
     if args.tile_w:
         r.set_trace_config(args.tile_w, not args.no_xcd)
+    elif not args.no_tune:
+        # the autotuner times its candidates on these cameras (outside the timed region) and keeps the fastest
+        tuned = r.autotune(cams, cfg["fov"], depth=True, object_ids=True, reps=2, cache=False)
+        print(f"# autotuned: {tuned}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(7)
     pose_sets = [torch.from_numpy(cfg["body_poses"]).cuda()]
     for _ in range(3):
         p = cfg["body_poses"].copy()
         p[..., :3] += rng.normal(0, 0.05, p[..., :3].shape).astype(np.float32)
         pose_sets.append(torch.from_numpy(p).cuda())
-    cams = torch.from_numpy(cfg["cameras"]).cuda()
     fov = cfg["fov"]
     k = [0]
 
@@ -127,7 +134,27 @@ def bench_config2(args, torch):
     torch.cuda.synchronize()
     hit_frac = float(torch.isfinite(out["depth"]).float().mean())
     return dict(cfg=cfg, sec=sec, build_ms=build_ms, render_ms=render_ms, views=views, instances=inst, graph=bool(graphs),
-                hit_frac=hit_frac, renderer=r)
+                hit_frac=hit_frac, renderer=r, trace_config=r.trace_config)
+
+
+def bench_config3(args, torch):
+    """LiDAR: 1024 worlds x 65,536 rays over 100,000 instances of a 138,000-triangle grid; rays resident on the
+    device, results left there (zero-copy)."""
+    from neraulgraphics_amd import workloads
+    cfg = workloads.config(3)
+    r = workloads.make_renderer(cfg)
+    o = torch.from_numpy(cfg["origins"]).cuda()
+    d = torch.from_numpy(cfg["directions"]).cuda()
+    for _ in range(2):
+        r.cast_rays(o, d, cfg["max_distance"])
+    torch.cuda.synchronize()
+    steps = max(3, args.steps // 4)
+    t = timed_steps(lambda: r.cast_rays(o, d, cfg["max_distance"]), steps, 1, torch)
+    n = o.shape[0] * o.shape[1]
+    out = r.cast_rays(o, d, cfg["max_distance"])
+    torch.cuda.synchronize()
+    return dict(rays_per_s=n / t, ms=t * 1e3, rays=n, hit_frac=float(torch.isfinite(out["distance"]).float().mean()),
+                trace_config=r.trace_config)
 
 
 def bench_config4(args, torch):
@@ -203,6 +230,8 @@ def main():
     ap.add_argument("--no-bwd", action="store_true")
     ap.add_argument("--tile-w", type=int, default=0)
     ap.add_argument("--no-xcd", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default trace configuration instead of autotuning")
+    ap.add_argument("--no-lidar", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 512 MiB three times with a plain copy kernel, to scale HBM counters")
@@ -229,6 +258,10 @@ def main():
     rb = render_bytes(c2["views"], 128, 128, c2["instances"])
     achieved = rb / (c2["render_ms"] * 1e-3) / 1e9
     print(f"# step {sec * 1e3:.3f} ms (build {c2['build_ms']:.3f} ms, render {c2['render_ms']:.3f} ms)", file=sys.stderr, flush=True)
+    c3 = None
+    if not args.no_lidar:
+        print("# config 3: LiDAR ray queries", file=sys.stderr, flush=True)
+        c3 = bench_config3(args, torch)
     c4 = None
     if not args.no_bwd:
         print("# config 4: forward and forward+backward", file=sys.stderr, flush=True)
@@ -258,7 +291,7 @@ def main():
         "config": {"workload": "configuration 2: 4096 worlds, 128x128 depth+object-ID, U{8..256} of a 256-instance "
                                "template per world, 64 body poses per world, new poses every step",
                    "worlds": 4096, "width": 128, "height": 128, "instances": c2["instances"], "outputs": "depth+ID",
-                   "hip_graph": c2["graph"]},
+                   "hip_graph": c2["graph"], "trace_config": c2["trace_config"]},
         "rays_per_s": c2["views"] * 128 * 128 / sec,
         "build_ms": c2["build_ms"],
         "render_ms": c2["render_ms"],
@@ -267,6 +300,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": rb},
         "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
     }
+    if c3:
+        line.update(lidar_rays_per_s_config3=c3["rays_per_s"], lidar_ms_config3=c3["ms"], lidar_hit_fraction=c3["hit_frac"])
     if c4:
         line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s_config4=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])

@@ -161,6 +161,7 @@ int32_t ngx_view_count(const ngx_ctx *ctx);
 int32_t ngx_instances_per_world(const ngx_ctx *ctx);
 int32_t ngx_instance_count(const ngx_ctx *ctx);   /* over all worlds */
 int32_t ngx_device_id(const ngx_ctx *ctx);
+ngx_status ngx_frame_size(const ngx_ctx *ctx, int32_t *width, int32_t *height);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *ngx_last_error(void);
@@ -172,9 +173,11 @@ const char *ngx_last_error(void);
  *  packet      1: the wavefront walks its 64 rays as a packet (the default); 0: each lane walks its own ray;
  *  pair_limit  packet walk: (ray, triangle) pairs queued before they are tested together, 1..64;
  *  persistent  1: a fixed grid of 8 workgroups per compute unit takes tiles from per-XCD counters instead
- *              of one wavefront per tile. */
+ *              of one wavefront per tile;
+ *  lane_min_triangles  packet walk: the bottom level of meshes with at least this many triangles is walked
+ *              lane by lane (each ray its own path) instead of as a packet (default 1000). */
 typedef struct {
-    int32_t tile_w, xcd_remap, packet, pair_limit, persistent;
+    int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles;
 } ngx_trace_config;
 ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config);
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);

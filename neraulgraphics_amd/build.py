@@ -45,6 +45,25 @@ def build(verbose=False, extra_flags=(), out=LIB):
     return out
 
 
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_batch.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_batch")
+
+
+def build_cpp_tests(verbose=False):
+    """Compiles the C++ front end's test program (tests/cpp/test_batch.cpp) against libngx.so with g++; the
+    binary finds the library through its run path, so it runs from any copy of the tree."""
+    os.makedirs(os.path.dirname(CPP_TEST_BIN), exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "cpp"), CPP_TEST,
+           "-L" + HERE, "-lngx", "-Wl,-rpath,$ORIGIN/../../../neraulgraphics_amd", "-o", CPP_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compiling {CPP_TEST} failed:\n{r.stderr}")
+    if verbose:
+        print("built", CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_tests(verbose=True)
     sys.exit(0)

@@ -104,6 +104,9 @@ struct ngx_ctx {
     float *d_mesh_bounds = nullptr;
     ngx::WideNode *d_blas_wide = nullptr;
     int *d_blas_parent = nullptr;
+    ngx::SkipNode *d_skip = nullptr;
+    int *d_mesh_skip_root = nullptr, *d_mesh_tris = nullptr;
+    int max_mesh_tris = 0;
     ngx::MeshRoot *d_mesh_roots = nullptr;
     float4 *d_tris = nullptr;
     float4 *d_attrs = nullptr;
@@ -148,20 +151,20 @@ struct ngx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     hipStream_t last_stream = nullptr;
-    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 0;
+    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 0, lane_min_tris = 1000;
     int persistent_blocks = 0;          // 8 workgroups of 4 wavefronts per compute unit
     int *d_tile_counters = nullptr;
 
     ~ngx_ctx() {
         int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_wide_parent,
-                       d_tile_counters};
+                       d_tile_counters, d_mesh_skip_root, d_mesh_tris};
         for (int *p : ints) if (p) (void)hipFree(p);
         float *floats[] = {d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
                            d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
                            d_dbg_world_bounds};
         for (float *p : floats) if (p) (void)hipFree(p);
         void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
-                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, big.key[0], big.key[1], big.val[0], big.val[1],
+                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, d_skip, big.key[0], big.key[1], big.val[0], big.val[1],
                           big.box, big.node_box, big.left, big.right, big.parent, big.arrivals, big.bounds, big.tile_counts};
         for (void *p : others) if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -187,6 +190,10 @@ struct ngx_ctx {
         s.floor_zmin = floor[2];
         s.floor_zmax = floor[3];
         s.pair_limit = pair_limit;
+        s.skip = d_skip;
+        s.mesh_skip_root = d_mesh_skip_root;
+        s.mesh_tris = d_mesh_tris;
+        s.lane_min_tris = lane_min_tris;
         return s;
     }
 
@@ -237,10 +244,14 @@ struct ngx_ctx {
         a.tiles_per_view = a.tiles_x * ((height + a.tile_h - 1) / a.tile_h);
         a.total_tiles = (long long)a.tiles_per_view * views;
         a.xcd_remap = xcd_remap;
-        a.packet = packet;
+        a.walk = walk();
         a.tile_counters = persistent ? d_tile_counters : nullptr;
         return a;
     }
+
+    // The kernels' walk: per-lane, packet, or packet with the bottom level of meshes of lane_min_tris or more
+    // triangles walked lane by lane.
+    int walk() const { return !packet ? 0 : (max_mesh_tris >= lane_min_tris ? 2 : 1); }
 
     bool capturing(hipStream_t s) const {
         hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
@@ -377,6 +388,10 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             upload(c.d_blas_parent, built.wide_parent.data(), built.wide_parent.size());
             if (!c.d_blas_parent) device_alloc(c.d_blas_parent, 1);
             upload(c.d_mesh_roots, built.mesh_roots.data(), built.mesh_roots.size());
+            upload(c.d_skip, built.skip.data(), built.skip.size());
+            upload(c.d_mesh_skip_root, built.roots.data(), built.roots.size());
+            upload(c.d_mesh_tris, built.tri_counts.data(), built.tri_counts.size());
+            for (const int t : built.tri_counts) c.max_mesh_tris = std::max(c.max_mesh_tris, t);
             upload(c.d_tris, reinterpret_cast<const float4 *>(built.triangles.data()), built.triangles.size() / 4);
             if (!built.attributes.empty())
                 upload(c.d_attrs, reinterpret_cast<const float4 *>(built.attributes.data()), built.attributes.size() / 4);
@@ -586,7 +601,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         a.normal = (outputs & NGX_OUTPUT_NORMAL) ? ctx->d_normal : nullptr;
         a.object_id = (outputs & NGX_OUTPUT_OBJECT_ID) ? ctx->d_id : nullptr;
         a.xcd_remap = ctx->xcd_remap;
-        a.packet = ctx->packet;
+        a.walk = ctx->walk();
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
         HIP_CHECK(ngx::launch_render(a, s, ctx->persistent ? ctx->persistent_blocks : 0));
@@ -643,7 +658,7 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
         a.distance = ctx->d_hit_dist;
         a.normal = ctx->d_hit_normal;
         a.object_id = ctx->d_hit_id;
-        a.packet = ctx->packet;
+        a.walk = ctx->walk();
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
         HIP_CHECK(ngx::launch_rays(a, s));
@@ -690,7 +705,7 @@ ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const 
         a.body_poses = body_poses;
         a.parts = parts;
         a.partial = ctx->d_partial;
-        a.packet = ctx->packet;
+        a.walk = ctx->walk();
         HIP_CHECK(ngx::launch_depth_backward(a, s));
         ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw * parts, ctx->d_partial, body_poses, grad_out};
         HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
@@ -722,7 +737,7 @@ ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const 
         a.dirs = directions;
         a.rays_per_world = rays_per_world;
         a.chunks = chunks;
-        a.packet = ctx->packet;
+        a.walk = ctx->walk();
         a.max_distance = max_distance;
         a.distance = distance;
         a.grad_distance = grad_distance;
@@ -766,6 +781,14 @@ float ngx_last_kernel_ms(ngx_ctx *ctx) {
 }
 
 int32_t ngx_world_count(const ngx_ctx *ctx) { return ctx ? ctx->worlds : 0; }
+
+ngx_status ngx_frame_size(const ngx_ctx *ctx, int32_t *width, int32_t *height) {
+    return guarded([&] {
+        require(ctx != nullptr && width != nullptr && height != nullptr, "arguments must not be null");
+        *width = ctx->width;
+        *height = ctx->height;
+    });
+}
 int32_t ngx_view_count(const ngx_ctx *ctx) { return ctx ? ctx->views : 0; }
 int32_t ngx_instances_per_world(const ngx_ctx *ctx) { return ctx ? ctx->per_world : 0; }
 int32_t ngx_instance_count(const ngx_ctx *ctx) { return ctx ? ctx->total : 0; }
@@ -782,29 +805,35 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
         ctx->packet = c.packet ? 1 : 0;
         ctx->pair_limit = c.pair_limit;
         ctx->persistent = c.persistent ? 1 : 0;
+        require(c.lane_min_triangles >= 0, "lane_min_triangles must not be negative");
+        ctx->lane_min_tris = c.lane_min_triangles;
     });
 }
 
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config) {
     return guarded([&] {
         require(ctx != nullptr && config != nullptr, "context and config must not be null");
-        *config = ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent};
+        *config = ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent, ctx->lane_min_tris};
     });
 }
 
 ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
                         void *stream, ngx_trace_config *best, float *best_ms) {
-    const ngx_trace_config start = ctx ? ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent}
+    const ngx_trace_config start = ctx ? ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent,
+                                                          ctx->lane_min_tris}
                                        : ngx_trace_config{};
     const ngx_status st = guarded([&] {
         require(ctx != nullptr && cams12 != nullptr, "context and cameras must not be null");
         require(reps >= 1 && reps <= 100, "reps must be 1 to 100");
         require(!ctx->capturing(static_cast<hipStream_t>(stream)), "autotune cannot run while the stream is being captured");
+        // the per-lane bottom level for: no mesh, meshes of 1000+ triangles, of 300+, every mesh
+        const int never = 1 << 30;
         std::vector<ngx_trace_config> cands;
         for (const int tw : {8, 16, 32}) {
-            for (const int limit : {32, 64})
-                for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers});
-            if (tw <= 16) cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0});
+            for (const int lanes : {never, 1000, 300, 0})
+                for (const int limit : {32, 64})
+                    for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers, lanes});
+            if (tw <= 16) cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never});
         }
         const hipStream_t s = static_cast<hipStream_t>(stream);
         hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -234,6 +234,51 @@ void build_meshes(const MeshInput &in, BuiltMeshes &out) {
         if (out.wide[i].left >= 0) out.wide_parent[out.wide[i].left] = (int)i;
         if (out.wide[i].right >= 0) out.wide_parent[out.wide[i].right] = (int)i;
     }
+    // The per-lane layout: every node with its octant-ordered escapes (the next node after its subtree).
+    out.skip.assign(out.nodes.size(), SkipNode{});
+    for (size_t i = 0; i < out.nodes.size(); ++i) {
+        const BlasNode &n = out.nodes[i];
+        SkipNode &k = out.skip[i];
+        for (int ax = 0; ax < 3; ++ax) {
+            k.lo[ax] = n.lo[ax];
+            k.hi[ax] = n.hi[ax];
+        }
+        k.a = n.a;
+        if (n.count > 0) {
+            k.meta = n.count;
+        } else {
+            const BlasNode &l = out.nodes[n.a], &r = out.nodes[n.a + 1];
+            int axis = 0;
+            float sep = -1.0f;
+            for (int ax = 0; ax < 3; ++ax) {
+                const float d = std::fabs((l.lo[ax] + l.hi[ax]) - (r.lo[ax] + r.hi[ax]));
+                if (d > sep) {
+                    sep = d;
+                    axis = ax;
+                }
+            }
+            const int flip = (l.lo[axis] + l.hi[axis]) > (r.lo[axis] + r.hi[axis]) ? 1 : 0;
+            k.meta = -(1 + 2 * axis + flip);
+        }
+    }
+    for (int o = 0; o < 8; ++o) {
+        std::vector<std::pair<int, int>> todo;  // (node, its escape)
+        for (const int root : out.roots) todo.push_back({root, -1});
+        while (!todo.empty()) {
+            const auto [node, esc] = todo.back();
+            todo.pop_back();
+            out.skip[node].escape[o] = esc;
+            const int meta = out.skip[node].meta;
+            if (meta > 0) continue;
+            const int code = -meta - 1, axis = code >> 1, flip = code & 1;
+            const bool positive = (o >> axis) & 1;
+            const int left = out.skip[node].a, right = left + 1;
+            const int first = (positive != (flip == 1)) ? left : right, second = first == left ? right : left;
+            todo.push_back({second, esc});
+            todo.push_back({first, second});
+        }
+    }
+    for (int m = 0; m < in.mesh_count; ++m) out.tri_counts.push_back((in.index_offsets[m + 1] - in.index_offsets[m]) / 3);
     for (size_t m = 0; m < out.roots.size(); ++m) {
         const BlasNode &root = out.nodes[out.roots[m]];
         MeshRoot mr{};

@@ -66,6 +66,19 @@ struct alignas(16) MeshRoot {
 };
 static_assert(sizeof(MeshRoot) == 32, "32-byte mesh root");
 
+// The bottom level again, for the per-lane walk: one 64-byte record per node of the reference layout (same
+// indices), with the next node to visit after this node's subtree for each of the eight ray-direction octants
+// (octant bit k set when the ray's direction component k is >= 0). Children are visited in octant order:
+// the child whose centre is lower along `axis` first when the direction is positive along it.
+struct alignas(64) SkipNode {
+    float lo[3];
+    int32_t a;        // interior: left child (the right is a + 1); leaf: first triangle
+    float hi[3];
+    int32_t meta;     // leaf: triangle count (> 0); interior: -(1 + 2 * axis + flip), flip set when the left child's centre is higher
+    int32_t escape[8];
+};
+static_assert(sizeof(SkipNode) == 64, "64-byte skip node");
+
 // ------------------------------------------------------------------ float3 arithmetic --
 
 __device__ __forceinline__ float3 mk3(float x, float y, float z) { return make_float3(x, y, z); }

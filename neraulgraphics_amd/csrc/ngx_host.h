@@ -30,6 +30,8 @@ struct BuiltMeshes {
     std::vector<WideNode> wide;       // the traversal layout of nodes' interior nodes
     std::vector<int> wide_parent;     // per wide node: its parent wide node, -1 at a mesh's root
     std::vector<MeshRoot> mesh_roots; // per mesh, for the tracer
+    std::vector<SkipNode> skip;       // per node of `nodes`: the per-lane walk's layout
+    std::vector<int> tri_counts;      // per mesh
 };
 
 // Binned SAH build (src/bvh.cpp:46-174). Returns the node count.

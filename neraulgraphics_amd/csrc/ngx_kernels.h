@@ -57,6 +57,10 @@ struct TraceScene {
     const float4 *spheres;     // centre xyz, radius
     float floor_y, floor_hx, floor_zmin, floor_zmax;
     int pair_limit;            // packet walk: (ray, triangle) pairs queued before they are tested, 1..64
+    const SkipNode *skip;      // the bottom level for the per-lane walk, same indices as the reference layout
+    const int *mesh_skip_root; // per mesh: its root in skip
+    const int *mesh_tris;      // per mesh: triangle count
+    int lane_min_tris;         // packet walk: meshes of at least this many triangles are walked lane by lane
 };
 
 struct RenderArgs {
@@ -71,7 +75,7 @@ struct RenderArgs {
     float *normal;             // [views][H][W][3] or null
     int32_t *object_id;        // [views][H][W] or null
     int xcd_remap;             // 1: consecutive tiles stay on one XCD
-    int packet;                // 1: wavefront-packet walk; 0: per-lane walk
+    int walk;                  // 0 per-lane, 1 packet, 2 packet with fine meshes lane by lane
     int *tile_counters;        // persistent mode (non-null): 8 counters, zeroed before the launch, one per XCD
 };
 
@@ -85,7 +89,7 @@ struct RayArgs {
     float *distance;
     float *normal;
     int32_t *object_id;
-    int packet;
+    int walk;
 };
 
 // The pose backward of depth: dL/d(body pose) for every world, through each pixel's fixed hit triangle.
@@ -102,7 +106,7 @@ struct DepthBackwardArgs {
     const float *body_poses;   // [worlds][bodies][7], the poses the tree was built from
     int parts;                 // workgroups per view, each taking every parts-th group of tiles
     float *partial;            // [views][parts][bodies][7]
-    int packet;                // the re-trace's walk, as the forward's
+    int walk;                  // the re-trace's walk, as the forward's
 };
 
 struct RangeBackwardArgs {
@@ -119,7 +123,7 @@ struct RangeBackwardArgs {
     const float *body_poses;
     int worlds;
     float *partial;            // [worlds][chunks][bodies][7]
-    int packet;
+    int walk;
 };
 
 struct PoseGradFinishArgs {

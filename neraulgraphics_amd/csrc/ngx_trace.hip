@@ -183,9 +183,99 @@ __device__ __forceinline__ void add_leaf(const TraceScene &S, PairLds *L, int &p
     }
 }
 
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ int wave_prefix_sum(int v) {
+    const int lane = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// The bottom level of one instance walked lane by lane, for meshes whose triangles are small next to a
+// wavefront's tile: there each ray of the packet needs its own leaves, so a shared walk visits the union
+// of 64 paths. Each lane follows its own ray through the octant-ordered skip layout (a box test per step;
+// a hit descends to the first child, a miss or a finished leaf goes to the node's escape for the ray's
+// octant), so no stack is kept. Leaves wait until every lane has one or is done, then all waiting lanes'
+// triangles are tested together as (ray, triangle) pairs. The root's box has been tested already.
+template <class Stats>
+__device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g, const uint64_t root_mask, const int root,
+                                    const float3 lo, const float3 ld, const float3 inv_ld, float &tmax, LaneHit &h, Stats &stats) {
+    const int lane = lane_id();
+    const int octant = (ld.x >= 0.0f ? 1 : 0) | (ld.y >= 0.0f ? 2 : 0) | (ld.z >= 0.0f ? 4 : 0);
+    bool live = in_mask(root_mask);
+    int cur = -1, leaf = -1, leaf_count = 0, after_leaf = -1;
+    auto first_child = [&](const int a, const int meta) {
+        const int code = -meta - 1, axis = code >> 1, flip = code & 1;
+        const bool positive = (octant >> axis) & 1;
+        return positive != (flip == 1) ? a : a + 1;
+    };
+    if (live) {
+        const SkipNode &r = S.skip[root];
+        if (r.meta > 0) {
+            leaf = r.a;
+            leaf_count = r.meta;
+            after_leaf = -1;
+        } else {
+            cur = first_child(r.a, r.meta);
+        }
+    }
+    while (true) {
+        // box steps until every live lane waits at a leaf or has finished
+        while (true) {
+            const bool stepping = live && leaf < 0 && cur >= 0;
+            const uint64_t sm = ballot(stepping);
+            if (sm == 0) break;
+            stats.add(ST_BLAS_NODES, 1);
+            stats.add(ST_BLAS_LANES, popc64(sm));
+            if (stepping) {
+                const float4 *np = reinterpret_cast<const float4 *>(S.skip + cur);
+                const float4 f0 = np[0], f1 = np[1];
+                const int esc = S.skip[cur].escape[octant];
+                const int a = __float_as_int(f0.w), meta = __float_as_int(f1.w);
+                if (box_entry(f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, lo, inv_ld, tmax) < NO_HIT) {
+                    if (meta > 0) {
+                        leaf = a;
+                        leaf_count = meta;
+                        after_leaf = esc;
+                    } else {
+                        cur = first_child(a, meta);
+                    }
+                } else {
+                    cur = esc;
+                }
+            }
+            if (live && leaf < 0 && cur < 0) live = false;
+        }
+        // the waiting lanes' triangles, as batched pairs
+        const uint64_t waiting = ballot(live && leaf >= 0);
+        if (waiting == 0) break;
+        int done = 0;
+        while (true) {
+            const bool more = live && leaf >= 0 && done < leaf_count;
+            const uint64_t mm = ballot(more);
+            if (mm == 0) break;
+            const int per = max(1, 64 / popc64(mm));
+            const int take = more ? min(leaf_count - done, per) : 0;
+            const int end = wave_prefix_sum(take);
+            const int total = __builtin_amdgcn_readlane(end, 63);
+            for (int k = 0; k < take; ++k) L->pair[end - take + k] = lane | ((leaf + done + k) << 6);
+            int pairs = total;
+            flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
+            done += take;
+        }
+        if (live && leaf >= 0) {
+            leaf = -1;
+            cur = after_leaf;
+            if (cur < 0) live = false;
+        }
+    }
+}
+
 // One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
 // renormalized so t is shared), then the mesh's own tree.
-template <class Stats>
+template <bool LANES, class Stats>
 __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, const uint64_t mask, const float3 o,
                                const float3 d, float &tmax, LaneHit &h, Stats &stats) {
     stats.add(ST_INSTANCES, 1);
@@ -203,6 +293,10 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     const uint64_t root_mask = ballot(in && box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, tmax) < NO_HIT);
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
+    if (LANES && S.mesh_tris[meta.x] >= S.lane_min_tris) {
+        walk_instance_lanes(S, L, g, root_mask, S.mesh_skip_root[meta.x], lo, ld, inv_d, tmax, h, stats);
+        return;
+    }
     const int root_ref = __float_as_int(m0.w);
     int pairs = 0;
     if (root_ref < 0) {
@@ -270,14 +364,14 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
 // distance to beat on entry and its closest distance on return.
-template <class Stats>
+template <bool LANES, class Stats>
 __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
                             const uint64_t live, float &tmax, LaneHit &h, Stats &stats) {
     if (live == 0) return;
     const int root = uniform(S.roots[world]);
     if (root < 0) return;
     if (root >= S.total - 1) {
-        visit_instance(S, L, root - (S.total - 1), live, o, d, tmax, h, stats);
+        visit_instance<LANES>(S, L, root - (S.total - 1), live, o, d, tmax, h, stats);
         return;
     }
     const float3 inv_d = safe_inverse(d);
@@ -317,7 +411,7 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
             next_ref = 0; next_mask = 0;
         }
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) visit_instance(S, L, ~next_ref, next_mask, o, d, tmax, h, stats);
+            if (next_mask != 0) visit_instance<LANES>(S, L, ~next_ref, next_mask, o, d, tmax, h, stats);
             if (st.sp == 0) return;
             st.pop(next_ref, next_mask);
         }
@@ -497,7 +591,11 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
 
 // Spheres, then the floor, then the meshes with the best so far as their limit (render_core.h:78-112).
 // Returns the object, or OBJECT_NONE; t and h describe the hit.
-template <bool PACKET, class Stats>
+// The walks: per-lane (both levels lane by lane), packet (both levels as a packet), and hybrid (the packet
+// walk with fine meshes' bottom levels walked lane by lane).
+enum Walk { WALK_LANES = 0, WALK_PACKET = 1, WALK_HYBRID = 2 };
+
+template <int WALK, class Stats>
 __device__ __forceinline__ int scene_hit(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
                                          const uint64_t live, float &best, LaneHit &h, Stats &stats, const float start = 1e30f) {
     best = start;
@@ -530,8 +628,8 @@ __device__ __forceinline__ int scene_hit(const TraceScene &S, PairLds *L, const 
     }
     h.inst = -1;
     h.tri = -1;
-    if (PACKET) {
-        trace_world(S, L, world, o, d, live, best, h, stats);
+    if (WALK != WALK_LANES) {
+        trace_world<WALK == WALK_HYBRID>(S, L, world, o, d, live, best, h, stats);
         wave_lds_order();
         if (h.inst >= 0) {
             const float2 uv = L->hit_uv[lane_id()];
@@ -620,10 +718,10 @@ __device__ __forceinline__ long long wave_tile(const int xcd_remap, const long l
 }
 
 // LDS of the packet walk's triangle batches: one PairLds per wavefront of a 256-thread workgroup.
-#define NGX_PAIR_LDS(name) __shared__ PairLds name##_all[4]; PairLds *name = PACKET ? &name##_all[threadIdx.x >> 6] : nullptr
+#define NGX_PAIR_LDS(name) __shared__ PairLds name##_all[4]; PairLds *name = WALK != WALK_LANES ? &name##_all[threadIdx.x >> 6] : nullptr
 
 // One 64-pixel tile of one view: camera ray per lane, closest hit, depth / normal / object ID.
-template <bool PACKET>
+template <int WALK>
 __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, const long long tile) {
     const int lane = lane_id();
     const int view = (int)(tile / A.tiles_per_view);
@@ -638,7 +736,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
     float best;
     LaneHit h;
     NoStats stats;
-    const int kind = scene_hit<PACKET>(A.s, L, world, cam.pos, d, ballot(valid), best, h, stats);
+    const int kind = scene_hit<WALK>(A.s, L, world, cam.pos, d, ballot(valid), best, h, stats);
     if (!valid) return;
     const size_t idx = ((size_t)view * A.height + y) * A.width + x;
     const bool found = kind != OBJECT_NONE;
@@ -655,20 +753,20 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
 }
 
 // One tile per wavefront, tiles dealt by the dispatcher.
-template <bool PACKET>
-__global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_kernel(const RenderArgs A) {
+template <int WALK>
+__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 8 : WALK == WALK_HYBRID ? 6 : 1) void render_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
     if (tile >= A.total_tiles) return;
-    render_tile<PACKET>(A, L, tile);
+    render_tile<WALK>(A, L, tile);
 }
 
 // Persistent: a fixed grid whose wavefronts take tiles until none are left. The tiles are cut into eight
 // contiguous ranges, one per XCD (the dispatcher deals workgroups round-robin over the XCDs, so workgroup b
 // runs on XCD b mod 8); each range has its own counter, so the worlds of a range stay in one XCD's L2.
 // Every wavefront leaves the loop once its range's counter passes the range's end.
-template <bool PACKET>
-__global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_persistent_kernel(const RenderArgs A) {
+template <int WALK>
+__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 8 : WALK == WALK_HYBRID ? 6 : 1) void render_persistent_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const int xcd = blockIdx.x & 7;
     const long long first = A.total_tiles * xcd / 8, count = A.total_tiles * (xcd + 1) / 8 - first;
@@ -677,7 +775,7 @@ __global__ __launch_bounds__(256, PACKET ? 8 : 1) void render_persistent_kernel(
         if (lane_id() == 0) t = atomicAdd(&A.tile_counters[xcd], 1);
         t = __builtin_amdgcn_readlane(t, 0);
         if (t >= count) break;
-        render_tile<PACKET>(A, L, first + t);
+        render_tile<WALK>(A, L, first + t);
     }
 }
 
@@ -688,7 +786,7 @@ __global__ void zero_counters_kernel(int *c) {
 #ifdef NGX_TRACE_STATS
 // The render kernel's traversal with its counters: each wave writes ST_COUNT ints to its own slot, lane k
 // storing counter k, so no atomics are involved.
-template <bool PACKET>
+template <int WALK>
 __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, int *out) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
@@ -708,7 +806,7 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
     WaveStats stats;
     for (int k = 0; k < ST_COUNT; ++k) stats.c[k] = 0;
     stats.add(ST_VALID, popc64(ballot(valid)));
-    scene_hit<PACKET>(A.s, L, world, cam.pos, d, ballot(valid), best, h, stats);
+    scene_hit<WALK>(A.s, L, world, cam.pos, d, ballot(valid), best, h, stats);
     int mine = 0;
     for (int k = 0; k < ST_COUNT; ++k) mine = lane == k ? stats.c[k] : mine;
     if (lane < ST_COUNT) out[(size_t)tile * ST_COUNT + lane] = mine;
@@ -717,7 +815,7 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
 // worlds walks each world's rays as its own packet.
-template <bool PACKET>
+template <int WALK>
 __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     NGX_PAIR_LDS(L);
     const long long base = ((long long)blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6)) * 64;
@@ -740,7 +838,7 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     h.inst = -1;
     h.tri = -1;
     int kind = OBJECT_NONE;
-    if (PACKET) {
+    if (WALK != WALK_LANES) {
         uint64_t pending = ballot(ok);
         while (pending != 0) {
             const int w = __builtin_amdgcn_readlane(world, first_lane(pending));
@@ -748,7 +846,7 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
             float b;
             LaneHit hh;
             NoStats stats;
-            const int k = scene_hit<true>(A.s, L, w, o, d, mine, b, hh, stats);
+            const int k = scene_hit<WALK>(A.s, L, w, o, d, mine, b, hh, stats);
             if (in_mask(mine)) {
                 best = b;
                 h = hh;
@@ -759,7 +857,7 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     } else {
         // each lane walks its own world
         NoStats stats;
-        kind = scene_hit<false>(A.s, L, world, o, d, ballot(ok), best, h, stats);
+        kind = scene_hit<WALK_LANES>(A.s, L, world, o, d, ballot(ok), best, h, stats);
     }
     if (!valid) return;
     const bool found = ok && kind != OBJECT_NONE && best <= A.max_distance;
@@ -845,7 +943,7 @@ __device__ __forceinline__ void wave_accumulate(const bool has, const int body, 
 constexpr int BWD_WAVES = 4;
 constexpr int MAX_BODIES = 256;
 
-template <bool PACKET>
+template <int WALK>
 __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const DepthBackwardArgs A) {
     extern __shared__ float acc_all[];  // [BWD_WAVES][bodies][7]
     NGX_PAIR_LDS(L);
@@ -871,7 +969,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
         float best;
         LaneHit h;
         NoStats stats;
-        const int kind = scene_hit<PACKET>(A.s, L, world, cam.pos, d, ballot(want), best, h, stats, bound);
+        const int kind = scene_hit<WALK>(A.s, L, world, cam.pos, d, ballot(want), best, h, stats, bound);
         int body = -1;
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         bool has = false;
@@ -887,7 +985,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const De
     }
 }
 
-template <bool PACKET>
+template <int WALK>
 __global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const RangeBackwardArgs A) {
     extern __shared__ float acc_all[];
     NGX_PAIR_LDS(L);
@@ -915,7 +1013,7 @@ __global__ __launch_bounds__(64 * BWD_WAVES) void range_backward_kernel(const Ra
         float best;
         LaneHit h;
         NoStats stats;
-        const int kind = scene_hit<PACKET>(A.s, L, world, o, d, ballot(want), best, h, stats, bound);
+        const int kind = scene_hit<WALK>(A.s, L, world, o, d, ballot(want), best, h, stats, bound);
         int body = -1;
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         bool has = false;
@@ -956,18 +1054,27 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
 
 }  // namespace
 
+#define NGX_BY_WALK(walk, launch) \
+    switch (walk) { \
+    case WALK_LANES: launch(WALK_LANES); break; \
+    case WALK_HYBRID: launch(WALK_HYBRID); break; \
+    default: launch(WALK_PACKET); break; \
+    }
+
 hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int persistent_blocks) {
     if (a.total_tiles <= 0) return hipSuccess;
     if (a.tile_counters && persistent_blocks > 0) {
         if (a.total_tiles >= INT32_MAX || persistent_blocks % 8 != 0) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 64, 0, stream>>>(a.tile_counters);
-        if (a.packet) render_persistent_kernel<true><<<persistent_blocks, 256, 0, stream>>>(a);
-        else render_persistent_kernel<false><<<persistent_blocks, 256, 0, stream>>>(a);
+#define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_blocks, 256, 0, stream>>>(a)
+        NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
         return hipGetLastError();
     }
     const long long blocks = (a.total_tiles + 3) / 4;
-    if (a.packet) render_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a);
-    else render_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a);
+#define NGX_LAUNCH(W) render_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
+    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
     return hipGetLastError();
 }
 
@@ -975,8 +1082,9 @@ hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int pers
 hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream) {
     if (a.total_tiles <= 0) return hipSuccess;
     const long long blocks = (a.total_tiles + 3) / 4;
-    if (a.packet) render_stats_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, out);
-    else render_stats_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, out);
+#define NGX_LAUNCH(W) render_stats_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a, out)
+    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
     return hipGetLastError();
 }
 #endif
@@ -985,8 +1093,9 @@ hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
     if (a.count <= 0) return hipSuccess;
     const long long waves = (a.count + 63) / 64;
     const long long blocks = (waves + 3) / 4;
-    if (a.packet) ray_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a);
-    else ray_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a);
+#define NGX_LAUNCH(W) ray_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
+    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
     return hipGetLastError();
 }
 
@@ -994,8 +1103,9 @@ hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream)
     if (a.views <= 0) return hipSuccess;
     if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
-    if (a.packet) depth_backward_kernel<true><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a);
-    else depth_backward_kernel<false><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a);
+#define NGX_LAUNCH(W) depth_backward_kernel<W><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a)
+    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
     return hipGetLastError();
 }
 
@@ -1003,8 +1113,9 @@ hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream)
     if (a.worlds <= 0 || a.chunks <= 0) return hipSuccess;
     if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
-    if (a.packet) range_backward_kernel<true><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a);
-    else range_backward_kernel<false><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a);
+#define NGX_LAUNCH(W) range_backward_kernel<W><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a)
+    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
     return hipGetLastError();
 }
 

@@ -1,0 +1,187 @@
+// The C++ BatchRenderer (neraulgraphics_amd/cpp/ngx/batch.hpp) exercised the way the reference's own C++ tests
+// exercise theirs (tests/test_batch.cpp:185-331, :376-418): known answers, worlds kept apart, bad input refused
+// with nothing changed. "cpu" runs only what needs no GPU (scene checks before any device work); "gpu" runs
+// everything. Exit status 0 when every check holds.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ngx/batch.hpp"
+
+namespace {
+
+int failures = 0;
+
+void check(const bool ok, const std::string &what) {
+    std::printf("  %s %s\n", ok ? "ok  " : "FAIL", what.c_str());
+    if (!ok) ++failures;
+}
+
+template <class E, class F>
+bool throws(F &&f) {
+    try {
+        f();
+    } catch (const E &) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+constexpr int W = 96, H = 54;
+constexpr float FOV = 1.5708f;
+
+ngx::Mesh box(const float hx, const float hy, const float hz) {
+    ngx::Mesh m;
+    for (int k = 0; k < 8; ++k) m.vertices.insert(m.vertices.end(), {(k & 1) ? hx : -hx, (k & 2) ? hy : -hy, (k & 4) ? hz : -hz});
+    const uint32_t faces[12][3] = {{0, 2, 1}, {1, 2, 3}, {4, 5, 6}, {5, 7, 6}, {0, 1, 4}, {1, 5, 4},
+                                   {2, 6, 3}, {3, 6, 7}, {0, 4, 2}, {2, 4, 6}, {1, 3, 5}, {3, 7, 5}};
+    for (const auto &f : faces) m.indices.insert(m.indices.end(), {f[0], f[1], f[2]});
+    return m;
+}
+
+void add_translation(std::vector<float> &out, const float x, const float y, const float z) {
+    out.insert(out.end(), {1, 0, 0, x, 0, 1, 0, y, 0, 0, 1, z});
+}
+
+// A camera at the origin looking down -z (camera-to-world, OpenGL convention).
+void add_camera(std::vector<float> &out) { add_translation(out, 0, 0, 0); }
+
+void cpu_checks() {
+    std::puts("scene checks, before any device work");
+    const std::vector<ngx::Mesh> meshes = {box(1, 1, 1)};
+    const std::vector<ngx::Instance> one = {ngx::Instance{}};
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(0, H, 1, 1, {}, {}, meshes, one); }),
+          "a frame of zero width is refused");
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 0, 1, {}, {}, meshes, one); }), "zero worlds are refused");
+    ngx::Instance wrong_mesh;
+    wrong_mesh.mesh = 3;
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, meshes, {wrong_mesh}); }),
+          "an instance of a mesh that does not exist is refused");
+    ngx::Instance flat;
+    flat.transform[10] = 0.0f;
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, meshes, {flat}); }),
+          "a singular instance transform is refused");
+    ngx::Mesh broken = box(1, 1, 1);
+    broken.indices[4] = 99;
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {broken}, one); }),
+          "a triangle index past the last vertex is refused");
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 2, 1, {}, {}, meshes, one, {1, 2}); }),
+          "a world holding more instances than the template is refused");
+}
+
+void gpu_checks() {
+    std::puts("worlds do not see one another");
+    {
+        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        std::vector<float> poses;
+        add_translation(poses, 0, 0, -8);
+        add_translation(poses, 0, 0, -3);
+        b.set_instance_transforms(poses.data());
+        std::vector<float> cams;
+        add_camera(cams);
+        add_camera(cams);
+        const ngx::HostBatchFrame f = b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH | ngx::OUTPUT_OBJECT_ID);
+        const size_t centre = (size_t)(H / 2) * W + W / 2, world1 = (size_t)W * H;
+        check(std::fabs(f.depth[centre] - 7.0f) < 0.02f && std::fabs(f.depth[world1 + centre] - 2.0f) < 0.02f,
+              "each camera sees its own world's box (front faces at 7 and 2)");
+        check(f.object_id[centre] == ngx::OBJECT_INSTANCE && f.object_id[0] == ngx::OBJECT_NONE, "object IDs: the box, and nothing in the corner");
+        const ngx::HostRayHits hits = b.cast_rays_host({0, 0, 0, 0, 0, 0}, {0, 0, -1, 0, 0, -1}, 1, 100.0f);
+        check(std::fabs(hits.distance[0] - 7.0f) < 1e-3f && std::fabs(hits.distance[1] - 2.0f) < 1e-3f,
+              "ray queries see only their own world as well");
+        const ngx::HostRayHits longer = b.cast_rays_host({0, 0, 0, 0, 0, 0}, {0, 0, -5, 0, 0, -0.25f}, 1, 100.0f);
+        check(std::fabs(longer.distance[0] - 7.0f) < 1e-3f && std::fabs(longer.distance[1] - 2.0f) < 1e-3f,
+              "a ray's distance does not depend on its direction's length");
+        const ngx::HostRayHits near = b.cast_rays_host({0, 0, 0, 0, 0, 0}, {0, 0, -1, 0, 0, -1}, 1, 6.9f);
+        check(std::isinf(near.distance[0]) && near.object_id[0] == ngx::OBJECT_NONE && std::fabs(near.distance[1] - 2.0f) < 1e-3f,
+              "a hit beyond max_distance is a miss");
+        check(throws<ngx::Unsupported>([&] { b.render(cams.data(), FOV, 1, ngx::OUTPUT_RGB); }), "the RGB picture is not drawn");
+    }
+
+    std::puts("one-instance worlds and a pile of coincident boxes");
+    {
+        ngx::BatchRenderer single(W, H, 4, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        std::vector<float> poses, cams;
+        for (int w = 0; w < 4; ++w) {
+            add_translation(poses, 0, 0, -4.0f - (float)w);
+            add_camera(cams);
+        }
+        single.set_instance_transforms(poses.data());
+        const ngx::HostBatchFrame f = single.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH);
+        bool ok = true;
+        for (int w = 0; w < 4; ++w) ok = ok && std::fabs(f.depth[(size_t)w * W * H + (H / 2) * W + W / 2] - (3.0f + (float)w)) < 0.02f;
+        check(ok, "a world whose tree is one leaf renders right (depths 3, 4, 5, 6)");
+
+        std::vector<ngx::Instance> pile(1000);
+        ngx::BatchRenderer many(W, H, 1, 1, {}, {}, {box(0.2f, 0.2f, 0.2f)}, pile);
+        std::vector<float> same;
+        for (int i = 0; i < 1000; ++i) add_translation(same, 0, 0, -5);
+        many.set_instance_transforms(same.data());
+        std::vector<float> cam;
+        add_camera(cam);
+        const ngx::HostBatchFrame g = many.render_host(cam.data(), FOV, 1, ngx::OUTPUT_DEPTH);
+        check(std::fabs(g.depth[(H / 2) * W + W / 2] - 4.8f) < 0.02f, "a thousand boxes in one place: the nearest face is 4.8 away");
+    }
+
+    std::puts("bad input after creation is refused and changes nothing");
+    {
+        ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        std::vector<float> good, cams;
+        add_translation(good, 0, 0, -6);
+        add_camera(cams);
+        b.set_instance_transforms(good.data());
+        const float before = b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH).depth[(H / 2) * W + W / 2];
+        std::vector<float> bad = good;
+        bad[0] = NAN;
+        check(throws<std::invalid_argument>([&] { b.set_instance_transforms(bad.data()); }), "a NaN in a transform is refused");
+        check(throws<std::invalid_argument>([&] { b.render(cams.data(), 0.0f, 1, ngx::OUTPUT_DEPTH); }), "a field of view of 0 is refused");
+        ngx::BodyBinding bind;
+        bind.bodies_per_world = 1;
+        bind.body_of_instance = {2};
+        bind.local_pose = {0, 0, 0, 0, 0, 0, 1};
+        bind.scale = {1, 1, 1};
+        check(throws<std::invalid_argument>([&] { b.bind_bodies(bind); }), "a body index past the last body is refused");
+        const float after = b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH).depth[(H / 2) * W + W / 2];
+        check(before == after && std::fabs(after - 5.0f) < 0.02f, "the scene is as it was");
+        ngx::BatchRenderer moved = std::move(b);
+        check(throws<std::logic_error>([&] { b.render(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH); }), "a moved-from renderer refuses work");
+        check(moved.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH).depth[(H / 2) * W + W / 2] == after, "the moved-to renderer works");
+    }
+
+    std::puts("poses from bodies");
+    {
+        // one body per world carrying the box with an offset of 1 along x and a 90 degree turn about y
+        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        ngx::BodyBinding bind;
+        bind.bodies_per_world = 1;
+        bind.body_of_instance = {0};
+        bind.local_pose = {1, 0, 0, 0, 0.70710678f, 0, 0.70710678f};
+        bind.scale = {1, 1, 2};
+        b.bind_bodies(bind);
+        const std::vector<float> bodies = {-1, 0, -9, 0, 0, 0, 1, -1, 0, -4, 0, 0, 0, 1};
+        b.set_body_poses(bodies.data());
+        std::vector<float> cams;
+        add_camera(cams);
+        add_camera(cams);
+        const ngx::HostBatchFrame f = b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH);
+        // the turn puts the box's local x axis (half-size 1) along the view axis and its scaled z (half-size 2) across it,
+        // and the offset moves it 1 along x onto the axis: its face toward the camera is 1 in front of the body
+        const size_t centre = (size_t)(H / 2) * W + W / 2, world1 = (size_t)W * H;
+        check(std::fabs(f.depth[centre] - 8.0f) < 0.02f && std::fabs(f.depth[world1 + centre] - 3.0f) < 0.02f,
+              "a body's pose, the instance's offset, rotation and scale compose (front faces at 8 and 3)");
+    }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const std::string mode = argc > 1 ? argv[1] : "gpu";
+    cpu_checks();
+    if (mode == "gpu") gpu_checks();
+    std::printf("%s: %d failure%s\n", mode.c_str(), failures, failures == 1 ? "" : "s");
+    return failures == 0 ? 0 : 1;
+}

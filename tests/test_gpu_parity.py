@@ -13,6 +13,11 @@ from .helpers import compare_images, compare_rays, world_subtree
 
 pytestmark = pytest.mark.gpu
 
+# The three walks: both levels as a packet; the packet walk with every mesh's bottom level walked lane by lane;
+# both levels lane by lane.
+WALKS = {"packet": dict(packet=True, lane_min_triangles=1 << 30), "hybrid": dict(packet=True, lane_min_triangles=0),
+         "lanes": dict(packet=False)}
+
 
 def _say(*a):
     print(*a, flush=True)
@@ -90,13 +95,13 @@ def test_morton_trees_see_what_the_sah_trees_see(gpu, oracle, worlds, per):
     assert hits > 1000 and diff <= a_id.size // 2000 and worst < 0.01
 
 
-@pytest.mark.parametrize("walk", ["packet", "lanes"])
+@pytest.mark.parametrize("walk", list(WALKS))
 @pytest.mark.parametrize("index,worlds", [(0, None), (1, 32), (2, 16), (4, 16)])
 def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds, walk):
     _say(f"render parity: config {index}, worlds {worlds}, {walk} walk")
     cfg = workloads.config(index, worlds=worlds)
     r = workloads.make_renderer(cfg)
-    r.set_trace_config(packet=walk == "packet")
+    r.set_trace_config(**WALKS[walk])
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
     rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, f"config {index}")
@@ -104,14 +109,14 @@ def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds, walk):
     assert rep["hits"] > 0.05 * rep["pixels"]
 
 
-@pytest.mark.parametrize("walk", ["packet", "lanes"])
 @pytest.mark.parametrize("tile_w,pair_limit,persistent", [(16, 16, True), (32, 1, False), (8, 64, True)])
 def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_limit, persistent):
     _say(f"render parity: config 2, 16 worlds, tile_w {tile_w}, pair_limit {pair_limit}, persistent {persistent}")
     cfg = workloads.config(2, worlds=16)
     r = workloads.make_renderer(cfg)
     r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
-    assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent))
+    assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent),
+                                  lane_min_triangles=1000)
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
     _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 2"))
@@ -133,11 +138,12 @@ def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, 
     compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 1 tuned")
 
 
+@pytest.mark.parametrize("walk", list(WALKS))
 def test_ray_queries_match_the_oracle(gpu, oracle, walk):
     _say(f"ray parity: config 3, 8 worlds x 4096 rays, {walk} walk")
     cfg = workloads.config(3, worlds=8, rays_per_world=4096)
     r = workloads.make_renderer(cfg)
-    r.set_trace_config(packet=walk == "packet")
+    r.set_trace_config(**WALKS[walk])
     out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
     st = forward(cfg, oracle, outputs=())
     ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
@@ -147,14 +153,14 @@ def test_ray_queries_match_the_oracle(gpu, oracle, walk):
     assert rep["hits"] > 0
 
 
-@pytest.mark.parametrize("walk", ["packet", "lanes"])
+@pytest.mark.parametrize("walk", list(WALKS))
 def test_rays_of_uneven_worlds_per_wave(gpu, oracle, walk):
     _say(f"ray parity: 37 rays per world, so waves straddle worlds, {walk} walk")
     cfg = workloads.config(3, worlds=8, rays_per_world=37 * 64)
     o = cfg["origins"][:, :37]
     d = cfg["directions"][:, ::64][:, :37]
     r = workloads.make_renderer(cfg)
-    r.set_trace_config(packet=walk == "packet")
+    r.set_trace_config(**WALKS[walk])
     out = r.cast_rays(o, d, 100.0)
     st = forward(cfg, oracle, outputs=())
     ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
