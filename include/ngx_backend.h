@@ -201,6 +201,11 @@ ngx_status ngx_debug_keep_build(ngx_ctx *ctx, int32_t keep);
 ngx_status ngx_debug_read_build(ngx_ctx *ctx, void *nodes, int32_t *roots, float *inv, float *matrices, uint64_t *keys,
                                 uint64_t *sorted_keys, int32_t *sorted_ids, float *aabb, float *world_bounds);
 
+/* The per-lane bottom-level layout of one mesh, no device needed: skip_nodes (capacity 2 x triangles) x 64 bytes
+ * (SkipNode: lo xyz, a, hi xyz, meta, escape[8]) in the same node order as ngx_build_blas_host's nodes. */
+ngx_status ngx_build_skip_host(const float *vertices, int32_t vertex_count, const uint32_t *indices, int32_t triangle_count,
+                               void *skip_nodes, int32_t *node_count);
+
 /* Checks, on device device_id, the reciprocal the tracer uses for 1/x (the hardware estimate and one
  * fused Newton step) against IEEE division for all 2^32 float bit patterns: mismatches among
  * 2^-126 <= |x| < 2^126 (where the tracer uses it), among all finite nonzero x, and one mismatching

@@ -922,6 +922,21 @@ ngx_status ngx_debug_read_build(ngx_ctx *ctx, void *nodes, int32_t *roots, float
     });
 }
 
+ngx_status ngx_build_skip_host(const float *vertices, int32_t vertex_count, const uint32_t *indices, int32_t triangle_count,
+                               void *skip_nodes, int32_t *node_count) {
+    return guarded([&] {
+        require(vertices && indices && skip_nodes && node_count, "arrays must not be null");
+        require(triangle_count > 0 && vertex_count > 0, "a mesh needs triangles");
+        for (int i = 0; i < 3 * triangle_count; ++i) require(indices[i] < (uint32_t)vertex_count, "index out of range");
+        const int32_t vo[2] = {0, vertex_count}, io[2] = {0, 3 * triangle_count};
+        const ngx::host::MeshInput mi{1, vertices, vo, indices, io, nullptr, nullptr};
+        ngx::host::BuiltMeshes built;
+        ngx::host::build_meshes(mi, built);
+        std::memcpy(skip_nodes, built.skip.data(), sizeof(ngx::SkipNode) * built.skip.size());
+        *node_count = (int32_t)built.skip.size();
+    });
+}
+
 ngx_status ngx_build_blas_host(const float *vertices, int32_t vertex_count, const uint32_t *indices, int32_t triangle_count,
                                void *nodes, int32_t *node_count, float *triangles, uint32_t *order) {
     return guarded([&] {

@@ -96,3 +96,62 @@ def test_bad_scenes_are_rejected_before_any_device_work(kw, match):
 def test_rgb_and_textures_are_refused_plainly():
     with pytest.raises(NotImplementedError, match="RGB"):
         ng.BatchRenderer(**_scene(textures=[object()]))
+
+
+@pytest.mark.parametrize("mesh", [ng.make_icosphere(1.0, 2), ng.make_box((0.5, 0.5, 0.5)), ng.make_grid(40, 25, 2.0, 1.25, 0.05, 7),
+                                  ng.make_grid(3, 2, 1.0, 1.0, 0.0, 1)], ids=["icosphere", "box", "grid", "flat-grid"])
+def test_skip_layout_is_an_ordered_walk_of_the_reference_tree(mesh):
+    """The per-lane walk's layout (ngx_build_skip_host) against the tree it is made from: same boxes and children;
+    for every octant, following first children into hits and escapes out of misses visits every node exactly once
+    in pre-order, each interior node's children in the octant's order (the child lower along the node's axis
+    first when the direction is positive along it), and a node's escape is the node right after its subtree."""
+    from neraulgraphics_amd.layouts import SKIP_NODE
+    lib = _lib.load()
+    nodes, _, _ = _blas(mesh)
+    T = mesh.triangle_count
+    skip = np.zeros(2 * T, SKIP_NODE)
+    n = ctypes.c_int32()
+    _lib.check(lib.ngx_build_skip_host(mesh.vertices.ctypes.data, len(mesh.vertices), mesh.indices.ctypes.data, T,
+                                       skip.ctypes.data, ctypes.byref(n)))
+    skip = skip[:n.value]
+    assert n.value == len(nodes)
+    assert np.array_equal(skip["lo"], nodes["lo"]) and np.array_equal(skip["hi"], nodes["hi"]) and np.array_equal(skip["a"], nodes["a"])
+    leaf = nodes["count"] > 0
+    assert np.array_equal(skip["meta"][leaf], nodes["count"][leaf]) and np.all(skip["meta"][~leaf] < 0)
+    for o in range(8):
+        positive = [(o >> k) & 1 for k in range(3)]
+
+        def children(i):
+            code = -int(skip["meta"][i]) - 1
+            axis, flip = code >> 1, code & 1
+            a = int(skip["a"][i])
+            centre = lambda c: nodes["lo"][c][axis] + nodes["hi"][c][axis]
+            assert (centre(a) > centre(a + 1)) == bool(flip), "flip records which child is higher along the axis"
+            lower_first = positive[axis] == 1
+            return (a, a + 1) if lower_first != bool(flip) else (a + 1, a)
+
+        # the pre-order the walk must produce when every box is entered
+        want, stack = [], [0]
+        while stack:
+            i = stack.pop()
+            want.append(i)
+            if not leaf[i]:
+                first, second = children(i)
+                stack += [second, first]
+        assert sorted(want) == list(range(len(nodes)))
+        got, cur = [], 0
+        while cur >= 0:
+            got.append(cur)
+            cur = children(cur)[0] if not leaf[cur] else int(skip["escape"][cur][o])
+        assert got == want, f"octant {o}: the walk with every box entered is not the pre-order"
+        # a node's escape is the first node after its subtree
+        pos = {v: k for k, v in enumerate(want)}
+        size = np.ones(len(nodes), np.int64)
+        for i in reversed(want):
+            if not leaf[i]:
+                a = int(skip["a"][i])
+                size[i] = 1 + size[a] + size[a + 1]
+        for i in range(len(nodes)):
+            after = pos[i] + size[i]
+            expect = want[after] if after < len(want) else -1
+            assert int(skip["escape"][i][o]) == expect, f"octant {o}: node {i}'s escape"
