@@ -172,8 +172,8 @@ const char *ngx_last_error(void);
  *              renumbered so each XCD works through one contiguous range of tiles (one L2 per XCD);
  *  packet      1: the wavefront walks its 64 rays as a packet (the default); 0: each lane walks its own ray;
  *  pair_limit  packet walk: (ray, triangle) pairs queued before they are tested together, 1..64;
- *  persistent  1: a fixed grid of 8 workgroups per compute unit takes tiles from per-XCD counters instead
- *              of one wavefront per tile;
+ *  persistent  1 (the default): a fixed grid of 8 workgroups per compute unit takes tiles from per-XCD
+ *              counters; 0: one wavefront per tile, dealt by the dispatcher;
  *  lane_min_triangles  packet walk: the bottom level of meshes with at least this many triangles is walked
  *              lane by lane (each ray its own path) instead of as a packet (default 1000). */
 typedef struct {

@@ -148,7 +148,7 @@ class BatchRenderer:
     def synchronize(self):
         _lib.check(self._lib.ngx_synchronize(self._h))
 
-    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=False, lane_min_triangles=1000):
+    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000):
         """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
         default) or per-lane (False) walks, (ray, triangle) pairs batched before testing (1..64), the
         persistent fixed-grid launch, and the triangle count from which a mesh's bottom level is walked lane by

@@ -151,7 +151,7 @@ struct ngx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     hipStream_t last_stream = nullptr;
-    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 0, lane_min_tris = 1000;
+    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000;
     int persistent_blocks = 0;          // 8 workgroups of 4 wavefronts per compute unit
     int *d_tile_counters = nullptr;
 

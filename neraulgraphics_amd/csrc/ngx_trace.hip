@@ -35,47 +35,6 @@ struct WaveStats {
     __device__ __forceinline__ void add(const int k, const int v) { c[k] += v; }
 };
 
-// A wave-uniform stack of (node reference, lane mask) entries, entry k held in lane k & 63 of register
-// set k >> 6: a push is a per-lane select, a pop a readlane.
-template <int SETS>
-struct WaveStack {
-    int ref[SETS];
-    uint32_t lo[SETS], hi[SETS];
-    int sp;
-    __device__ __forceinline__ void init() {
-        sp = 0;
-        for (int s = 0; s < SETS; ++s) { ref[s] = 0; lo[s] = 0; hi[s] = 0; }
-    }
-    __device__ __forceinline__ bool full() const { return sp >= 64 * SETS; }
-    __device__ __forceinline__ void push(const int r, const uint64_t m) {
-        const int set = sp >> 6, slot = sp & 63;
-        const bool mine = lane_id() == slot;
-#pragma unroll
-        for (int s = 0; s < SETS; ++s)
-            if (s == set) {
-                ref[s] = mine ? r : ref[s];
-                lo[s] = mine ? (uint32_t)m : lo[s];
-                hi[s] = mine ? (uint32_t)(m >> 32) : hi[s];
-            }
-        ++sp;
-    }
-    __device__ __forceinline__ void pop(int &r, uint64_t &m) {
-        --sp;
-        const int set = sp >> 6, slot = sp & 63;
-        int rr = 0;
-        uint32_t l = 0, h = 0;
-#pragma unroll
-        for (int s = 0; s < SETS; ++s)
-            if (s == set) {
-                rr = __builtin_amdgcn_readlane(ref[s], slot);
-                l = (uint32_t)__builtin_amdgcn_readlane((int)lo[s], slot);
-                h = (uint32_t)__builtin_amdgcn_readlane((int)hi[s], slot);
-            }
-        r = rr;
-        m = ((uint64_t)h << 32) | l;
-    }
-};
-
 struct Wide {
     float l_lo[3], l_hi[3], r_lo[3], r_hi[3];
     int left, right, lcount, rcount;
@@ -109,6 +68,10 @@ struct PairLds {
     unsigned long long best[64];   // per ray lane: distance bits << 32 | pair number
     float2 uv[64];                 // per pair: barycentrics of its hit
     float2 hit_uv[64];             // per ray lane: barycentrics of its closest hit so far
+    int hit_inst[64], hit_tri[64]; // per ray lane: instance and triangle of its closest hit so far (-1: none)
+    // the packet walk's two wave-uniform stacks (top level, bottom level): node reference and lane mask
+    int st_ref[2][64];
+    uint32_t st_lo[2][64], st_hi[2][64];
 };
 
 // Orders a wavefront's LDS accesses: a wavefront's LDS operations are performed in order, so a
@@ -122,6 +85,32 @@ __device__ __forceinline__ void wave_lds_order() {
 __device__ __forceinline__ int rank_in(const uint64_t mask) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
+
+// A wave-uniform stack of (node reference, lane mask) in the wavefront's LDS: one lane writes a push, every
+// lane reads a pop (the same address for all, so it is broadcast) and the value is taken as uniform.
+struct LdsStack {
+    int *ref;
+    uint32_t *lo, *hi;
+    int sp;
+    __device__ __forceinline__ LdsStack(PairLds *L, const int level) : ref(L->st_ref[level]), lo(L->st_lo[level]), hi(L->st_hi[level]), sp(0) {}
+    // Depth bounds keep sp below 64 (a top-level tree has fewer than 64 levels below a world's root, a bottom-level
+    // tree at most 60); the guard only keeps a malformed tree from writing past the stack.
+    __device__ __forceinline__ void push(const int r, const uint64_t m) {
+        if (sp >= 64) return;
+        if (lane_id() == 0) {
+            ref[sp] = r;
+            lo[sp] = (uint32_t)m;
+            hi[sp] = (uint32_t)(m >> 32);
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int &r, uint64_t &m) {
+        --sp;
+        wave_lds_order();
+        r = uniform(ref[sp]);
+        m = ((uint64_t)uniform_u(hi[sp]) << 32) | uniform_u(lo[sp]);
+    }
+};
 
 // Tests the queued pairs of instance inst, whose mesh-space ray is (lo, ld) in each ray's lane.
 template <class Stats>
@@ -152,9 +141,10 @@ __device__ __forceinline__ void flush_pairs(const TraceScene &S, PairLds *L, int
     const unsigned p = (unsigned)b;
     if (p != 0xffffffffu) {
         tmax = __uint_as_float((unsigned)(b >> 32));
-        h.inst = inst;
-        h.tri = L->pair[p] >> 6;
-        L->hit_uv[lane] = L->uv[p];  // kept in LDS, not registers, until the walk is over
+        // the hit is kept in LDS, not registers, until the walk is over
+        L->hit_inst[lane] = inst;
+        L->hit_tri[lane] = L->pair[p] >> 6;
+        L->hit_uv[lane] = L->uv[p];
     }
     wave_lds_order();
     pairs = 0;
@@ -304,8 +294,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
         return;
     }
-    WaveStack<1> st;
-    st.init();
+    LdsStack st(L, 1);
     int cur = root_ref;
     uint64_t cm = root_mask;
     while (true) {
@@ -379,8 +368,7 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
     // One entry is pushed per level at most. Below a world's root every level fixes at least one more bit
     // of a Morton code (30 bits) or, among equal codes, of the leaves' positions (at most ceil(log2 n) + 1
     // bits over n consecutive positions): fewer than 64 levels for any n < 2^31, so 64 entries do.
-    WaveStack<1> st;
-    st.init();
+    LdsStack st(L, 0);
     int cur = root;
     uint64_t cm = live;
     while (true) {
@@ -629,10 +617,15 @@ __device__ __forceinline__ int scene_hit(const TraceScene &S, PairLds *L, const 
     h.inst = -1;
     h.tri = -1;
     if (WALK != WALK_LANES) {
+        const int lane = lane_id();
+        L->hit_inst[lane] = -1;
+        wave_lds_order();
         trace_world<WALK == WALK_HYBRID>(S, L, world, o, d, live, best, h, stats);
         wave_lds_order();
+        h.inst = L->hit_inst[lane];
         if (h.inst >= 0) {
-            const float2 uv = L->hit_uv[lane_id()];
+            h.tri = L->hit_tri[lane];
+            const float2 uv = L->hit_uv[lane];
             h.u = uv.x;
             h.v = uv.y;
         }
@@ -763,19 +756,23 @@ __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 8 : WALK == WALK_HYBRID 
 
 // Persistent: a fixed grid whose wavefronts take tiles until none are left. The tiles are cut into eight
 // contiguous ranges, one per XCD (the dispatcher deals workgroups round-robin over the XCDs, so workgroup b
-// runs on XCD b mod 8); each range has its own counter, so the worlds of a range stay in one XCD's L2.
-// Every wavefront leaves the loop once its range's counter passes the range's end.
+// runs on XCD b mod 8); each range has its own counter, so the worlds of a range stay in one XCD's L2. A
+// wavefront takes PERSISTENT_CHUNK consecutive tiles per counter increment, so cheap tiles do not queue on
+// the counter. Every wavefront leaves the loop once its range's counter passes the range's end.
+constexpr int PERSISTENT_CHUNK = 8;
+
 template <int WALK>
-__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 8 : WALK == WALK_HYBRID ? 6 : 1) void render_persistent_kernel(const RenderArgs A) {
+__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? 6 : 1) void render_persistent_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const int xcd = blockIdx.x & 7;
     const long long first = A.total_tiles * xcd / 8, count = A.total_tiles * (xcd + 1) / 8 - first;
     while (true) {
         int t = 0;
-        if (lane_id() == 0) t = atomicAdd(&A.tile_counters[xcd], 1);
+        if (lane_id() == 0) t = atomicAdd(&A.tile_counters[xcd], PERSISTENT_CHUNK);
         t = __builtin_amdgcn_readlane(t, 0);
         if (t >= count) break;
-        render_tile<WALK>(A, L, first + t);
+        const int end = (int)min((long long)t + PERSISTENT_CHUNK, count);
+        for (int k = t; k < end; ++k) render_tile<WALK>(A, L, first + k);
     }
 }
 
@@ -1064,7 +1061,7 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
 hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int persistent_blocks) {
     if (a.total_tiles <= 0) return hipSuccess;
     if (a.tile_counters && persistent_blocks > 0) {
-        if (a.total_tiles >= INT32_MAX || persistent_blocks % 8 != 0) return hipErrorInvalidValue;
+        if (a.total_tiles >= INT32_MAX / 2 || persistent_blocks % 8 != 0) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 64, 0, stream>>>(a.tile_counters);
 #define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_blocks, 256, 0, stream>>>(a)
         NGX_BY_WALK(a.walk, NGX_LAUNCH)
