@@ -175,9 +175,11 @@ const char *ngx_last_error(void);
  *  persistent  1 (the default): a fixed grid of 8 workgroups per compute unit takes tiles from per-XCD
  *              counters; 0: one wavefront per tile, dealt by the dispatcher;
  *  lane_min_triangles  packet walk: the bottom level of meshes with at least this many triangles is walked
- *              lane by lane (each ray its own path) instead of as a packet (default 1000). */
+ *              lane by lane (each ray its own path) instead of as a packet (default 1000);
+ *  sort_rays   ray queries of 1024 to 262,144 rays a world: 1 (the default) traces each world's rays in an order
+ *              of their directions, so a wavefront's rays are neighbours; results are in the caller's order. */
 typedef struct {
-    int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles;
+    int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles, sort_rays;
 } ngx_trace_config;
 ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config);
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);

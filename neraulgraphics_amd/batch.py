@@ -148,13 +148,14 @@ class BatchRenderer:
     def synchronize(self):
         _lib.check(self._lib.ngx_synchronize(self._h))
 
-    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000):
+    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000,
+                         sort_rays=True):
         """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
         default) or per-lane (False) walks, (ray, triangle) pairs batched before testing (1..64), the
-        persistent fixed-grid launch, and the triangle count from which a mesh's bottom level is walked lane by
-        lane inside the packet walk."""
+        persistent fixed-grid launch, the triangle count from which a mesh's bottom level is walked lane by lane
+        inside the packet walk, and whether ray queries trace each world's rays in an order of their directions."""
         c = _lib.TraceConfig(int(tile_w), int(bool(xcd_remap)), int(bool(packet)), int(pair_limit), int(bool(persistent)),
-                             int(lane_min_triangles))
+                             int(lane_min_triangles), int(bool(sort_rays)))
         _lib.check(self._lib.ngx_set_trace_config(self._h, ctypes.byref(c)))
 
     @property

@@ -137,6 +137,8 @@ struct ngx_ctx {
     long long ray_in_cap = 0;
     float *d_hit_dist = nullptr, *d_hit_normal = nullptr;
     int32_t *d_hit_id = nullptr;
+    uint32_t *d_ray_keys = nullptr;     // the direction order's scratch: 2 x rays
+    int *d_ray_vals = nullptr;
     long long ray_cap = 0;
     float *d_partial = nullptr;
     size_t partial_cap = 0;
@@ -151,7 +153,7 @@ struct ngx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     hipStream_t last_stream = nullptr;
-    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000;
+    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000, sort_rays = 1;
     int persistent_blocks = 0;          // 8 workgroups of 4 wavefronts per compute unit
     int *d_tile_counters = nullptr;
 
@@ -164,7 +166,7 @@ struct ngx_ctx {
                            d_dbg_world_bounds};
         for (float *p : floats) if (p) (void)hipFree(p);
         void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
-                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, d_skip, big.key[0], big.key[1], big.val[0], big.val[1],
+                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, d_skip, d_ray_keys, d_ray_vals, big.key[0], big.key[1], big.val[0], big.val[1],
                           big.box, big.node_box, big.left, big.right, big.parent, big.arrivals, big.bounds, big.tile_counts};
         for (void *p : others) if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -646,6 +648,8 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
             device_alloc(ctx->d_hit_dist, (size_t)count);
             device_alloc(ctx->d_hit_normal, 3 * (size_t)count);
             device_alloc(ctx->d_hit_id, (size_t)count);
+            device_alloc(ctx->d_ray_keys, 2 * (size_t)count);
+            device_alloc(ctx->d_ray_vals, 2 * (size_t)count);
             ctx->ray_cap = count;
         }
         ngx::RayArgs a{};
@@ -661,6 +665,11 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
         a.walk = ctx->walk();
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
+        if (ctx->sort_rays && a.walk != 0 && rays_per_world >= 1024 && rays_per_world <= ngx::RAY_ORDER_MAX_RAYS) {
+            int *order = nullptr;
+            HIP_CHECK(ngx::launch_ray_order(d, rays_per_world, ctx->worlds, ctx->d_ray_keys, ctx->d_ray_vals, &order, s));
+            a.order = order;
+        }
         HIP_CHECK(ngx::launch_rays(a, s));
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
         ctx->timed = timed;
@@ -807,20 +816,22 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
         ctx->persistent = c.persistent ? 1 : 0;
         require(c.lane_min_triangles >= 0, "lane_min_triangles must not be negative");
         ctx->lane_min_tris = c.lane_min_triangles;
+        ctx->sort_rays = c.sort_rays ? 1 : 0;
     });
 }
 
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config) {
     return guarded([&] {
         require(ctx != nullptr && config != nullptr, "context and config must not be null");
-        *config = ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent, ctx->lane_min_tris};
+        *config = ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent, ctx->lane_min_tris,
+                                   ctx->sort_rays};
     });
 }
 
 ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
                         void *stream, ngx_trace_config *best, float *best_ms) {
     const ngx_trace_config start = ctx ? ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent,
-                                                          ctx->lane_min_tris}
+                                                          ctx->lane_min_tris, ctx->sort_rays}
                                        : ngx_trace_config{};
     const ngx_status st = guarded([&] {
         require(ctx != nullptr && cams12 != nullptr, "context and cameras must not be null");
@@ -832,8 +843,8 @@ ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, fl
         for (const int tw : {8, 16, 32}) {
             for (const int lanes : {never, 1000, 300, 0})
                 for (const int limit : {32, 64})
-                    for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers, lanes});
-            if (tw <= 16) cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never});
+                    for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers, lanes, start.sort_rays});
+            if (tw <= 16) cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never, start.sort_rays});
         }
         const hipStream_t s = static_cast<hipStream_t>(stream);
         hipEvent_t e0 = nullptr, e1 = nullptr;

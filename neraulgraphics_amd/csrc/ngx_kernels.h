@@ -90,6 +90,7 @@ struct RayArgs {
     float *normal;
     int32_t *object_id;
     int walk;
+    const int *order;          // null, or per world the order to trace its rays in (indices within the world)
 };
 
 // The pose backward of depth: dL/d(body pose) for every world, through each pixel's fixed hit triangle.
@@ -149,6 +150,12 @@ struct BigBuildScratch {
     int *bounds;               // [6]: centre bounds as ordered ints
     int *tile_counts;          // [16 * ceil(capacity / 1024)]
 };
+
+// Per world, a stable order of its rays by direction cell, for coherent wavefronts: keys and vals are scratch
+// of 2 x rays_per_world x worlds each; *order points into vals.
+constexpr int RAY_ORDER_MAX_RAYS = 262144;
+hipError_t launch_ray_order(const float *dirs, int rays_per_world, int worlds, uint32_t *keys, int *vals, int **order,
+                            hipStream_t stream);
 
 // Builds world `world` (instances first .. first+n-1, n > 1) with grid-wide kernels.
 hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &s, int world, int first, int n, hipStream_t stream);

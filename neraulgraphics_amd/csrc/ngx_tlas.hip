@@ -638,6 +638,118 @@ __global__ __launch_bounds__(256) void big_write_kernel(const TlasBuildArgs A, c
     write_interior(A, i, n, a, root_index, S.left[i], S.right[i], S.parent[i], box, id);
 }
 
+// ------------------------------------------------------------------ ray order for ray queries --
+//
+// A query's rays come in the caller's order; a LiDAR's consecutive rays fan out over many degrees, so a
+// wavefront of 64 of them shares little of either tree. Each world's rays are put in a stable order of a
+// 12-bit direction cell (octahedral map of the direction, 64 x 64 cells, Morton order) by one workgroup per
+// world: three passes of the same 4-bit stable radix sort, over chunks of 1024 rays. The tracer then gives
+// a wavefront 64 consecutive rays of that order; results still go to each ray's own place.
+constexpr int RAY_ORDER_MAX_CHUNKS = RAY_ORDER_MAX_RAYS / 1024;
+
+__device__ __forceinline__ uint32_t spread6(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 6; ++b) r |= ((v >> b) & 1u) << (2 * b);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t direction_cell(const float x, const float y, const float z) {
+    const float s = fabsf(x) + fabsf(y) + fabsf(z);
+    if (!(s > 0.0f) || !isfinite(s)) return 4095u;
+    float u = x / s, v = y / s;
+    if (z < 0.0f) {
+        const float uu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+        const float vv = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+        u = uu;
+        v = vv;
+    }
+    const uint32_t qu = (uint32_t)fminf(63.0f, fmaxf(0.0f, (u * 0.5f + 0.5f) * 64.0f));
+    const uint32_t qv = (uint32_t)fminf(63.0f, fmaxf(0.0f, (v * 0.5f + 0.5f) * 64.0f));
+    return (spread6(qu) << 1) | spread6(qv);
+}
+
+__global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, const int R, uint32_t *ka, uint32_t *kb, int *va, int *vb) {
+    constexpr int NW = 16;
+    __shared__ int counts[16 * RAY_ORDER_MAX_CHUNKS];
+    __shared__ int hist[16 * NW];
+    __shared__ int wave_tot[NW];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * R;
+    const int chunks = (R + 1023) / 1024;
+    for (int i = tid; i < R; i += 1024) {
+        const float *p = dirs + 3 * (base + i);
+        ka[base + i] = direction_cell(p[0], p[1], p[2]);
+        va[base + i] = i;
+    }
+    __syncthreads();
+    uint32_t *ks = ka, *kd = kb;
+    int *vs = va, *vd = vb;
+    for (int pass = 0; pass < 3; ++pass) {
+        const int shift = 4 * pass;
+        for (int c = 0; c < chunks; ++c) {
+            const int i = c * 1024 + tid;
+            const bool has = i < R;
+            if (tid < 16 * NW) hist[tid] = 0;
+            __syncthreads();
+            uint32_t digit;
+            int rank, cnt;
+            tile_digit(has, has ? ks[base + i] : 0u, shift, digit, rank, cnt);
+            if (has && rank == 0) hist[digit * NW + wave] = cnt;
+            __syncthreads();
+            if (tid < 16) {
+                int sum = 0;
+                for (int wv = 0; wv < NW; ++wv) sum += hist[tid * NW + wv];
+                counts[tid * chunks + c] = sum;
+            }
+            __syncthreads();
+        }
+        // exclusive scan of the counts in digit-major order
+        const int total = 16 * chunks;
+        if (tid == 0) carry = 0;
+        __syncthreads();
+        for (int b0 = 0; b0 < total; b0 += 1024) {
+            const int cin = carry;
+            const int idx = b0 + tid;
+            const int x = idx < total ? counts[idx] : 0;
+            int incl = x;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            if (lane == 63) wave_tot[wave] = incl;
+            __syncthreads();
+            for (int wv = 0; wv < wave; ++wv) incl += wave_tot[wv];
+            __syncthreads();
+            if (idx < total) counts[idx] = cin + incl - x;
+            if (tid == 1023) carry = cin + incl;
+            __syncthreads();
+        }
+        for (int c = 0; c < chunks; ++c) {
+            const int i = c * 1024 + tid;
+            const bool has = i < R;
+            const uint32_t k = has ? ks[base + i] : 0u;
+            const int v = has ? vs[base + i] : 0;
+            if (tid < 16 * NW) hist[tid] = 0;
+            __syncthreads();
+            uint32_t digit;
+            int rank, cnt;
+            tile_digit(has, k, shift, digit, rank, cnt);
+            if (has && rank == 0) hist[digit * NW + wave] = cnt;
+            __syncthreads();
+            block_exclusive_scan(hist, 16 * NW, wave_tot);
+            if (has) {
+                const int dest = counts[digit * chunks + c] + hist[digit * NW + wave] - hist[digit * NW] + rank;
+                kd[base + dest] = k;
+                vd[base + dest] = v;
+            }
+            __syncthreads();
+        }
+        uint32_t *tk = ks; ks = kd; kd = tk;
+        int *tv = vs; vs = vd; vd = tv;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_tlas_build(const TlasBuildArgs &a, const int max_count, hipStream_t stream) {
@@ -667,6 +779,15 @@ hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &
     big_karras_kernel<<<blocks, 256, 0, stream>>>(a, s, world, first, n);
     big_refit_kernel<<<blocks, 256, 0, stream>>>(s, n);
     big_write_kernel<<<blocks, 256, 0, stream>>>(a, s, world, first, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_ray_order(const float *dirs, const int rays_per_world, const int worlds, uint32_t *keys, int *vals, int **order,
+                            hipStream_t stream) {
+    if (rays_per_world <= 0 || rays_per_world > RAY_ORDER_MAX_RAYS || worlds <= 0) return hipErrorInvalidValue;
+    const size_t n = (size_t)rays_per_world * worlds;
+    ray_order_kernel<<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals, vals + n);
+    *order = vals + n;  // three passes end in the second buffers
     return hipGetLastError();
 }
 

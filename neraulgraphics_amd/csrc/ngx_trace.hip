@@ -818,9 +818,11 @@ __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     const long long base = ((long long)blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6)) * 64;
     if (base >= A.count) return;
     const int lane = lane_id();
-    const long long i = base + lane;
-    const bool valid = i < A.count;
-    const int world = valid ? (int)(i / A.rays_per_world) : -1;
+    const long long slot = base + lane;
+    const bool valid = slot < A.count;
+    const int world = valid ? (int)(slot / A.rays_per_world) : -1;
+    // the ray this lane traces: the slot itself, or the world's ray at that place in the direction order
+    const long long i = valid && A.order ? (long long)world * A.rays_per_world + A.order[slot] : slot;
     float3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
     bool ok = false;
     if (valid) {

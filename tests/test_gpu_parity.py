@@ -116,7 +116,7 @@ def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_l
     r = workloads.make_renderer(cfg)
     r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
     assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent),
-                                  lane_min_triangles=1000)
+                                  lane_min_triangles=1000, sort_rays=1)
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
     _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 2"))
@@ -138,12 +138,12 @@ def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, 
     compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 1 tuned")
 
 
-@pytest.mark.parametrize("walk", list(WALKS))
-def test_ray_queries_match_the_oracle(gpu, oracle, walk):
-    _say(f"ray parity: config 3, 8 worlds x 4096 rays, {walk} walk")
+@pytest.mark.parametrize("walk,sort_rays", [("packet", True), ("packet", False), ("hybrid", True), ("lanes", False)])
+def test_ray_queries_match_the_oracle(gpu, oracle, walk, sort_rays):
+    _say(f"ray parity: config 3, 8 worlds x 4096 rays, {walk} walk, direction order {sort_rays}")
     cfg = workloads.config(3, worlds=8, rays_per_world=4096)
     r = workloads.make_renderer(cfg)
-    r.set_trace_config(**WALKS[walk])
+    r.set_trace_config(**WALKS[walk], sort_rays=sort_rays)
     out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
     st = forward(cfg, oracle, outputs=())
     ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
