@@ -53,7 +53,7 @@ enum {
 /* Memory (include/neraulgraphics/renderer.h:49). */
 typedef enum { NGX_MEM_HOST = 0, NGX_MEM_DEVICE = 1 } ngx_memory;
 
-/* Output bits (include/neraulgraphics/renderer.h:53-58). NGX_OUTPUT_RGB is not produced by this backend. */
+/* Output bits (include/neraulgraphics/renderer.h:53-58). */
 enum { NGX_OUTPUT_RGB = 1, NGX_OUTPUT_DEPTH = 2, NGX_OUTPUT_NORMAL = 4, NGX_OUTPUT_OBJECT_ID = 8 };
 
 /* Object IDs (include/neraulgraphics/scene.h:19-22). */
@@ -90,7 +90,7 @@ typedef struct {
     int32_t memory;     /* ngx_memory: always NGX_MEM_DEVICE */
     int32_t device_id;
     int32_t views, width, height;
-    const uint8_t *rgb; /* always NULL */
+    const uint8_t *rgb; /* views x H x W x 3, or NULL when not asked for */
     const float *depth;
     const float *normal;
     const int32_t *object_id;
@@ -105,6 +105,43 @@ typedef struct {
     const float *normal;
     const int32_t *object_id;
 } ngx_ray_hits;
+
+/* ---- the RGB picture (the classic plane: direct light from sphere lights with soft shadows, mirror
+ *      reflections, an ambient term and a sky gradient; src/render_core.h:146-283) ---- */
+#define NGX_MAX_LIGHTS 8
+
+/* The classic fields of a material (include/neraulgraphics/scene.h:35-40). */
+typedef struct {
+    float diffuse_color[3];
+    float diffuse_albedo, specular_albedo, reflectivity, specular_exponent;
+} ngx_material;
+
+/* include/neraulgraphics/scene.h:82-92: a sphere light (radius 0: a point), its falloff distance (0: none),
+ * and a spotlight cone along direction of half-angle spot_angle (0: no cone), fading over its outer
+ * spot_softness. */
+typedef struct {
+    float position[3];
+    float intensity, radius, falloff;
+    float direction[3];
+    float spot_angle, spot_softness;
+    float color[3];
+} ngx_light;
+
+/* Everything the picture needs beyond the geometry: the materials of the template's instances (one per
+ * template entry, the same in every world), of the spheres and of the floor (its checker tint at
+ * floor_checker_scale), the lights, the ambient level, and the sky gradient (include/neraulgraphics/scene.h:109-121;
+ * the reference's defaults are up (0, 1, 0), up colour (0.10, 0.26, 0.60), down colour (0.35, 0.52, 0.70),
+ * intensity 1, ambient 0). */
+typedef struct {
+    const ngx_material *instance_materials;  /* instances_per_world */
+    const ngx_material *sphere_materials;    /* sphere_count */
+    ngx_material floor_material;
+    float floor_checker_scale;
+    int32_t light_count;                     /* <= NGX_MAX_LIGHTS */
+    const ngx_light *lights;
+    float ambient;
+    float sky_up[3], sky_color_up[3], sky_color_down[3], sky_intensity;
+} ngx_shading;
 
 /* make_cuda_backend (src/backend.h:88) with BatchRenderer's constructor checks (src/batch.cpp:47-75). */
 ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out);
@@ -123,8 +160,15 @@ ngx_status ngx_bind_bodies(ngx_ctx *ctx, int32_t bodies_per_world, const int32_t
  * worlds x bodies_per_world x 7 floats. Composed in FP64 as src/instances.cpp:137-164 does. */
 ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memory, void *stream);
 
+/* Materials, lights, ambient and sky for the RGB picture: the reference takes them with the scene
+ * (src/backend.h:19-34, :74-80); here they are set after creation, before the first RGB render
+ * (NGX_BAD_STATE without them). Host arrays; checked before anything changes. */
+ngx_status ngx_set_shading(ngx_ctx *ctx, const ngx_shading *shading);
+
 /* Backend::render_batch (src/backend.h:67-68); checks of src/batch.cpp:146-162. views x 12 camera-to-world
- * floats (OpenGL convention). spp and frame only vary the RGB picture, which this backend does not draw. */
+ * floats (OpenGL convention). NGX_OUTPUT_RGB draws the classic picture (spp jittered samples a pixel, seeded
+ * by pixel and frame, src/render_core.h:251-263), 3 bytes a pixel; depth, normal and ID do not depend on spp
+ * or frame. */
 ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, int32_t spp,
                             uint32_t outputs, int32_t frame, void *stream, ngx_batch_frame *out);
 

@@ -41,6 +41,27 @@ class BatchFrame(ctypes.Structure):
                 ("depth", ctypes.c_void_p), ("normal", ctypes.c_void_p), ("object_id", ctypes.c_void_p)]
 
 
+class Material(ctypes.Structure):
+    _fields_ = [("diffuse_color", ctypes.c_float * 3), ("diffuse_albedo", ctypes.c_float), ("specular_albedo", ctypes.c_float),
+                ("reflectivity", ctypes.c_float), ("specular_exponent", ctypes.c_float)]
+
+
+class Light(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("intensity", ctypes.c_float), ("radius", ctypes.c_float),
+                ("falloff", ctypes.c_float), ("direction", ctypes.c_float * 3), ("spot_angle", ctypes.c_float),
+                ("spot_softness", ctypes.c_float), ("color", ctypes.c_float * 3)]
+
+
+class Shading(ctypes.Structure):
+    _fields_ = [("instance_materials", ctypes.c_void_p), ("sphere_materials", ctypes.c_void_p), ("floor_material", Material),
+                ("floor_checker_scale", ctypes.c_float), ("light_count", ctypes.c_int32), ("lights", ctypes.c_void_p),
+                ("ambient", ctypes.c_float), ("sky_up", ctypes.c_float * 3), ("sky_color_up", ctypes.c_float * 3),
+                ("sky_color_down", ctypes.c_float * 3), ("sky_intensity", ctypes.c_float)]
+
+
+assert ctypes.sizeof(Material) == 28 and ctypes.sizeof(Light) == 56
+
+
 class TraceConfig(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("xcd_remap", ctypes.c_int32), ("packet", ctypes.c_int32),
                 ("pair_limit", ctypes.c_int32), ("persistent", ctypes.c_int32), ("lane_min_triangles", ctypes.c_int32),
@@ -79,6 +100,7 @@ _SIGNATURES = {
     "ngx_last_error": (ctypes.c_char_p, []),
     "ngx_set_trace_config": (ctypes.c_int32, [_VP, ctypes.POINTER(TraceConfig)]),
     "ngx_get_trace_config": (ctypes.c_int32, [_VP, ctypes.POINTER(TraceConfig)]),
+    "ngx_set_shading": (ctypes.c_int32, [_VP, ctypes.POINTER(Shading)]),
     "ngx_autotune": (ctypes.c_int32, [_VP, _VP, ctypes.c_int32, ctypes.c_float, ctypes.c_uint32, ctypes.c_int32, _VP,
                                       ctypes.POINTER(TraceConfig), ctypes.POINTER(ctypes.c_float)]),
     "ngx_debug_keep_build": (ctypes.c_int32, [_VP, ctypes.c_int32]),

@@ -4,7 +4,8 @@ render, cast_rays and the same properties. Inputs may be NumPy arrays (checked, 
 torch tensors on the GPU (used where they are). Device work is queued on torch's current stream.
 
 Differences from the reference, all deliberate:
-  * the RGB picture is not drawn (rgb defaults to False here; asking for it raises NotImplementedError);
+  * the RGB picture is the classic one (materials, sphere lights, mirror reflections, sky gradient); textures and
+    environment maps are not supported (NotImplementedError), and rgb defaults to False here;
   * instances_in_world gives every world its own count of the template's first instances;
   * pose_backward_depth / pose_backward_range give gradients with respect to body poses.
 Errors: ValueError for bad input (the reference's std::invalid_argument), RuntimeError for device failures."""
@@ -14,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from .dlpack import to_torch
-from .geometry import Instance, Mesh, Plane
+from .geometry import Environment, Instance, Mesh, Plane, Sphere, shading_arrays
 
 _T = None
 
@@ -67,8 +68,9 @@ class BatchRenderer:
         if device not in ("auto", "cuda", "gpu"):
             raise ValueError(f"this backend runs on the GPU; device {device!r} is not one it serves")
         if textures:
-            raise NotImplementedError("textures only change the RGB picture, which this backend does not draw")
+            raise NotImplementedError("textured meshes are not supported by this backend's RGB picture")
         floor = floor if floor is not None else Plane()
+        spheres = [s if isinstance(s, Sphere) else Sphere(*s) for s in spheres]
         meshes = [m if isinstance(m, Mesh) else Mesh(*m) for m in meshes]
         instances = [i if isinstance(i, Instance) else Instance(*i) for i in instances]
         if device_id is None:
@@ -120,6 +122,54 @@ class BatchRenderer:
         self._bodies = None
         self._device_id = int(device_id)
         self._keep = []
+        self._shading = shading_arrays(instances, spheres, floor, list(lights))
+        self._apply_shading()
+
+    def _apply_shading(self):
+        """Hands the materials, lights, ambient level and sky to the backend (ngx_set_shading)."""
+        sh = self._shading
+        im = np.ascontiguousarray(sh["instance_materials"], np.float32)
+        sm = np.ascontiguousarray(sh["sphere_materials"], np.float32)
+        lights = np.ascontiguousarray(sh["lights"], np.float32)
+
+        def material(row):
+            return _lib.Material((ctypes.c_float * 3)(*row[:3]), *[float(v) for v in row[3:7]])
+
+        # the C structs are laid out as the flat arrays without their padding float
+        im7 = np.ascontiguousarray(im[:, :7]) if len(im) else np.zeros((1, 7), np.float32)
+        sm7 = np.ascontiguousarray(sm[:, :7]) if len(sm) else np.zeros((1, 7), np.float32)
+        l14 = lights if len(lights) else np.zeros((1, 14), np.float32)
+        sky = sh["sky"]
+        desc = _lib.Shading(im7.ctypes.data, sm7.ctypes.data, material(sh["floor_material"]), float(sh["floor_checker_scale"]),
+                            len(lights), l14.ctypes.data, float(sh["ambient"]), (ctypes.c_float * 3)(*sky[0:3]),
+                            (ctypes.c_float * 3)(*sky[3:6]), (ctypes.c_float * 3)(*sky[6:9]), float(sky[9]))
+        _lib.check(self._lib.ngx_set_shading(self._h, ctypes.byref(desc)))
+
+    # ---- the RGB picture's lighting (include/neraulgraphics/batch.h:206-212)
+    def set_ambient(self, ambient):
+        old = self._shading["ambient"]
+        self._shading["ambient"] = float(ambient)
+        try:
+            self._apply_shading()
+        except Exception:
+            self._shading["ambient"] = old
+            raise
+
+    @property
+    def ambient(self):
+        return self._shading["ambient"]
+
+    def set_environment(self, environment):
+        """The sky gradient (an Environment); maps are not supported."""
+        if getattr(environment, "map", None):
+            raise NotImplementedError("environment maps are not supported by this backend's RGB picture")
+        old = self._shading["sky"]
+        self._shading["sky"] = (environment if isinstance(environment, Environment) else Environment(**environment)).as_array()
+        try:
+            self._apply_shading()
+        except Exception:
+            self._shading["sky"] = old
+            raise
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -259,8 +309,8 @@ class BatchRenderer:
                                               self._stream_for(mem), ctypes.byref(f)))
         self._hold(keep, mem)
         v, h, w = views, self._height, self._width
-        shapes = {"depth": ((v, h, w), np.float32, f.depth), "normals": ((v, h, w, 3), np.float32, f.normal),
-                  "object_ids": ((v, h, w), np.int32, f.object_id)}
+        shapes = {"rgb": ((v, h, w, 3), np.uint8, f.rgb), "depth": ((v, h, w), np.float32, f.depth),
+                  "normals": ((v, h, w, 3), np.float32, f.normal), "object_ids": ((v, h, w), np.int32, f.object_id)}
         return self._outputs(shapes, zero_copy)
 
     def cast_rays(self, origins, directions, max_distance=float("inf"), zero_copy=None):

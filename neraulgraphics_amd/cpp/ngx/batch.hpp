@@ -6,10 +6,12 @@
 // (render_host, cast_rays_host), every call finished on return -- and the same error classes:
 // std::invalid_argument for bad input, checked before anything changes (src/batch.cpp:91-192), and
 // ngx::DeviceError, the counterpart of the reference's CudaError (include/neraulgraphics/renderer.h:32-39),
-// for device failures. The RGB picture is not drawn: asking for it throws ngx::Unsupported.
+// for device failures. The RGB picture is the reference's classic one; textures and environment maps are not
+// supported.
 //
 // Link with -lngx (neraulgraphics_amd/libngx.so) and include include/ and this directory.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -60,13 +62,35 @@ struct Vec3 {
     float x, y, z;
 };
 
-struct Sphere {  // include/neraulgraphics/scene.h:63-67, without the material (no RGB)
+struct Material {  // the classic fields of include/neraulgraphics/scene.h:35-40 (python/bindings.cpp:393-395 defaults)
+    Vec3 diffuse_color = {0.8f, 0.8f, 0.8f};
+    float diffuse_albedo = 0.9f, specular_albedo = 0.1f, reflectivity = 0.0f, specular_exponent = 50.0f;
+};
+
+struct Sphere {  // include/neraulgraphics/scene.h:63-67
     Vec3 center;
     float radius;
+    Material material{};
 };
 
 struct Plane {  // include/neraulgraphics/scene.h:96-103; half_extent_x 0 means no floor
-    float y = 0.0f, half_extent_x = 0.0f, z_min = 0.0f, z_max = 0.0f;
+    float y = 0.0f, half_extent_x = 0.0f, z_min = 0.0f, z_max = 0.0f, checker_scale = 0.5f;
+    Material material{{1.0f, 1.0f, 1.0f}, 1.0f, 0.0f, 0.0f, 1.0f};
+};
+
+struct Light {  // include/neraulgraphics/scene.h:78-92
+    Vec3 position;
+    float intensity = 1.0f, radius = 0.0f, falloff = 0.0f;
+    Vec3 direction = {0.0f, 0.0f, -1.0f};
+    float spot_angle = 0.0f, spot_softness = 0.25f;
+    Vec3 color = {1.0f, 1.0f, 1.0f};
+};
+
+struct Environment {  // include/neraulgraphics/scene.h:109-121, the gradient only
+    Vec3 up = {0.0f, 1.0f, 0.0f};
+    Vec3 color_up = {0.10f, 0.26f, 0.60f};
+    Vec3 color_down = {0.35f, 0.52f, 0.70f};
+    float intensity = 1.0f;
 };
 
 struct Mesh {  // include/neraulgraphics/scene.h: vertices and triangles, optional per-vertex normals
@@ -78,6 +102,7 @@ struct Mesh {  // include/neraulgraphics/scene.h: vertices and triangles, option
 struct Instance {  // a mesh placed by the top three rows of a 4x4 transform
     int mesh = 0;
     float transform[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    Material material{};
 };
 
 struct BodyBinding {  // include/neraulgraphics/batch.h:52-57
@@ -123,8 +148,9 @@ public:
     // world_count worlds, each holding the instances (the first instances_in_world[w] of them when given),
     // cameras_per_world views each, on HIP device device_id.
     BatchRenderer(int width, int height, int world_count, int cameras_per_world, const std::vector<Sphere> &spheres,
-                  const Plane &floor, const std::vector<Mesh> &meshes, const std::vector<Instance> &instances,
-                  const std::vector<int> &instances_in_world = {}, int device_id = 0) {
+                  const std::vector<Light> &lights, const Plane &floor, const std::vector<Mesh> &meshes,
+                  const std::vector<Instance> &instances, const std::vector<int> &instances_in_world = {}, int device_id = 0)
+        : instance_materials_(materials_of(instances)), sphere_materials_(materials_of(spheres)), floor_(floor), lights_(lights) {
         std::vector<float> verts, normals, sph, tf;
         std::vector<uint32_t> idx;
         std::vector<int32_t> vo{0}, io{0}, inst_mesh;
@@ -176,12 +202,50 @@ public:
         d.floor_z_max = floor.z_max;
         d.device_id = device_id;
         check(ngx_create(&d, &ctx_));
+        try {
+            apply_shading();
+        } catch (...) {
+            ngx_destroy(ctx_);
+            throw;
+        }
     }
     ~BatchRenderer() { ngx_destroy(ctx_); }
-    BatchRenderer(BatchRenderer &&o) noexcept : ctx_(std::exchange(o.ctx_, nullptr)) {}
+    BatchRenderer(BatchRenderer &&o) noexcept
+        : ctx_(std::exchange(o.ctx_, nullptr)), instance_materials_(std::move(o.instance_materials_)),
+          sphere_materials_(std::move(o.sphere_materials_)), floor_(o.floor_), lights_(std::move(o.lights_)), ambient_(o.ambient_),
+          environment_(o.environment_) {}
     BatchRenderer &operator=(BatchRenderer &&o) noexcept {
         std::swap(ctx_, o.ctx_);
+        std::swap(instance_materials_, o.instance_materials_);
+        std::swap(sphere_materials_, o.sphere_materials_);
+        std::swap(floor_, o.floor_);
+        std::swap(lights_, o.lights_);
+        std::swap(ambient_, o.ambient_);
+        std::swap(environment_, o.environment_);
         return *this;
+    }
+
+    // The RGB picture's ambient level and sky (batch.h:206-212); refused values change nothing.
+    void set_ambient(const float ambient) {
+        const float old = ambient_;
+        ambient_ = ambient;
+        try {
+            apply_shading();
+        } catch (...) {
+            ambient_ = old;
+            throw;
+        }
+    }
+    float ambient() const { return ambient_; }
+    void set_environment(const Environment &environment) {
+        const Environment old = environment_;
+        environment_ = environment;
+        try {
+            apply_shading();
+        } catch (...) {
+            environment_ = old;
+            throw;
+        }
     }
     BatchRenderer(const BatchRenderer &) = delete;
     BatchRenderer &operator=(const BatchRenderer &) = delete;
@@ -219,6 +283,7 @@ public:
         h.width = f.width;
         h.height = f.height;
         const size_t px = (size_t)f.views * f.width * f.height;
+        if (f.rgb) h.rgb = copy_out(f.rgb, 3 * px);
         if (f.depth) h.depth = copy_out(f.depth, px);
         if (f.normal) h.normal = copy_out(f.normal, 3 * px);
         if (f.object_id) h.object_id = copy_out(f.object_id, px);
@@ -256,6 +321,45 @@ public:
 
 private:
     ngx_ctx *ctx_ = nullptr;
+    std::vector<ngx_material> instance_materials_, sphere_materials_;
+    Plane floor_;
+    std::vector<Light> lights_;
+    float ambient_ = 0.0f;
+    Environment environment_{};
+
+    static ngx_material to_c(const Material &m) {
+        return ngx_material{{m.diffuse_color.x, m.diffuse_color.y, m.diffuse_color.z}, m.diffuse_albedo, m.specular_albedo, m.reflectivity,
+                            m.specular_exponent};
+    }
+    template <class T>
+    static std::vector<ngx_material> materials_of(const std::vector<T> &items) {
+        std::vector<ngx_material> out;
+        for (const T &i : items) out.push_back(to_c(i.material));
+        return out;
+    }
+    void apply_shading() {
+        std::vector<ngx_light> ls;
+        for (const Light &l : lights_)
+            ls.push_back(ngx_light{{l.position.x, l.position.y, l.position.z}, l.intensity, l.radius, l.falloff,
+                                   {l.direction.x, l.direction.y, l.direction.z}, l.spot_angle, l.spot_softness,
+                                   {l.color.x, l.color.y, l.color.z}});
+        ngx_shading h{};
+        h.instance_materials = instance_materials_.data();
+        h.sphere_materials = sphere_materials_.data();
+        h.floor_material = to_c(floor_.material);
+        h.floor_checker_scale = floor_.checker_scale;
+        h.light_count = (int32_t)ls.size();
+        h.lights = ls.data();
+        h.ambient = ambient_;
+        const Environment &e = environment_;
+        const float sky[10] = {e.up.x, e.up.y, e.up.z, e.color_up.x, e.color_up.y, e.color_up.z, e.color_down.x, e.color_down.y,
+                               e.color_down.z, e.intensity};
+        std::copy(sky, sky + 3, h.sky_up);
+        std::copy(sky + 3, sky + 6, h.sky_color_up);
+        std::copy(sky + 6, sky + 9, h.sky_color_down);
+        h.sky_intensity = sky[9];
+        check(ngx_set_shading(live(), &h));
+    }
 
     ngx_ctx *live() const {
         if (!ctx_) throw std::logic_error("this BatchRenderer was moved from");

@@ -26,6 +26,9 @@ struct Failure {
 void require(const bool ok, const std::string &message) {
     if (!ok) fail(NGX_INVALID_ARGUMENT, message);
 }
+void require_state(const bool ok, const std::string &message) {
+    if (!ok) fail(NGX_BAD_STATE, message);
+}
 void hip_check(const hipError_t err, const char *what) {
     if (err == hipSuccess) return;
     (void)hipGetLastError();
@@ -129,6 +132,12 @@ struct ngx_ctx {
     float *d_body_stage = nullptr;
     const float *last_body_poses = nullptr;
 
+    // shading of the RGB picture
+    bool shading_set = false;
+    ngx::ShadeScene shade{};
+    float4 *d_inst_mat = nullptr, *d_sphere_mat = nullptr, *d_lights = nullptr;
+    uint8_t *d_rgb = nullptr;
+
     // outputs and staging, grown as needed
     float *d_depth = nullptr, *d_normal = nullptr;
     int32_t *d_id = nullptr;
@@ -166,7 +175,8 @@ struct ngx_ctx {
                            d_dbg_world_bounds};
         for (float *p : floats) if (p) (void)hipFree(p);
         void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
-                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, d_skip, d_ray_keys, d_ray_vals, big.key[0], big.key[1], big.val[0], big.val[1],
+                          d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, d_skip, d_ray_keys, d_ray_vals, d_inst_mat, d_sphere_mat,
+                          d_lights, d_rgb, big.key[0], big.key[1], big.val[0], big.val[1],
                           big.box, big.node_box, big.left, big.right, big.parent, big.arrivals, big.bounds, big.tile_counts};
         for (void *p : others) if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -562,6 +572,101 @@ ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memo
     });
 }
 
+namespace {
+// validate.cpp:23-31 for the classic fields
+void check_material(const ngx_material &m, const std::string &name) {
+    require(finite_all(m.diffuse_color, 3) && std::isfinite(m.diffuse_albedo) && std::isfinite(m.specular_albedo) &&
+                std::isfinite(m.reflectivity) && std::isfinite(m.specular_exponent),
+            name + " has a NaN or infinite value");
+}
+void pack_material(const ngx_material &m, float4 out[2]) {
+    out[0] = make_float4(m.diffuse_color[0], m.diffuse_color[1], m.diffuse_color[2], m.diffuse_albedo);
+    out[1] = make_float4(m.specular_albedo, m.reflectivity, m.specular_exponent, 0.0f);
+}
+}  // namespace
+
+ngx_status ngx_set_shading(ngx_ctx *ctx, const ngx_shading *shading) {
+    return guarded([&] {
+        require(ctx != nullptr && shading != nullptr, "context and shading must not be null");
+        const ngx_shading &h = *shading;
+        require(ctx->per_world == 0 || h.instance_materials != nullptr, "instance_materials must hold one material per template instance");
+        require(ctx->sphere_count == 0 || h.sphere_materials != nullptr, "sphere_materials must hold one material per sphere");
+        for (int i = 0; i < ctx->per_world; ++i) check_material(h.instance_materials[i], "instance " + std::to_string(i));
+        for (int i = 0; i < ctx->sphere_count; ++i) check_material(h.sphere_materials[i], "sphere " + std::to_string(i));
+        check_material(h.floor_material, "floor");
+        require(std::isfinite(h.floor_checker_scale), "floor_checker_scale must be finite");
+        require(h.light_count >= 0 && h.light_count <= NGX_MAX_LIGHTS,
+                "a scene holds at most " + std::to_string(NGX_MAX_LIGHTS) + " lights, got " + std::to_string(h.light_count));
+        require(h.light_count == 0 || h.lights != nullptr, "lights must not be null");
+        // validate.cpp:40-62
+        for (int i = 0; i < h.light_count; ++i) {
+            const ngx_light &l = h.lights[i];
+            const std::string name = "light " + std::to_string(i);
+            require(finite_all(l.position, 3) && std::isfinite(l.intensity) && std::isfinite(l.radius), name + " has a NaN or infinite value");
+            require(l.radius >= 0.0f, name + " radius must be 0 or more, got " + std::to_string(l.radius));
+            require(std::isfinite(l.falloff) && l.falloff >= 0.0f, name + " falloff must be 0 or more, got " + std::to_string(l.falloff));
+            require(finite_all(l.direction, 3) && std::isfinite(l.spot_angle) && std::isfinite(l.spot_softness), name + " has a NaN or infinite value");
+            require(l.spot_angle >= 0.0f && l.spot_angle < 3.14159f,
+                    name + " spot_angle must be 0 (no cone) to below pi radians, got " + std::to_string(l.spot_angle));
+            require(finite_all(l.color, 3) && l.color[0] >= 0.0f && l.color[1] >= 0.0f && l.color[2] >= 0.0f,
+                    name + " color must be finite and not negative");
+            require(l.spot_softness >= 0.0f && l.spot_softness <= 1.0f, name + " spot_softness must be 0 to 1, got " + std::to_string(l.spot_softness));
+            require(l.spot_angle == 0.0f || l.direction[0] != 0.0f || l.direction[1] != 0.0f || l.direction[2] != 0.0f,
+                    name + " is a spotlight (spot_angle > 0) but its direction is zero");
+        }
+        require(std::isfinite(h.ambient) && h.ambient >= 0.0f, "ambient must be 0 or more, got " + std::to_string(h.ambient));
+        require(finite_all(h.sky_up, 3) && finite_all(h.sky_color_up, 3) && finite_all(h.sky_color_down, 3) && std::isfinite(h.sky_intensity),
+                "the sky has a NaN or infinite value");
+        const double un = std::sqrt((double)h.sky_up[0] * h.sky_up[0] + (double)h.sky_up[1] * h.sky_up[1] + (double)h.sky_up[2] * h.sky_up[2]);
+        require(un > 0.0, "the sky's up axis must not be zero");
+        ctx->set_device();
+        HIP_CHECK(hipDeviceSynchronize());
+        std::vector<float4> im(2 * (size_t)std::max(1, ctx->per_world)), sm(2 * (size_t)std::max(1, ctx->sphere_count)),
+            lights(4 * (size_t)std::max(1, (int)h.light_count));
+        for (int i = 0; i < ctx->per_world; ++i) pack_material(h.instance_materials[i], &im[2 * i]);
+        for (int i = 0; i < ctx->sphere_count; ++i) pack_material(h.sphere_materials[i], &sm[2 * i]);
+        for (int i = 0; i < h.light_count; ++i) {
+            // the device form of a light (device_scene.h:77-84): unit direction, the cone as cosines
+            const ngx_light &l = h.lights[i];
+            float dx = l.direction[0], dy = l.direction[1], dz = l.direction[2];
+            const float n = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (n > 0.0f) {
+                dx = dx / n;
+                dy = dy / n;
+                dz = dz / n;
+            } else {
+                dx = 0.0f;
+                dy = 0.0f;
+                dz = -1.0f;
+            }
+            const bool spot = l.spot_angle > 0.0f;
+            const float cos_outer = spot ? std::cos(l.spot_angle) : -2.0f;
+            const float cos_inner = spot ? std::cos(l.spot_angle * (1.0f - l.spot_softness)) : -2.0f;
+            lights[4 * i] = make_float4(l.position[0], l.position[1], l.position[2], l.intensity);
+            lights[4 * i + 1] = make_float4(l.radius, l.falloff, cos_outer, cos_inner);
+            lights[4 * i + 2] = make_float4(dx, dy, dz, 0.0f);
+            lights[4 * i + 3] = make_float4(l.color[0], l.color[1], l.color[2], 0.0f);
+        }
+        upload(ctx->d_inst_mat, im.data(), im.size());
+        upload(ctx->d_sphere_mat, sm.data(), sm.size());
+        upload(ctx->d_lights, lights.data(), lights.size());
+        ngx::ShadeScene &e = ctx->shade;
+        e.instance_materials = ctx->d_inst_mat;
+        e.sphere_materials = ctx->d_sphere_mat;
+        pack_material(h.floor_material, e.floor_material);
+        e.floor_checker_scale = h.floor_checker_scale;
+        e.light_count = h.light_count;
+        e.lights = ctx->d_lights;
+        e.ambient = h.ambient;
+        // the sky's up axis, normalized in FP64 as the reference's environment frame is (environment.cpp:30-33)
+        e.sky_up = make_float3((float)(h.sky_up[0] / un), (float)(h.sky_up[1] / un), (float)(h.sky_up[2] / un));
+        e.sky_color_up = make_float3(h.sky_color_up[0], h.sky_color_up[1], h.sky_color_up[2]);
+        e.sky_color_down = make_float3(h.sky_color_down[0], h.sky_color_down[1], h.sky_color_down[2]);
+        e.sky_intensity = h.sky_intensity;
+        ctx->shading_set = true;
+    });
+}
+
 ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, int32_t spp, uint32_t outputs,
                             int32_t frame, void *stream, ngx_batch_frame *out) {
     return guarded([&] {
@@ -574,7 +679,8 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
         require(cams12 != nullptr, "camera_matrices must not be null");
         if (memory == NGX_MEM_HOST) require(finite_all(cams12, 12 * (size_t)ctx->views), "a camera has a NaN or infinite value");
-        if (outputs & NGX_OUTPUT_RGB) fail(NGX_UNSUPPORTED, "this backend renders depth, normal and object ID, not the RGB picture");
+        if (outputs & NGX_OUTPUT_RGB)
+            require_state(ctx->shading_set, "the RGB picture needs materials and lights: call ngx_set_shading first");
         ctx->set_device();
         const hipStream_t s = static_cast<hipStream_t>(stream);
         const size_t pixels = (size_t)ctx->views * ctx->width * ctx->height;
@@ -604,13 +710,31 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         a.object_id = (outputs & NGX_OUTPUT_OBJECT_ID) ? ctx->d_id : nullptr;
         a.xcd_remap = ctx->xcd_remap;
         a.walk = ctx->walk();
+        uint8_t *rgb = nullptr;
+        if (outputs & NGX_OUTPUT_RGB) {
+            if (!ctx->d_rgb) device_alloc(ctx->d_rgb, 3 * pixels);
+            rgb = ctx->d_rgb;
+        }
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
-        HIP_CHECK(ngx::launch_render(a, s, ctx->persistent ? ctx->persistent_blocks : 0));
+        if (a.depth || a.normal || a.object_id) HIP_CHECK(ngx::launch_render(a, s, ctx->persistent ? ctx->persistent_blocks : 0));
+        if (rgb) {
+            ngx::RgbArgs g{};
+            g.r = a;
+            g.r.depth = nullptr;
+            g.r.normal = nullptr;
+            g.r.object_id = nullptr;
+            g.r.tile_counters = nullptr;
+            g.shade = ctx->shade;
+            g.rgb = rgb;
+            g.spp = spp;
+            g.frame = frame;
+            HIP_CHECK(ngx::launch_rgb(g, s));
+        }
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
         ctx->timed = timed;
         ctx->last_stream = s;
-        *out = ngx_batch_frame{NGX_MEM_DEVICE, ctx->device, ctx->views, ctx->width, ctx->height, nullptr, a.depth, a.normal, a.object_id};
+        *out = ngx_batch_frame{NGX_MEM_DEVICE, ctx->device, ctx->views, ctx->width, ctx->height, rgb, a.depth, a.normal, a.object_id};
     });
 }
 

@@ -79,6 +79,29 @@ struct RenderArgs {
     int *tile_counters;        // persistent mode (non-null): 8 counters, zeroed before the launch, one per XCD
 };
 
+// The RGB picture's shading inputs, on the device: materials as two float4 (diffuse rgb, diffuse albedo;
+// specular albedo, reflectivity, specular exponent, 0), lights as four (position, intensity; radius, falloff,
+// cos_outer, cos_inner; direction, 0; colour, 0).
+constexpr int MAX_LIGHTS = 8;
+struct ShadeScene {
+    const float4 *instance_materials;  // [template instances][2]
+    const float4 *sphere_materials;    // [spheres][2]
+    float4 floor_material[2];
+    float floor_checker_scale;
+    int light_count;
+    const float4 *lights;              // [light_count][4]
+    float ambient;
+    float3 sky_up, sky_color_up, sky_color_down;
+    float sky_intensity;
+};
+
+struct RgbArgs {
+    RenderArgs r;                      // the frame, cameras, walk and tiles (its AOV outputs unused)
+    ShadeScene shade;
+    uint8_t *rgb;                      // [views][H][W][3]
+    int spp, frame;
+};
+
 struct RayArgs {
     TraceScene s;
     const float *origins;      // [worlds][R][3]
@@ -159,6 +182,9 @@ hipError_t launch_ray_order(const float *dirs, int rays_per_world, int worlds, u
 
 // Builds world `world` (instances first .. first+n-1, n > 1) with grid-wide kernels.
 hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &s, int world, int first, int n, hipStream_t stream);
+// The RGB picture of every view, one 64-pixel tile per wavefront.
+hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream);
+
 // Renders every view; with a.tile_counters set, a fixed grid of `persistent_blocks` workgroups takes tiles
 // from per-XCD counters until none are left (the counters are zeroed by this call first).
 hipError_t launch_render(const RenderArgs &a, hipStream_t stream, int persistent_blocks = 0);

@@ -810,6 +810,224 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
 }
 #endif
 
+// ------------------------------------------------------------------------------ RGB picture --
+//
+// The classic picture (render_core.h:146-283): per pixel, spp jittered camera rays; per ray, up to
+// MAX_BOUNCES mirror bounces; at each hit, direct light from every sphere light sampled at a random point of
+// the light (so shadows are soft), tested with a shadow ray, plus an ambient term; rays that leave the scene
+// see the sky gradient. The random stream is the reference's: a PCG hash seeded by the pixel and the frame,
+// drawn in the same order. A wavefront traces its 64 rays together with the walk of the depth image, so its
+// primary rays are a packet; after a bounce the lanes' rays go their own ways and the packet walk still
+// serves them (each lane tests with its own ray).
+constexpr int MAX_BOUNCES = 4;
+constexpr float MIN_THROUGHPUT = 1e-3f;
+constexpr float SURFACE_EPS = 1e-3f;
+
+__device__ __forceinline__ uint32_t pcg_hash(const uint32_t x) {
+    const uint32_t state = x * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float rand01(uint32_t &state) {
+    state = pcg_hash(state);
+    return (float)state * 2.3283064365386963e-10f;
+}
+__device__ __forceinline__ float saturate(const float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ float3 mul3v(const float3 a, const float3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float3 reflect3(const float3 i, const float3 n) { return sub3(i, mul3(n, 2.0f * dot3(i, n))); }
+__device__ __forceinline__ float3 offset_origin(const float3 p, const float3 n, const float3 dir) {
+    return add3(p, mul3(n, dot3(dir, n) < 0.0f ? -SURFACE_EPS : SURFACE_EPS));
+}
+__device__ __forceinline__ float3 sky(const ShadeScene &E, const float3 dir) {
+    const float t = saturate(dot3(dir, E.sky_up) * 0.5f + 0.5f);
+    return mul3(add3(E.sky_color_down, mul3(sub3(E.sky_color_up, E.sky_color_down), t)), E.sky_intensity);
+}
+// geom_dev.h:141-152: a uniform direction, then a point on the light's sphere
+__device__ __forceinline__ float3 sample_sphere_light(const float3 c, const float radius, uint32_t &rng) {
+    const float z = 1.0f - 2.0f * rand01(rng);
+    const float rad = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = 6.28318530718f * rand01(rng);
+    return add3(c, mul3(mk3(rad * cosf(phi), rad * sinf(phi), z), radius));
+}
+// device_scene.h:98-111
+__device__ __forceinline__ float light_factor(const float4 q, const float4 dir, const float3 towards, const float d) {
+    float f = 1.0f;
+    if (q.y > 0.0f) {
+        const float r = d / q.y;
+        f = 1.0f / (1.0f + r * r);
+    }
+    if (q.z > -1.5f) {
+        const float c = -(dir.x * towards.x + dir.y * towards.y + dir.z * towards.z);
+        const float t = fminf(fmaxf((c - q.z) / fmaxf(q.w - q.z, 1e-6f), 0.0f), 1.0f);
+        f *= t * t * (3.0f - 2.0f * t);
+    }
+    return f;
+}
+
+// Whether anything lies between o and o + max_dist d, for the lanes in live (render_core.h:146-160).
+template <int WALK>
+__device__ __forceinline__ bool occluded(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
+                                         const uint64_t live, const float max_dist) {
+    const bool mine = in_mask(live);
+    bool blocked = false;
+    for (int i = 0; i < S.sphere_count; ++i) {
+        const float4 sp = sload4(S.spheres + i);
+        const float3 Lc = sub3(xyz(sp), o);
+        const float tca = dot3(Lc, d);
+        const float r2 = sp.w * sp.w;
+        const float d2 = dot3(Lc, Lc) - tca * tca;
+        if (d2 > r2) continue;
+        const float thc = sqrtf(r2 - d2);
+        float t = tca - thc;
+        if (t < 0.0f) t = tca + thc;
+        if (t >= 0.0f && t < max_dist) blocked = true;
+    }
+    if (fabsf(d.y) > 1e-3f) {
+        const float dist = (S.floor_y - o.y) / d.y;
+        if (dist > 0.0f && dist < max_dist) {
+            const float3 pt = add3(o, mul3(d, dist));
+            if (fabsf(pt.x) < S.floor_hx && pt.z > S.floor_zmin && pt.z < S.floor_zmax) blocked = true;
+        }
+    }
+    // a closest-hit walk limited to max_dist finds a hit exactly when an any-hit walk would (mesh_dev.h:208-227)
+    const uint64_t rest = ballot(mine && !blocked);
+    if (rest != 0) {
+        float best = max_dist;
+        LaneHit h;
+        NoStats stats;
+        if (WALK != WALK_LANES) {
+            const int lane = lane_id();
+            L->hit_inst[lane] = -1;
+            wave_lds_order();
+            trace_world<WALK == WALK_HYBRID>(S, L, world, o, d, rest, best, h, stats);
+            wave_lds_order();
+            if (in_mask(rest) && L->hit_inst[lane] >= 0) blocked = true;
+        } else {
+            h.inst = -1;
+            trace_world_lanes(S, world, o, d, in_mask(rest), best, h, stats);
+            if (in_mask(rest) && h.inst >= 0) blocked = true;
+        }
+    }
+    return mine && blocked;
+}
+
+__device__ __forceinline__ float3 camera_ray_at(const Cam &c, const float fov_scale, const float aspect, const float px, const float py,
+                                                const int width, const int height) {
+    const float sx = (2.0f * px / (float)width - 1.0f) * fov_scale * aspect;
+    const float sy = -(2.0f * py / (float)height - 1.0f) * fov_scale;
+    return normalize3(add3(add3(mul3(c.right, sx), mul3(c.up, sy)), c.fwd));
+}
+
+__device__ __forceinline__ float srgb_channel(const float v) {
+    const float x = saturate(v);
+    return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+
+template <int WALK>
+__global__ __launch_bounds__(256) void rgb_kernel(const RgbArgs G) {
+    NGX_PAIR_LDS(L);
+    const RenderArgs &A = G.r;
+    const ShadeScene &E = G.shade;
+    const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
+    if (tile >= A.total_tiles) return;
+    const int lane = lane_id();
+    const int view = (int)(tile / A.tiles_per_view);
+    const int tt = (int)(tile % A.tiles_per_view);
+    const int x = (tt % A.tiles_x) * A.tile_w + lane % A.tile_w;
+    const int y = (tt / A.tiles_x) * A.tile_h + lane / A.tile_w;
+    const bool valid = x < A.width && y < A.height;
+    const int world = view / A.cameras_per_world;
+    const Cam cam = load_camera(A.cams + 12 * (size_t)view);
+    uint32_t rng = pcg_hash((uint32_t)(y * A.width + x) ^ pcg_hash((uint32_t)G.frame + 1u));  // geom_dev.h:122-126
+    float3 sum = mk3(0.0f, 0.0f, 0.0f);
+    for (int sample = 0; sample < G.spp; ++sample) {
+        const float jx = rand01(rng), jy = rand01(rng);
+        float3 o = cam.pos;
+        float3 d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)x + jx, (float)y + jy, A.width, A.height);
+        float3 radiance = mk3(0.0f, 0.0f, 0.0f);
+        float throughput = 1.0f;
+        bool active = valid;
+        for (int depth = 0; ballot(active); ++depth) {
+            float best;
+            LaneHit h;
+            NoStats stats;
+            int kind = OBJECT_NONE;
+            if (depth <= MAX_BOUNCES) kind = scene_hit<WALK>(A.s, L, world, o, d, ballot(active), best, h, stats);
+            const bool hit = active && kind != OBJECT_NONE;
+            if (active && !hit) {
+                radiance = add3(radiance, mul3(sky(E, d), throughput));
+                active = false;
+            }
+            float3 normal = mk3(0.0f, 1.0f, 0.0f), point = o;
+            float4 m0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m1 = m0;
+            if (hit) {
+                point = add3(o, mul3(d, best));
+                int object;
+                surface(A.s, kind, o, d, best, h, normal, object);
+                if (kind == OBJECT_INSTANCE) {
+                    const float4 *mp = E.instance_materials + 2 * (size_t)A.s.inst[h.inst].tmpl;
+                    m0 = mp[0];
+                    m1 = mp[1];
+                } else if (kind == OBJECT_FLOOR) {
+                    m0 = E.floor_material[0];
+                    m1 = E.floor_material[1];
+                    const int cx = (int)floorf(point.x * E.floor_checker_scale), cz = (int)floorf(point.z * E.floor_checker_scale);
+                    const float3 tint = ((cx + cz) & 1) ? mk3(1.0f, 1.0f, 1.0f) : mk3(1.0f, 0.7f, 0.3f);
+                    m0.x = tint.x * 0.3f;
+                    m0.y = tint.y * 0.3f;
+                    m0.z = tint.z * 0.3f;
+                } else {
+                    const float4 *mp = E.sphere_materials + 2 * (size_t)(kind - OBJECT_SPHERE);
+                    m0 = mp[0];
+                    m1 = mp[1];
+                }
+            }
+            float3 diffuse = mk3(0.0f, 0.0f, 0.0f), specular = mk3(0.0f, 0.0f, 0.0f);
+            for (int i = 0; i < E.light_count; ++i) {
+                const float4 lp0 = E.lights[4 * i], lq = E.lights[4 * i + 1], ldir = E.lights[4 * i + 2], lcol = E.lights[4 * i + 3];
+                float3 light_dir = mk3(0.0f, 1.0f, 0.0f);
+                float light_distance = 0.0f, strength = 0.0f;
+                if (hit) {
+                    const float3 lp = sample_sphere_light(xyz(lp0), lq.x, rng);
+                    const float3 to_light = sub3(lp, point);
+                    light_distance = sqrtf(dot3(to_light, to_light));
+                    light_dir = mul3(to_light, rcp_rn(light_distance));
+                    strength = lp0.w * light_factor(lq, ldir, light_dir, light_distance);
+                }
+                const bool test = hit && strength > 0.0f;
+                if (!ballot(test)) continue;
+                const bool blocked = occluded<WALK>(A.s, L, world, offset_origin(point, normal, light_dir), light_dir, ballot(test),
+                                                    light_distance);
+                if (test && !blocked) {
+                    const float3 lit = mul3(xyz(lcol), strength);
+                    diffuse = add3(diffuse, mul3(lit, fmaxf(0.0f, dot3(light_dir, normal))));
+                    specular = add3(specular, mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z)));
+                }
+            }
+            if (hit) {
+                const float3 dc = xyz(m0);
+                const float3 direct = add3(mul3v(dc, mul3(diffuse, m0.w)), mul3(specular, m1.x));
+                radiance = add3(radiance, mul3(add3(direct, mul3(dc, E.ambient * m0.w)), throughput));
+                throughput *= m1.y;
+                if (throughput < MIN_THROUGHPUT) {
+                    active = false;
+                } else {
+                    d = reflect3(d, normal);
+                    o = offset_origin(point, normal, d);
+                }
+            }
+        }
+        sum = add3(sum, radiance);
+    }
+    if (!valid) return;
+    const float3 c = mul3(sum, 1.0f / (float)G.spp);
+    const size_t idx = ((size_t)view * A.height + y) * A.width + x;
+    uint8_t *out = G.rgb + 3 * idx;
+    out[0] = (uint8_t)(saturate(srgb_channel(c.x)) * 255.0f + 0.5f);
+    out[1] = (uint8_t)(saturate(srgb_channel(c.y)) * 255.0f + 0.5f);
+    out[2] = (uint8_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
+}
+
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
 // worlds walks each world's rays as its own packet.
 template <int WALK>
@@ -1087,6 +1305,16 @@ hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream
     return hipGetLastError();
 }
 #endif
+
+hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream) {
+    if (a.r.total_tiles <= 0) return hipSuccess;
+    if (a.spp < 1 || a.shade.light_count < 0 || a.shade.light_count > MAX_LIGHTS) return hipErrorInvalidValue;
+    const long long blocks = (a.r.total_tiles + 3) / 4;
+#define NGX_LAUNCH(W) rgb_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
+    NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
+    return hipGetLastError();
+}
 
 hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
     if (a.count <= 0) return hipSuccess;

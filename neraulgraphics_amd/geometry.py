@@ -43,8 +43,18 @@ class Transform:
 
 @dataclass
 class Material:
-    """Accepted for signature compatibility: depth, normal and object ID do not depend on materials."""
-    diffuse_color: tuple = (0.5, 0.5, 0.5)
+    """The classic fields of a material (include/neraulgraphics/scene.h:35-40), with the reference Python
+    binding's defaults (python/bindings.cpp:393-395). The physically based fields only serve the reference's path
+    tracer, which the batched path does not use."""
+    diffuse_color: tuple = (0.8, 0.8, 0.8)
+    diffuse_albedo: float = 0.9
+    specular_albedo: float = 0.1
+    reflectivity: float = 0.0
+    specular_exponent: float = 50.0
+
+    def as_array(self):
+        return np.array([*self.diffuse_color, self.diffuse_albedo, self.specular_albedo, self.reflectivity,
+                         self.specular_exponent, 0.0], np.float32)
 
 
 @dataclass
@@ -68,16 +78,52 @@ class Plane:
     half_extent_x: float = 0.0
     z_min: float = 0.0
     z_max: float = 0.0
-    checker_scale: float = 1.0
-    material: Material = field(default_factory=Material)
+    checker_scale: float = 0.5
+    material: Material = field(default_factory=lambda: Material((1.0, 1.0, 1.0), 1.0, 0.0, 0.0, 1.0))
 
 
 @dataclass
 class Light:
-    """Accepted for signature compatibility: lights only change the RGB picture."""
+    """A sphere light (include/neraulgraphics/scene.h:78-92, defaults of python/bindings.cpp:511-512): radius 0 gives
+    hard shadows; falloff > 0 divides the intensity by 1 + (distance / falloff)^2; spot_angle > 0 makes it a spotlight
+    along direction, fading over the cone's outer spot_softness; color tints it."""
     position: tuple
     intensity: float = 1.0
     radius: float = 0.0
+    falloff: float = 0.0
+    direction: tuple = (0.0, 0.0, -1.0)
+    spot_angle: float = 0.0
+    spot_softness: float = 0.25
+    color: tuple = (1.0, 1.0, 1.0)
+
+    def as_array(self):
+        return np.array([*self.position, self.intensity, self.radius, self.falloff, *self.direction, self.spot_angle,
+                         self.spot_softness, *self.color], np.float32)
+
+
+@dataclass
+class Environment:
+    """The sky's gradient (include/neraulgraphics/scene.h:109-121): brighter or darker along up. Environment maps
+    are not supported by this backend."""
+    up: tuple = (0.0, 1.0, 0.0)
+    color_up: tuple = (0.10, 0.26, 0.60)
+    color_down: tuple = (0.35, 0.52, 0.70)
+    intensity: float = 1.0
+
+    def as_array(self):
+        return np.array([*self.up, *self.color_up, *self.color_down, self.intensity], np.float32)
+
+
+def shading_arrays(instances, spheres=(), floor=None, lights=(), ambient=0.0, environment=None):
+    """The RGB picture's inputs as flat float32 arrays, in the layout of the C ABI's ngx_shading (and of the
+    oracle): materials 8 floats each, lights 14, sky 10."""
+    floor = floor if floor is not None else Plane()
+    env = environment if environment is not None else Environment()
+    im = np.stack([i.material.as_array() for i in instances]) if instances else np.zeros((0, 8), np.float32)
+    sm = np.stack([s.material.as_array() for s in spheres]) if spheres else np.zeros((0, 8), np.float32)
+    ls = np.stack([l.as_array() for l in lights]) if lights else np.zeros((0, 14), np.float32)
+    return dict(instance_materials=im, sphere_materials=sm, floor_material=floor.material.as_array(),
+                floor_checker_scale=float(floor.checker_scale), lights=ls, ambient=float(ambient), sky=env.as_array())
 
 
 def make_box(h=(1.0, 1.0, 1.0)):

@@ -1039,6 +1039,209 @@ static OrcScene make_scene(const OrcTlasNode *tlas, const int32_t *roots, const 
     return s;
 }
 
+/* ------------------------------------------------------------------------ RGB picture -- */
+
+/* geom_dev.h:114-132: the PCG hash stream, seeded by pixel and frame */
+static uint32_t pcg_hash(uint32_t x) {
+    const uint32_t state = x * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+static float rand_float(uint32_t *state) {
+    *state = pcg_hash(*state);
+    return (float)(*state) * 2.3283064365386963e-10f;
+}
+static float saturatef(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); } /* geom_dev.h:77-83, host branch */
+static f3 mulv(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static f3 reflect3(f3 i, f3 n) { return sub3(i, mul3(n, 2.0f * dot3(i, n))); } /* geom_dev.h:97 */
+
+typedef struct {
+    const OrcScene *s;
+    const float *inst_mat, *sphere_mat, *floor_mat; /* 8 floats each */
+    float floor_checker;
+    int light_count;
+    const float *lights;                            /* 16 floats each, the device form */
+    float ambient;
+    f3 sky_up, sky_cu, sky_cd;
+    float sky_intensity;
+} OrcShade;
+
+static f3 sky(const OrcShade *e, f3 dir) { /* render_core.h:184-188, gradient */
+    const float t = saturatef(dot3(dir, e->sky_up) * 0.5f + 0.5f);
+    return mul3(add3(e->sky_cd, mul3(sub3(e->sky_cu, e->sky_cd), t)), e->sky_intensity);
+}
+
+static f3 offset_origin(f3 p, f3 n, f3 dir) { /* render_core.h:194-196 */
+    return add3(p, mul3(n, dot3(dir, n) < 0.0f ? -1e-3f : 1e-3f));
+}
+
+static int scene_occluded(const OrcScene *s, int world, f3 o, f3 d, float max_dist) { /* render_core.h:146-160 */
+    for (int i = 0; i < s->sphere_count; ++i) {
+        float t;
+        if (sphere_hit(s->spheres + 4 * i, o, d, &t) && t < max_dist) return 1;
+    }
+    if (fabsf(d.y) > 1e-3f) {
+        const float dist = (s->floor_y - o.y) / d.y;
+        if (dist > 0.0f && dist < max_dist) {
+            const f3 pt = add3(o, mul3(d, dist));
+            if (fabsf(pt.x) < s->floor_hx && pt.z > s->floor_zmin && pt.z < s->floor_zmax) return 1;
+        }
+    }
+    if (s->instance_count == 0) return 0;
+    /* mesh_occluded (mesh_dev.h:208-227): a hit closer than max_dist exists exactly when the closest hit
+       limited to max_dist is found */
+    float tmax = max_dist;
+    OrcHit h = {0};
+    mesh_intersect(s, world, o, d, &tmax, &h);
+    return h.found;
+}
+
+static float light_factor(const float *l, f3 towards, float d) { /* device_scene.h:98-111 */
+    float f = 1.0f;
+    if (l[5] > 0.0f) {
+        const float q = d / l[5];
+        f = 1.0f / (1.0f + q * q);
+    }
+    if (l[6] > -1.5f) {
+        const float c = -(l[8] * towards.x + l[9] * towards.y + l[10] * towards.z);
+        const float t = fminf(fmaxf((c - l[6]) / fmaxf(l[7] - l[6], 1e-6f), 0.0f), 1.0f);
+        f *= t * t * (3.0f - 2.0f * t);
+    }
+    return f;
+}
+
+static f3 trace_rgb(const OrcShade *e, int world, f3 origin, f3 dir, uint32_t *rng) { /* render_core.h:200-241 */
+    f3 radiance = mk(0.0f, 0.0f, 0.0f);
+    float throughput = 1.0f;
+    for (int depth = 0;; ++depth) {
+        OrcSceneHit hit;
+        if (depth > 4 || !scene_intersect(e->s, world, origin, dir, &hit)) {
+            radiance = add3(radiance, mul3(sky(e, dir), throughput));
+            break;
+        }
+        const f3 point = add3(origin, mul3(dir, hit.t));
+        const float *m;
+        float dc[3];
+        if (hit.inst >= 0) {
+            m = e->inst_mat + 8 * (size_t)(hit.inst - e->s->world_off[world]);
+            dc[0] = m[0]; dc[1] = m[1]; dc[2] = m[2];
+        } else if (hit.object == OBJECT_FLOOR) {
+            m = e->floor_mat;
+            const int cx = (int)floorf(point.x * e->floor_checker), cz = (int)floorf(point.z * e->floor_checker);
+            const int odd = (cx + cz) & 1;
+            dc[0] = 1.0f * 0.3f; dc[1] = (odd ? 1.0f : 0.7f) * 0.3f; dc[2] = (odd ? 1.0f : 0.3f) * 0.3f;
+        } else {
+            m = e->sphere_mat + 8 * (size_t)(hit.object - OBJECT_SPHERE);
+            dc[0] = m[0]; dc[1] = m[1]; dc[2] = m[2];
+        }
+        f3 diffuse = mk(0.0f, 0.0f, 0.0f), specular = mk(0.0f, 0.0f, 0.0f);
+        for (int i = 0; i < e->light_count; ++i) {
+            const float *l = e->lights + 16 * i;
+            /* sample_sphere_light (geom_dev.h:134-152) */
+            const float z = 1.0f - 2.0f * rand_float(rng);
+            const float rad = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            const float phi = 6.28318530718f * rand_float(rng);
+            const f3 lp = add3(mk(l[0], l[1], l[2]), mul3(mk(rad * cosf(phi), rad * sinf(phi), z), l[4]));
+            const f3 to_light = sub3(lp, point);
+            const float light_distance = sqrtf(dot3(to_light, to_light));
+            const f3 light_dir = mul3(to_light, 1.0f / light_distance);
+            const float strength = l[3] * light_factor(l, light_dir, light_distance);
+            if (strength <= 0.0f) continue;
+            if (scene_occluded(e->s, world, offset_origin(point, hit.normal, light_dir), light_dir, light_distance)) continue;
+            const f3 lit = mul3(mk(l[12], l[13], l[14]), strength);
+            diffuse = add3(diffuse, mul3(lit, fmaxf(0.0f, dot3(light_dir, hit.normal))));
+            specular = add3(specular, mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, hit.normal), dir)), m[6])));
+        }
+        const f3 dcv = mk(dc[0], dc[1], dc[2]);
+        const f3 direct = add3(mulv(dcv, mul3(diffuse, m[3])), mul3(specular, m[4]));
+        radiance = add3(radiance, mul3(add3(direct, mul3(dcv, e->ambient * m[3])), throughput));
+        throughput *= m[5];
+        if (throughput < 1e-3f) break;
+        dir = reflect3(dir, hit.normal);
+        origin = offset_origin(point, hit.normal, dir);
+    }
+    return radiance;
+}
+
+static float srgb_channel(float v) { /* geom_dev.h:166-172, host branch */
+    const float x = saturatef(v);
+    return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+
+typedef struct {
+    const OrcShade *e;
+    const OrcCamera *cams;
+    int width, height, cpw, spp, frame;
+    uint8_t *rgb;
+} RgbCtx;
+
+static void rgb_rows(void *vctx, int begin, int end) { /* render_pixel_rgb8 and sample_pixel, render_core.h:244-283 */
+    RgbCtx *c = (RgbCtx *)vctx;
+    for (int item = begin; item < end; ++item) {
+        const int view = item / c->height, y = item % c->height, world = view / c->cpw;
+        const OrcCamera *cam = &c->cams[view];
+        for (int x = 0; x < c->width; ++x) {
+            uint32_t rng = pcg_hash((uint32_t)(y * c->width + x) ^ pcg_hash((uint32_t)c->frame + 1u));
+            f3 sum = mk(0.0f, 0.0f, 0.0f);
+            for (int sample = 0; sample < c->spp; ++sample) {
+                const float jx = rand_float(&rng);
+                const float jy = rand_float(&rng);
+                sum = add3(sum, trace_rgb(c->e, world, cam->position, camera_ray(cam, (float)x + jx, (float)y + jy, c->width, c->height), &rng));
+            }
+            const f3 v = mul3(sum, 1.0f / (float)c->spp);
+            uint8_t *out = c->rgb + 3 * ((size_t)view * c->width * c->height + (size_t)y * c->width + x);
+            out[0] = (uint8_t)(saturatef(srgb_channel(v.x)) * 255.0f + 0.5f);
+            out[1] = (uint8_t)(saturatef(srgb_channel(v.y)) * 255.0f + 0.5f);
+            out[2] = (uint8_t)(saturatef(srgb_channel(v.z)) * 255.0f + 0.5f);
+        }
+    }
+}
+
+/* CpuBackend::render_batch (cpu_backend.cpp:239-272) for the RGB picture. Materials: 8 floats each (diffuse
+ * rgb, diffuse albedo, specular albedo, reflectivity, specular exponent, 0) per template instance, per sphere
+ * and for the floor; lights: 14 floats each as the C ABI's ngx_light (position, intensity, radius, falloff,
+ * direction, spot angle, spot softness, colour), turned into the device form here as device_scene.h:77-84
+ * does; sky: up (any length), colour up, colour down, intensity. */
+void orc_render_rgb(const OrcTlasNode *tlas, const int32_t *roots, const float *inv, const int32_t *blas_root,
+                    const OrcNode *blas, const float *tris, const float *attrs, int worlds, const int32_t *world_off,
+                    int sphere_count, const float *spheres, const float *floor4,
+                    const float *inst_mat, const float *sphere_mat, const float *floor_mat, float floor_checker,
+                    int light_count, const float *lights14, float ambient, const float *sky10,
+                    const float *cams, float fov_scale, int width, int height, int cameras_per_world, int spp, int frame,
+                    uint8_t *rgb, int threads) {
+    const OrcScene s = make_scene(tlas, roots, inv, blas_root, blas, tris, attrs, worlds, world_off, sphere_count, spheres, floor4);
+    float dev[8 * 16];
+    for (int i = 0; i < light_count && i < 8; ++i) {
+        const float *l = lights14 + 14 * i;
+        float *o = dev + 16 * i;
+        float dx = l[6], dy = l[7], dz = l[8];
+        const float n = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (n > 0.0f) { dx = dx / n; dy = dy / n; dz = dz / n; } else { dx = 0.0f; dy = 0.0f; dz = -1.0f; }
+        const int spot = l[9] > 0.0f;
+        o[0] = l[0]; o[1] = l[1]; o[2] = l[2]; o[3] = l[3];
+        o[4] = l[4]; o[5] = l[5];
+        o[6] = spot ? cosf(l[9]) : -2.0f;
+        o[7] = spot ? cosf(l[9] * (1.0f - l[10])) : -2.0f;
+        o[8] = dx; o[9] = dy; o[10] = dz; o[11] = 0.0f;
+        o[12] = l[11]; o[13] = l[12]; o[14] = l[13]; o[15] = 0.0f;
+    }
+    OrcShade e;
+    e.s = &s;
+    e.inst_mat = inst_mat; e.sphere_mat = sphere_mat; e.floor_mat = floor_mat; e.floor_checker = floor_checker;
+    e.light_count = light_count; e.lights = dev; e.ambient = ambient;
+    const double un = sqrt((double)sky10[0] * sky10[0] + (double)sky10[1] * sky10[1] + (double)sky10[2] * sky10[2]);
+    e.sky_up = mk((float)(sky10[0] / un), (float)(sky10[1] / un), (float)(sky10[2] / un));
+    e.sky_cu = mk(sky10[3], sky10[4], sky10[5]);
+    e.sky_cd = mk(sky10[6], sky10[7], sky10[8]);
+    e.sky_intensity = sky10[9];
+    const int views = worlds * cameras_per_world;
+    OrcCamera *cc = (OrcCamera *)malloc(sizeof(OrcCamera) * (size_t)(views > 0 ? views : 1));
+    for (int v = 0; v < views; ++v) cc[v] = camera_from_matrix(cams + 12 * (size_t)v, fov_scale, width, height);
+    RgbCtx ctx = {&e, cc, width, height, cameras_per_world, spp, frame, rgb};
+    parallel_chunks(views * height, 4, threads, rgb_rows, &ctx);
+    free(cc);
+}
+
 /* CpuBackend::render_batch (cpu_backend.cpp:239-272) for depth, normal and object ID. fov_scale is
  * tanf(fov / 2), computed by the caller as the reference's host does. Any output may be NULL. */
 void orc_render_batch(const OrcTlasNode *tlas, const int32_t *roots, const float *inv, const int32_t *blas_root,

@@ -61,6 +61,9 @@ class Oracle:
         L.orc_cast_rays_batch.argtypes = [VP] * 7 + [ctypes.c_int, VP, ctypes.c_int, VP, VP, VP, VP, ctypes.c_int,
                                                      ctypes.c_float] + [VP] * 5 + [ctypes.c_int]
         L.orc_pose_grad.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int] + [VP] * 8 + [ctypes.c_int] + [VP] * 6
+        L.orc_render_rgb.argtypes = [VP] * 7 + [ctypes.c_int, VP, ctypes.c_int, VP, VP, VP, VP, VP, ctypes.c_float, ctypes.c_int, VP,
+                                                ctypes.c_float, VP, VP, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, VP, ctypes.c_int]
 
     # ---- bottom level
     def sah_build(self, bounds6, max_leaf):
@@ -167,6 +170,31 @@ class Oracle:
         return dict(depth=None if depth is None else depth.reshape(shape),
                     normal=None if normal is None else normal.reshape(shape + (3,)),
                     object_id=None if oid is None else oid.reshape(shape), hit_inst=hi.reshape(shape), hit_tri=ht.reshape(shape))
+
+    def render_rgb(self, tlas, roots, inv, blas_root, blas, tris, attrs, world_off, cams, fov, width, height, shading,
+                   cameras_per_world=1, spp=1, frame=0, spheres=None, floor=None, threads=None):
+        """The classic RGB picture (render_core.h:146-283) as (views, H, W, 3) bytes; shading as from
+        neraulgraphics_amd.geometry.shading_arrays."""
+        world_off = np.ascontiguousarray(world_off, np.int32)
+        W = len(world_off) - 1
+        V = W * cameras_per_world
+        rgb = np.zeros(V * width * height * 3, np.uint8)
+        sph = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4) if spheres is not None and len(spheres) else None
+        fl = np.ascontiguousarray(floor, np.float32) if floor is not None else None
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        im, sm, fm = f32(shading["instance_materials"]), f32(shading["sphere_materials"]), f32(shading["floor_material"])
+        lights, sky = f32(shading["lights"]), f32(shading["sky"])
+        self.lib.orc_render_rgb(_p(np.ascontiguousarray(tlas, TLAS_NODE)), _p(np.ascontiguousarray(roots, np.int32)),
+                                _p(np.ascontiguousarray(inv, np.float32)), _p(np.ascontiguousarray(blas_root, np.int32)),
+                                _p(np.ascontiguousarray(blas, BLAS_NODE)), _p(np.ascontiguousarray(tris, np.float32)),
+                                _p(attrs), ctypes.c_int(W), _p(world_off),
+                                ctypes.c_int(0 if sph is None else len(sph)), _p(sph), _p(fl),
+                                _p(im), _p(sm), _p(fm), ctypes.c_float(shading["floor_checker_scale"]),
+                                ctypes.c_int(len(lights)), _p(lights), ctypes.c_float(shading["ambient"]), _p(sky),
+                                _p(np.ascontiguousarray(cams, np.float32)), ctypes.c_float(_tanf_half(fov)),
+                                ctypes.c_int(width), ctypes.c_int(height), ctypes.c_int(cameras_per_world), ctypes.c_int(spp),
+                                ctypes.c_int(frame), _p(rgb), ctypes.c_int(threads or os.cpu_count()))
+        return rgb.reshape(V, height, width, 3)
 
     def cast_rays(self, tlas, roots, inv, blas_root, blas, tris, attrs, world_off, origins, directions, max_distance,
                   spheres=None, floor=None, threads=None):
@@ -349,8 +377,16 @@ def forward(cfg, oracle=None, tlas="lbvh", threads=None, body_poses=None, matric
         st = {}
         nodes, roots, inv = SahTlas(o, world_off).update(mats, local_bounds)
     out = {}
-    if "cameras" in cfg and outputs:
+    spheres, floor = cfg.get("spheres"), cfg.get("floor")
+    aov = tuple(k for k in outputs if k != "rgb")
+    if "cameras" in cfg and aov:
         out = o.render(nodes, roots, inv, blas_root, meshes["nodes"], meshes["tris"], meshes["attrs"], world_off, cfg["cameras"],
-                       cfg["fov"], cfg["width"], cfg["height"], cfg["cameras_per_world"], threads=threads, outputs=outputs)
+                       cfg["fov"], cfg["width"], cfg["height"], cfg["cameras_per_world"], spheres=spheres, floor=floor,
+                       threads=threads, outputs=aov)
+    if "cameras" in cfg and "rgb" in outputs:
+        out["rgb"] = o.render_rgb(nodes, roots, inv, blas_root, meshes["nodes"], meshes["tris"], meshes["attrs"], world_off,
+                                  cfg["cameras"], cfg["fov"], cfg["width"], cfg["height"], cfg["shading"],
+                                  cameras_per_world=cfg["cameras_per_world"], spp=cfg.get("spp", 1), frame=cfg.get("frame", 0),
+                                  spheres=spheres, floor=floor, threads=threads)
     return dict(out, state=st, nodes=nodes, roots=roots, inv=inv, matrices=mats, world_off=world_off, meshes=meshes,
                 blas_root=blas_root, local_bounds=local_bounds)

@@ -55,29 +55,29 @@ void cpu_checks() {
     std::puts("scene checks, before any device work");
     const std::vector<ngx::Mesh> meshes = {box(1, 1, 1)};
     const std::vector<ngx::Instance> one = {ngx::Instance{}};
-    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(0, H, 1, 1, {}, {}, meshes, one); }),
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(0, H, 1, 1, {}, {}, {}, meshes, one); }),
           "a frame of zero width is refused");
-    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 0, 1, {}, {}, meshes, one); }), "zero worlds are refused");
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 0, 1, {}, {}, {}, meshes, one); }), "zero worlds are refused");
     ngx::Instance wrong_mesh;
     wrong_mesh.mesh = 3;
-    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, meshes, {wrong_mesh}); }),
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {}, meshes, {wrong_mesh}); }),
           "an instance of a mesh that does not exist is refused");
     ngx::Instance flat;
     flat.transform[10] = 0.0f;
-    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, meshes, {flat}); }),
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {}, meshes, {flat}); }),
           "a singular instance transform is refused");
     ngx::Mesh broken = box(1, 1, 1);
     broken.indices[4] = 99;
-    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {broken}, one); }),
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {}, {broken}, one); }),
           "a triangle index past the last vertex is refused");
-    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 2, 1, {}, {}, meshes, one, {1, 2}); }),
+    check(throws<std::invalid_argument>([&] { ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {}, meshes, one, {1, 2}); }),
           "a world holding more instances than the template is refused");
 }
 
 void gpu_checks() {
     std::puts("worlds do not see one another");
     {
-        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
         std::vector<float> poses;
         add_translation(poses, 0, 0, -8);
         add_translation(poses, 0, 0, -3);
@@ -99,12 +99,16 @@ void gpu_checks() {
         const ngx::HostRayHits near = b.cast_rays_host({0, 0, 0, 0, 0, 0}, {0, 0, -1, 0, 0, -1}, 1, 6.9f);
         check(std::isinf(near.distance[0]) && near.object_id[0] == ngx::OBJECT_NONE && std::fabs(near.distance[1] - 2.0f) < 1e-3f,
               "a hit beyond max_distance is a miss");
-        check(throws<ngx::Unsupported>([&] { b.render(cams.data(), FOV, 1, ngx::OUTPUT_RGB); }), "the RGB picture is not drawn");
+        // a ray that leaves the scene sees the sky gradient (render_core.h:184-188), in sRGB bytes
+        const ngx::HostBatchFrame rgb = b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_RGB);
+        check(rgb.rgb.size() == 2u * W * H * 3, "the RGB picture has 3 bytes a pixel");
+        const uint8_t *corner = &rgb.rgb[0];
+        check(corner[2] > corner[1] && corner[1] > corner[0], "the sky above the horizon is blue (blue > green > red)");
     }
 
     std::puts("one-instance worlds and a pile of coincident boxes");
     {
-        ngx::BatchRenderer single(W, H, 4, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        ngx::BatchRenderer single(W, H, 4, 1, {}, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
         std::vector<float> poses, cams;
         for (int w = 0; w < 4; ++w) {
             add_translation(poses, 0, 0, -4.0f - (float)w);
@@ -117,7 +121,7 @@ void gpu_checks() {
         check(ok, "a world whose tree is one leaf renders right (depths 3, 4, 5, 6)");
 
         std::vector<ngx::Instance> pile(1000);
-        ngx::BatchRenderer many(W, H, 1, 1, {}, {}, {box(0.2f, 0.2f, 0.2f)}, pile);
+        ngx::BatchRenderer many(W, H, 1, 1, {}, {}, {}, {box(0.2f, 0.2f, 0.2f)}, pile);
         std::vector<float> same;
         for (int i = 0; i < 1000; ++i) add_translation(same, 0, 0, -5);
         many.set_instance_transforms(same.data());
@@ -129,7 +133,7 @@ void gpu_checks() {
 
     std::puts("bad input after creation is refused and changes nothing");
     {
-        ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
         std::vector<float> good, cams;
         add_translation(good, 0, 0, -6);
         add_camera(cams);
@@ -155,7 +159,7 @@ void gpu_checks() {
     std::puts("poses from bodies");
     {
         // one body per world carrying the box with an offset of 1 along x and a 90 degree turn about y
-        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
         ngx::BodyBinding bind;
         bind.bodies_per_world = 1;
         bind.body_of_instance = {0};

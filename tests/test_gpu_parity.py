@@ -323,3 +323,81 @@ def render_target(r, cfg):
     d = r.render(cfg["cameras"], cfg["fov"], depth=True)["depth"]
     r.set_body_poses(cfg["body_poses"])
     return d
+
+
+# ------------------------------------------------------------------------------------------ RGB picture --
+
+def _rgb_pair(oracle, cfg, spheres=(), floor=None, lights=(), ambient=0.0, env=None, spp=1, frame=0):
+    """The GPU's RGB picture and the oracle's for one scene."""
+    import neraulgraphics_amd as ng
+    floor = floor if floor is not None else ng.Plane()
+    r = workloads.make_renderer(cfg, spheres=list(spheres), floor=floor, lights=list(lights))
+    if ambient:
+        r.set_ambient(ambient)
+    if env is not None:
+        r.set_environment(env)
+    got = r.render(cfg["cameras"], cfg["fov"], spp, rgb=True, frame=frame)["rgb"]
+    c = dict(cfg)
+    c["spheres"] = np.array([[*s.center, s.radius] for s in spheres], np.float32) if spheres else None
+    c["floor"] = np.array([floor.y, floor.half_extent_x, floor.z_min, floor.z_max], np.float32)
+    c["shading"] = ng.shading_arrays(cfg["instances"], list(spheres), floor, list(lights), ambient, env)
+    c["spp"], c["frame"] = spp, frame
+    want = forward(c, oracle, outputs=("rgb",))["rgb"]
+    return got.astype(np.int32), want.astype(np.int32)
+
+
+def test_rgb_picture_matches_the_oracle(gpu, oracle):
+    """tests/test_batch.cpp:149-181's bar for the picture: mean colour difference below 0.05 of a level per channel.
+    Eight worlds of 32 instanced meshes, with materials of every kind (matte, glossy, mirror-like), two spheres, a
+    checker floor, a soft sphere light, a spotlight with falloff and an ambient term, two samples a pixel."""
+    import neraulgraphics_amd as ng
+    _say("rgb parity: 8 worlds x 32 instances, 2 lights, spheres, floor, 2 spp")
+    cfg = workloads.config(1, worlds=8)
+    mats = [ng.Material((0.9, 0.2, 0.2), 0.9, 0.1, 0.0, 50.0), ng.Material((0.2, 0.8, 0.3), 0.7, 0.5, 0.3, 200.0),
+            ng.Material((0.7, 0.7, 0.9), 0.2, 0.3, 0.8, 20.0)]
+    cfg["instances"] = [ng.Instance(i.mesh, material=mats[k % 3]) for k, i in enumerate(cfg["instances"])]
+    spheres = [ng.Sphere((0.5, -1.0, -9.0), 1.0, ng.Material((0.9, 0.9, 0.2), 0.8, 0.2, 0.5, 80.0)),
+               ng.Sphere((-2.0, 1.0, -7.0), 0.7)]
+    floor = ng.Plane(y=-4.0, half_extent_x=30.0, z_min=-60.0, z_max=5.0, checker_scale=0.5)
+    lights = [ng.Light((2.0, 6.0, -4.0), intensity=1.2, radius=0.8),
+              ng.Light((-3.0, 4.0, -2.0), intensity=2.0, falloff=8.0, direction=(0.3, -1.0, -0.8), spot_angle=0.6, color=(1.0, 0.8, 0.6))]
+    got, want = _rgb_pair(oracle, cfg, spheres, floor, lights, ambient=0.15, spp=2, frame=3)
+    diff = np.abs(got - want)
+    mean = float(diff.mean())
+    _say(f"mean colour difference {mean:.4f} of a level; {int((diff.max(axis=-1) > 2).sum())} of {diff.shape[0] * diff.shape[1] * diff.shape[2]} "
+         f"pixels differ by more than 2 levels; bit-identical {float(np.mean(np.all(diff == 0, axis=-1))):.4f}")
+    assert mean < 0.05
+    assert len(np.unique(want.reshape(-1, 3), axis=0)) > 1000  # a picture, not a flat colour
+
+
+def test_rgb_known_answers_on_the_device(gpu, oracle):
+    """The oracle's closed-form cases (tests/test_oracle_rgb.py) through the device: a sky of one colour, a floor lit
+    from straight above, a shadowed point, a mirror floor."""
+    import neraulgraphics_amd as ng
+    from .test_oracle_rgb import scene, srgb_byte
+    _say("rgb known answers on the device")
+
+    def gpu_rgb(cfg, spheres=(), floor=None, lights=(), ambient=0.0, env=None):
+        got, want = _rgb_pair(oracle, cfg, spheres, floor, lights, ambient, env, spp=cfg["spp"])
+        assert np.array_equal(got, want) or np.abs(got - want).mean() < 0.05
+        return got[0]
+
+    c = (0.2, 0.5, 0.8)
+    env = ng.Environment(color_up=c, color_down=c, intensity=0.75)
+    sky = gpu_rgb(scene(env=env), env=env)
+    assert np.all(sky.reshape(-1, 3) == srgb_byte(np.array(c, np.float32) * np.float32(0.75)))
+    fm = ng.Material((1.0, 1.0, 1.0), 0.8, 0.0, 0.0, 1.0)
+    floor = ng.Plane(y=-1.0, half_extent_x=50.0, z_min=-50.0, z_max=50.0, checker_scale=0.0, material=fm)
+    light = ng.Light((0.0, 3.0, 0.0), intensity=1.5)
+    down = ng.look_at((0, 0, 0), (0, -1, 0), up=(0, 0, -1))
+    lit = gpu_rgb(scene(lights=[light], floor=floor, cam=down, size=33), floor=floor, lights=[light])
+    want = srgb_byte(np.float32(0.3) * np.array([1.0, 0.7, 0.3], np.float32) * np.float32(1.5 * 0.8))
+    assert np.all(np.abs(lit[16, 16] - want) <= 1)
+    blocker = ng.Sphere((0.0, 1.5, 0.0), 0.5)
+    dark = gpu_rgb(scene(lights=[light], spheres=[blocker], floor=floor, cam=down, size=33), [blocker], floor, [light])
+    assert np.all(dark[16, 16] == 0)
+    mirror = ng.Plane(y=-1.0, half_extent_x=50.0, z_min=-50.0, z_max=50.0, checker_scale=0.0,
+                      material=ng.Material((1.0, 1.0, 1.0), 0.0, 0.0, 1.0, 1.0))
+    env2 = ng.Environment(color_up=(0.3, 0.6, 0.1), color_down=(0.3, 0.6, 0.1))
+    m = gpu_rgb(scene(floor=mirror, cam=down, env=env2), floor=mirror, env=env2)
+    assert np.all(m.reshape(-1, 3) == srgb_byte(np.array((0.3, 0.6, 0.1), np.float32)))
