@@ -3,13 +3,13 @@
 A step is one pass of the hot path over one batch: new body poses for every world (one kernel composes them
 in FP64 and rebuilds every world's top-level BVH), then depth + object-ID images of every view (one trace
 kernel). The workload is 4096 worlds at 128x128 with U{8..256} instances per world out of a 256-instance
-template carried by 64 bodies (neraulgraphics_amd.workloads, configuration 2); four resident pose sets are
+template carried by 64 bodies (neraulgraphics_amd.workloads, body_worlds); four resident pose sets are
 cycled so every step has new poses. Inputs are resident in HBM before timing; the step is replayed from a
 HIP graph. Before timing, the trace configuration is autotuned on the benchmark's own cameras (--no-tune keeps
 the defaults).
 
-Also reported, on the same JSON line: LiDAR ray queries of configuration 3 (1024 worlds x 65,536 rays,
-rays and results resident on the device); the forward + pose-backward pass of configuration 4 (1024 worlds,
+Also reported, on the same JSON line: LiDAR ray queries (lidar_sweeps: 1024 worlds x 65,536 rays,
+rays and results resident on the device); the forward + pose-backward pass (posed_bodies: 1024 worlds,
 64x64 depth, 32 bodies) and its ratio to the forward alone; the render kernel against the HBM roofline;
 and the reference CPU algorithm (this repository's C restatement of the reference CPU backend: FP64 body
 poses, per-world SAH top level with refit, the same traversal, rows of views in chunks of 4 over threads)
@@ -56,9 +56,9 @@ def timed_steps(step, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
-def bench_config2(args, torch):
+def bench_body_worlds(args, torch):
     from neraulgraphics_amd import workloads
-    cfg = workloads.config(2)
+    cfg = workloads.make("body_worlds")
     r = workloads.make_renderer(cfg)
     cams = torch.from_numpy(cfg["cameras"]).cuda()
     tuned = None
@@ -137,11 +137,11 @@ def bench_config2(args, torch):
                 hit_frac=hit_frac, renderer=r, trace_config=r.trace_config)
 
 
-def bench_config3(args, torch):
+def bench_lidar(args, torch):
     """LiDAR: 1024 worlds x 65,536 rays over 100,000 instances of a 138,000-triangle grid; rays resident on the
     device, results left there (zero-copy)."""
     from neraulgraphics_amd import workloads
-    cfg = workloads.config(3)
+    cfg = workloads.make("lidar_sweeps")
     r = workloads.make_renderer(cfg)
     o = torch.from_numpy(cfg["origins"]).cuda()
     d = torch.from_numpy(cfg["directions"]).cuda()
@@ -157,10 +157,10 @@ def bench_config3(args, torch):
                 trace_config=r.trace_config)
 
 
-def bench_config4(args, torch):
+def bench_posed_bodies(args, torch):
     from neraulgraphics_amd import workloads
     from neraulgraphics_amd.autograd import render_depth
-    cfg = workloads.config(4)
+    cfg = workloads.make("posed_bodies")
     r = workloads.make_renderer(cfg)
     cams = torch.from_numpy(cfg["cameras"]).cuda()
     r.set_body_poses(cfg["target_poses"])
@@ -187,13 +187,13 @@ def bench_config4(args, torch):
 
 
 def cpu_baseline(seconds=12.0):
-    """The reference CPU algorithm on this host, on a sample of configuration 2's worlds."""
+    """The reference CPU algorithm on this host, on a sample of the body_worlds workload's worlds."""
     from neraulgraphics_amd import workloads
     from oracle.oracle import Oracle, SahTlas
     o = Oracle()
     threads = min(16, os.cpu_count() or 1)
     sample = 64
-    cfg = workloads.config(2, worlds=sample)
+    cfg = workloads.make("body_worlds", worlds=sample)
     counts = cfg["counts"]
     world_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     meshes = o.build_meshes(cfg["meshes"])
@@ -216,7 +216,7 @@ def cpu_baseline(seconds=12.0):
         if el > seconds:
             break
     return dict(value=images / el, unit="images/s", cores=threads, kind="port",
-                sample=f"{images} images of configuration 2 ({sample} worlds a step, 128x128 depth+ID, body poses, "
+                sample=f"{images} images of body_worlds ({sample} worlds a step, 128x128 depth+ID, body poses, "
                        f"per-world SAH top level with refit) in {el:.1f} s on {threads} threads")
 
 
@@ -252,25 +252,26 @@ def main():
         torch.cuda.synchronize()
         print("# copy calibration: 3 copies of 536870912 bytes", file=sys.stderr, flush=True)
         del src, dst
-    print("# config 2: 4096 worlds 128x128 depth+ID", file=sys.stderr, flush=True)
-    c2 = bench_config2(args, torch)
+    print("# body_worlds: 4096 worlds 128x128 depth+ID", file=sys.stderr, flush=True)
+    c2 = bench_body_worlds(args, torch)
     sec = c2["sec"]
     rb = render_bytes(c2["views"], 128, 128, c2["instances"])
     achieved = rb / (c2["render_ms"] * 1e-3) / 1e9
     print(f"# step {sec * 1e3:.3f} ms (build {c2['build_ms']:.3f} ms, render {c2['render_ms']:.3f} ms)", file=sys.stderr, flush=True)
     c3 = None
     if not args.no_lidar:
-        print("# config 3: LiDAR ray queries", file=sys.stderr, flush=True)
-        c3 = bench_config3(args, torch)
+        print("# lidar_sweeps: LiDAR ray queries", file=sys.stderr, flush=True)
+        c3 = bench_lidar(args, torch)
     c4 = None
     if not args.no_bwd:
-        print("# config 4: forward and forward+backward", file=sys.stderr, flush=True)
-        c4 = bench_config4(args, torch)
+        print("# posed_bodies: forward and forward+backward", file=sys.stderr, flush=True)
+        c4 = bench_posed_bodies(args, torch)
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_seconds)
-    prof = os.path.join(ROOT, "profiles", "render_kernel_config2_traffic.json")
+    kernel = "render_persistent_kernel" if c2["trace_config"].get("persistent") else "render_kernel"
+    prof = os.path.join(ROOT, "profiles", f"{kernel}_body_worlds_traffic.json")
     traffic = None
     if os.path.exists(prof):
         with open(prof) as f:
@@ -288,7 +289,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "configuration 2: 4096 worlds, 128x128 depth+object-ID, U{8..256} of a 256-instance "
+        "config": {"workload": "body_worlds: 4096 worlds, 128x128 depth+object-ID, U{8..256} of a 256-instance "
                                "template per world, 64 body poses per world, new poses every step",
                    "worlds": 4096, "width": 128, "height": 128, "instances": c2["instances"], "outputs": "depth+ID",
                    "hip_graph": c2["graph"], "trace_config": c2["trace_config"]},
@@ -296,14 +297,14 @@ def main():
         "build_ms": c2["build_ms"],
         "render_ms": c2["render_ms"],
         "hit_fraction": c2["hit_frac"],
-        "roofline": {"bound": "hbm", "kernel": "render_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": rb},
         "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
     }
     if c3:
-        line.update(lidar_rays_per_s_config3=c3["rays_per_s"], lidar_ms_config3=c3["ms"], lidar_hit_fraction=c3["hit_frac"])
+        line.update(lidar_rays_per_s=c3["rays_per_s"], lidar_ms=c3["ms"], lidar_hit_fraction=c3["hit_frac"])
     if c4:
-        line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s_config4=c4["fwd_images_per_s"],
+        line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])
     line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

@@ -1,21 +1,22 @@
-"""Seeded synthetic workloads: the benchmark configurations of this backend, at full size or scaled down.
+"""Seeded synthetic workloads of this backend, at full size or scaled down, by name (`make(name, ...)`).
 
-  0  one world, 64x64 depth+ID, 16 icospheres (320 triangles) at U[-5,5]^3 + (0,0,-10), random unit
-     quaternion, uniform scale U[0.5,1.5]; camera at the origin looking down -z; fov 1.0.
-  1  4096 worlds, 64x64 depth+ID; 32 instances per world over three shared meshes (icosphere 320, box 12,
-     displaced grid 2000), placed as in 0 per world; one camera per world at U[-2,2]^3 looking at the
-     world's instance centroid; new instance matrices every step.
-  2  4096 worlds, 128x128 depth+ID; a 256-instance template over the three meshes, U{8..256} instances per
-     world; instances carried by 64 bodies (body i mod 64, local offset U[-0.5,0.5]^3 and random rotation,
-     per-axis scale U[0.5,1.5]); body poses at U[-5,5]^3 + (0,0,-10) with random rotations; new body poses
-     every step.
-  3  LiDAR: 1024 worlds of 65,536 rays (64 beams over elevations [-15, 15] degrees x 1024 azimuths over
-     [0, 360)) from an origin U[-1,1]^3; 100,000 instances over all worlds (672 worlds of 98, 352 of 97) of
-     one 138,000-triangle displaced grid at U[-10,10]^3; max distance 100, miss = inf.
-  4  differentiable depth: 1024 worlds, 64x64 depth, 32 instances as in 1, each on its own body; the target
-     is rendered from poses perturbed by N(0, 0.01) in translation and 1 degree about a random axis.
+  one_world_icospheres  one world, 64x64 depth+ID, 16 icospheres (320 triangles) at U[-5,5]^3 + (0,0,-10),
+      random unit quaternion, uniform scale U[0.5,1.5]; camera at the origin looking down -z; fov 1.0. Seed 1000.
+  instanced_worlds  4096 worlds, 64x64 depth+ID; 32 instances per world over three shared meshes (icosphere
+      320, box 12, displaced grid 2000), placed as above per world; one camera per world at U[-2,2]^3 looking
+      at the world's instance centroid; new instance matrices every step. Seed 1001.
+  body_worlds  4096 worlds, 128x128 depth+ID; a 256-instance template over the three meshes, U{8..256}
+      instances per world; instances carried by 64 bodies (body i mod 64, local offset U[-0.5,0.5]^3 and
+      random rotation, per-axis scale U[0.5,1.5]); body poses at U[-5,5]^3 + (0,0,-10) with random rotations;
+      new body poses every step. Seed 1002.
+  lidar_sweeps  1024 worlds of 65,536 rays (64 beams over elevations [-15, 15] degrees x 1024 azimuths over
+      [0, 360)) from an origin U[-1,1]^3; 100,000 instances over all worlds (672 worlds of 98, 352 of 97) of
+      one 138,000-triangle displaced grid at U[-10,10]^3; max distance 100, miss = inf. Seed 1003.
+  posed_bodies  differentiable depth: 1024 worlds, 64x64 depth, 32 instances as in instanced_worlds, each on
+      its own body; the target is rendered from poses perturbed by N(0, 0.01) in translation and 1 degree
+      about a random axis. Seed 1004.
 
-Seed = 1000 + config. Quaternions are Gaussian four-vectors normalized. Everything is float32."""
+Quaternions are Gaussian four-vectors normalized. Everything is float32."""
 import numpy as np
 
 from .geometry import Instance, Transform, look_at, make_box, make_grid, make_icosphere, make_transforms
@@ -48,18 +49,23 @@ def _cameras_at(rng, targets):
     return cams
 
 
-def config(index, worlds=None, rays_per_world=None, seed=None):
-    """The workload as a dict (see the module docstring); `worlds` and `rays_per_world` scale it down."""
-    rng = np.random.default_rng(1000 + index if seed is None else seed)
-    if index == 0:
+NAMES = ("one_world_icospheres", "instanced_worlds", "body_worlds", "lidar_sweeps", "posed_bodies")
+
+
+def make(name, worlds=None, rays_per_world=None, seed=None):
+    """The named workload as a dict (see the module docstring); `worlds` and `rays_per_world` scale it down."""
+    if name not in NAMES:
+        raise ValueError(f"no workload {name!r}; the workloads are {', '.join(NAMES)}")
+    rng = np.random.default_rng(1000 + NAMES.index(name) if seed is None else seed)
+    if name == "one_world_icospheres":
         mesh = make_icosphere(1.0, 2)
         pos, quat, scale, mats = _placed_worlds(rng, 1, 16)
         cams = np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]], np.float32)
         return dict(width=64, height=64, worlds=1, cameras_per_world=1, meshes=[mesh],
                     instances=[Instance(0, Transform(mats[0, i].reshape(3, 4))) for i in range(16)], counts=None,
                     mode="matrices", matrices=mats, cameras=cams, fov=FOV)
-    if index in (1, 4):
-        W = worlds or (4096 if index == 1 else 1024)
+    if name in ("instanced_worlds", "posed_bodies"):
+        W = worlds or (4096 if name == "instanced_worlds" else 1024)
         P = 32
         meshes = three_meshes()
         pos, quat, scale, mats = _placed_worlds(rng, W, P)
@@ -67,7 +73,7 @@ def config(index, worlds=None, rays_per_world=None, seed=None):
         insts = [Instance(i % 3) for i in range(P)]
         out = dict(width=64, height=64, worlds=W, cameras_per_world=1, meshes=meshes, instances=insts, counts=None,
                    cameras=cams, fov=FOV)
-        if index == 1:
+        if name == "instanced_worlds":
             out.update(mode="matrices", matrices=mats, positions=pos, quaternions=quat, scales=scale)
             return out
         # each instance on its own body: the body pose is the instance's position and rotation
@@ -85,7 +91,7 @@ def config(index, worlds=None, rays_per_world=None, seed=None):
         out.update(mode="bodies", bodies=P, body_of_instance=np.arange(P, dtype=np.int32), local_pose=local, scale=sc,
                    body_poses=poses, target_poses=target.astype(np.float32))
         return out
-    if index == 2:
+    if name == "body_worlds":
         W = worlds or 4096
         P, B = 256, 64
         meshes = three_meshes()
@@ -100,7 +106,7 @@ def config(index, worlds=None, rays_per_world=None, seed=None):
         return dict(width=128, height=128, worlds=W, cameras_per_world=1, meshes=meshes, instances=insts, counts=counts,
                     mode="bodies", bodies=B, body_of_instance=boi, local_pose=local, scale=sc, body_poses=poses,
                     cameras=cams, fov=FOV)
-    if index == 3:
+    if name == "lidar_sweeps":
         W = worlds or 1024
         R = rays_per_world or 65536
         mesh = make_grid(460, 150, 4.0, 4.0 * 150 / 460, 0.05, seed=11)
@@ -123,7 +129,7 @@ def config(index, worlds=None, rays_per_world=None, seed=None):
         insts = [Instance(0) for _ in range(P)]
         return dict(width=64, height=64, worlds=W, cameras_per_world=1, meshes=[mesh], instances=insts, counts=counts,
                     mode="matrices", matrices=mats, origins=origins, directions=dirs, max_distance=100.0)
-    raise ValueError(f"no configuration {index}")
+    raise AssertionError(name)
 
 
 class _Lcg:

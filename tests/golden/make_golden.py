@@ -42,7 +42,7 @@ def main():
     scale = rng.uniform(0.5, 1.5, 16).astype(np.float32)
     v["transform_pos"], v["transform_quat"], v["transform_scale"] = pos, quat, scale
     v["transform_out"] = np.stack([ref.make_transform(pos[i], quat[i], float(scale[i])) for i in range(16)])
-    cfg = workloads.config(1, worlds=2)
+    cfg = workloads.make("instanced_worlds", worlds=2)
     W, H = 32, 32
     cams = cfg["cameras"]
     o, d, f = ref.camera_rays(cams, cfg["fov"], W, H)

@@ -44,10 +44,10 @@ def _check_build(r, ref, world_off):
         assert np.array_equal(g["lo"], e["lo"]) and np.array_equal(g["hi"], e["hi"]), f"world {w} boxes differ"
 
 
-@pytest.mark.parametrize("index,worlds", [(1, 64), (2, 64), (3, 16)])
-def test_top_level_build_matches_the_reference_algorithm(gpu, oracle, index, worlds):
-    _say(f"build parity: config {index}, {worlds} worlds")
-    cfg = workloads.config(index, worlds=worlds, rays_per_world=64)
+@pytest.mark.parametrize("name,worlds", [("instanced_worlds", 64), ("body_worlds", 64), ("lidar_sweeps", 16)])
+def test_top_level_build_matches_the_reference_algorithm(gpu, oracle, name, worlds):
+    _say(f"build parity: {name}, {worlds} worlds")
+    cfg = workloads.make(name, worlds=worlds, rays_per_world=64)
     r = workloads.make_renderer(cfg)
     r.keep_build_intermediates(True)
     if cfg["mode"] == "bodies":
@@ -96,37 +96,38 @@ def test_morton_trees_see_what_the_sah_trees_see(gpu, oracle, worlds, per):
 
 
 @pytest.mark.parametrize("walk", list(WALKS))
-@pytest.mark.parametrize("index,worlds", [(0, None), (1, 32), (2, 16), (4, 16)])
-def test_depth_normal_id_match_the_oracle(gpu, oracle, index, worlds, walk):
-    _say(f"render parity: config {index}, worlds {worlds}, {walk} walk")
-    cfg = workloads.config(index, worlds=worlds)
+@pytest.mark.parametrize("name,worlds", [("one_world_icospheres", None), ("instanced_worlds", 32), ("body_worlds", 16),
+                                         ("posed_bodies", 16)])
+def test_depth_normal_id_match_the_oracle(gpu, oracle, name, worlds, walk):
+    _say(f"render parity: {name}, worlds {worlds}, {walk} walk")
+    cfg = workloads.make(name, worlds=worlds)
     r = workloads.make_renderer(cfg)
     r.set_trace_config(**WALKS[walk])
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
-    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, f"config {index}")
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, name)
     _say(rep)
     assert rep["hits"] > 0.05 * rep["pixels"]
 
 
 @pytest.mark.parametrize("tile_w,pair_limit,persistent", [(16, 16, True), (32, 1, False), (8, 64, True)])
 def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_limit, persistent):
-    _say(f"render parity: config 2, 16 worlds, tile_w {tile_w}, pair_limit {pair_limit}, persistent {persistent}")
-    cfg = workloads.config(2, worlds=16)
+    _say(f"render parity: body_worlds, 16 worlds, tile_w {tile_w}, pair_limit {pair_limit}, persistent {persistent}")
+    cfg = workloads.make("body_worlds", worlds=16)
     r = workloads.make_renderer(cfg)
     r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
     assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent),
                                   lane_min_triangles=1000, sort_rays=1)
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
-    _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 2"))
+    _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "body_worlds"))
 
 
 def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, monkeypatch):
     monkeypatch.setenv("NGX_TUNE_CACHE", str(tmp_path / "tune.json"))
-    cfg = workloads.config(1, worlds=64)
+    cfg = workloads.make("instanced_worlds", worlds=64)
     r = workloads.make_renderer(cfg)
-    _say("autotune: config 1, 64 worlds")
+    _say("autotune: instanced_worlds, 64 worlds")
     got = r.autotune(cfg["cameras"], cfg["fov"], reps=2)
     _say(got)
     assert got["ms"] > 0 and got["tile_w"] in (8, 16, 32, 64)
@@ -135,20 +136,20 @@ def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, 
     assert again["ms"] is None and {k: again[k] for k in r.trace_config} == r.trace_config
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
-    compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "config 1 tuned")
+    compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "instanced_worlds tuned")
 
 
 @pytest.mark.parametrize("walk,sort_rays", [("packet", True), ("packet", False), ("hybrid", True), ("lanes", False)])
 def test_ray_queries_match_the_oracle(gpu, oracle, walk, sort_rays):
-    _say(f"ray parity: config 3, 8 worlds x 4096 rays, {walk} walk, direction order {sort_rays}")
-    cfg = workloads.config(3, worlds=8, rays_per_world=4096)
+    _say(f"ray parity: lidar_sweeps, 8 worlds x 4096 rays, {walk} walk, direction order {sort_rays}")
+    cfg = workloads.make("lidar_sweeps", worlds=8, rays_per_world=4096)
     r = workloads.make_renderer(cfg)
     r.set_trace_config(**WALKS[walk], sort_rays=sort_rays)
     out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
     st = forward(cfg, oracle, outputs=())
     ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
                            st["meshes"]["attrs"], st["world_off"], cfg["origins"], cfg["directions"], cfg["max_distance"])
-    rep = compare_rays(out, ref, "config 3")
+    rep = compare_rays(out, ref, "lidar_sweeps")
     _say(rep)
     assert rep["hits"] > 0
 
@@ -156,7 +157,7 @@ def test_ray_queries_match_the_oracle(gpu, oracle, walk, sort_rays):
 @pytest.mark.parametrize("walk", list(WALKS))
 def test_rays_of_uneven_worlds_per_wave(gpu, oracle, walk):
     _say(f"ray parity: 37 rays per world, so waves straddle worlds, {walk} walk")
-    cfg = workloads.config(3, worlds=8, rays_per_world=37 * 64)
+    cfg = workloads.make("lidar_sweeps", worlds=8, rays_per_world=37 * 64)
     o = cfg["origins"][:, :37]
     d = cfg["directions"][:, ::64][:, :37]
     r = workloads.make_renderer(cfg)
@@ -201,7 +202,7 @@ def test_single_instance_worlds_and_a_pile_of_coincident_boxes(gpu):
 def test_spheres_and_floor(gpu, oracle):
     """Analytic spheres and the floor before the meshes (render_core.h:78-112); a mesh must be strictly closer."""
     import neraulgraphics_amd as ng
-    cfg = workloads.config(1, worlds=4)
+    cfg = workloads.make("instanced_worlds", worlds=4)
     spheres = [ng.Sphere((0.5, -1.0, -9.0), 1.0), ng.Sphere((-2.0, 1.0, -7.0), 0.7)]
     floor = ng.Plane(y=-2.0, half_extent_x=30.0, z_min=-60.0, z_max=5.0)
     r = workloads.make_renderer(cfg, spheres=spheres, floor=floor)
@@ -219,7 +220,7 @@ def test_spheres_and_floor(gpu, oracle):
 
 def test_zero_copy_views_are_overwritten_by_the_next_render(gpu):
     import torch
-    cfg = workloads.config(1, worlds=4)
+    cfg = workloads.make("instanced_worlds", worlds=4)
     r = workloads.make_renderer(cfg)
     cams = torch.from_numpy(cfg["cameras"]).cuda()
     first = r.render(cams, cfg["fov"], depth=True, zero_copy=True)["depth"]
@@ -233,7 +234,7 @@ def test_zero_copy_views_are_overwritten_by_the_next_render(gpu):
 
 def test_device_poses_equal_host_poses(gpu):
     import torch
-    cfg = workloads.config(2, worlds=8)
+    cfg = workloads.make("body_worlds", worlds=8)
     r = workloads.make_renderer(cfg)
     host = r.render(cfg["cameras"], cfg["fov"], depth=True, object_ids=True)
     r.set_body_poses(torch.from_numpy(cfg["body_poses"]).cuda())
@@ -244,7 +245,7 @@ def test_device_poses_equal_host_poses(gpu):
 
 
 def test_bad_input_is_rejected_and_changes_nothing(gpu):
-    cfg = workloads.config(1, worlds=2)
+    cfg = workloads.make("instanced_worlds", worlds=2)
     r = workloads.make_renderer(cfg)
     before = r.render(cfg["cameras"], cfg["fov"], depth=True)["depth"]
     bad = cfg["matrices"].copy()
@@ -276,8 +277,8 @@ def test_bad_input_is_rejected_and_changes_nothing(gpu):
 def test_pose_gradient_matches_fp64_closed_form_and_differences(gpu, oracle):
     import torch
     from neraulgraphics_amd.autograd import render_depth
-    _say("backward: config 4, 8 worlds")
-    cfg = workloads.config(4, worlds=8)
+    _say("backward: posed_bodies, 8 worlds")
+    cfg = workloads.make("posed_bodies", worlds=8)
     r = workloads.make_renderer(cfg)
     target = torch.from_numpy(render_target(r, cfg)).cuda()
     poses = torch.from_numpy(cfg["body_poses"]).cuda().requires_grad_(True)
@@ -352,7 +353,7 @@ def test_rgb_picture_matches_the_oracle(gpu, oracle):
     checker floor, a soft sphere light, a spotlight with falloff and an ambient term, two samples a pixel."""
     import neraulgraphics_amd as ng
     _say("rgb parity: 8 worlds x 32 instances, 2 lights, spheres, floor, 2 spp")
-    cfg = workloads.config(1, worlds=8)
+    cfg = workloads.make("instanced_worlds", worlds=8)
     mats = [ng.Material((0.9, 0.2, 0.2), 0.9, 0.1, 0.0, 50.0), ng.Material((0.2, 0.8, 0.3), 0.7, 0.5, 0.3, 200.0),
             ng.Material((0.7, 0.7, 0.9), 0.2, 0.3, 0.8, 20.0)]
     cfg["instances"] = [ng.Instance(i.mesh, material=mats[k % 3]) for k, i in enumerate(cfg["instances"])]

@@ -57,7 +57,7 @@ def test_ray_distance_and_max_distance(oracle):
 
 def test_depth_is_ray_distance_along_the_view_axis(oracle):
     """tests/test_renderer.cpp:822-834: depth equals the distance of the pixel's ray times cos of its angle."""
-    cfg = workloads.config(1, worlds=2)
+    cfg = workloads.make("instanced_worlds", worlds=2)
     out = forward(cfg, oracle)
     W, H = cfg["width"], cfg["height"]
     o = np.zeros((2, H * W, 3), np.float32)
@@ -115,7 +115,7 @@ def test_body_poses_compose_as_the_reference_test_does(oracle):
 
 def test_body_poses_against_a_library_rotation(oracle):
     from scipy.spatial.transform import Rotation
-    cfg = workloads.config(2, worlds=3)
+    cfg = workloads.make("body_worlds", worlds=3)
     off = np.concatenate([[0], np.cumsum(cfg["counts"])]).astype(np.int32)
     got = oracle.body_poses_to_matrices(off, cfg["bodies"], cfg["body_of_instance"], cfg["local_pose"], cfg["scale"], cfg["body_poses"])
     for w in range(3):
@@ -138,7 +138,7 @@ def test_zero_quaternion_is_the_identity(oracle):
 def test_fp64_pose_gradient_equals_central_differences(oracle):
     """The oracle's closed-form dt/d(pose) through a fixed triangle against FP64 central differences: a wrong
     sign, a dropped normalization term or a transposed rotation is off by O(1)."""
-    cfg = workloads.config(4, worlds=3)
+    cfg = workloads.make("posed_bodies", worlds=3)
     out = forward(cfg, oracle)
     V, H, W = out["depth"].shape
     rng = np.random.default_rng(1)

@@ -52,7 +52,7 @@ def test_camera_rays_are_the_references(oracle):
 
 def test_closest_hits_are_the_references(oracle):
     """Every pixel's instance, triangle and distance equal the reference's mesh_intersect."""
-    cfg = workloads.config(1, worlds=2)
+    cfg = workloads.make("instanced_worlds", worlds=2)
     cfg.update(width=32, height=32)
     assert np.array_equal(cfg["matrices"], GOLDEN["hits_matrices"])
     out = forward(cfg, oracle)
@@ -82,7 +82,7 @@ def test_sah_build_and_refit_match_the_live_reference(oracle, ref):
 def test_traversal_matches_the_live_reference_over_both_top_levels(oracle, ref):
     """The oracle's render equals the reference's mesh_intersect per pixel, over the LBVH and over the
     reference CPU backend's per-world SAH trees."""
-    cfg = workloads.config(2, worlds=6)
+    cfg = workloads.make("body_worlds", worlds=6)
     for tlas in ("lbvh", "sah"):
         out = forward(cfg, oracle, tlas=tlas)
         W, H = cfg["width"], cfg["height"]
@@ -106,8 +106,8 @@ def test_spread_bits_interleaves():
         assert o.spread_bits(v) == want
 
 
-def _lbvh(oracle, index=2, worlds=24, seed=None):
-    cfg = workloads.config(index, worlds=worlds, seed=seed)
+def _lbvh(oracle, name="body_worlds", worlds=24, seed=None):
+    cfg = workloads.make(name, worlds=worlds, seed=seed)
     return cfg, forward(cfg, oracle, outputs=())
 
 
@@ -230,8 +230,8 @@ def test_each_worlds_subtree_is_its_own_radix_tree_with_global_tie_breaks(oracle
     """What the one-workgroup-per-world GPU build relies on: a world's subtree in the global tree is the radix
     tree of that world's keys alone (equal keys split by global position), rooted at the world's first or
     last position by the rule of the host (root_at_end), and every other interior node keeps its index."""
-    for index, worlds in [(2, 40), (1, 33), (3, 9)]:
-        cfg, out = _lbvh(oracle, index, worlds)
+    for name, worlds in [("body_worlds", 40), ("instanced_worlds", 33), ("lidar_sweeps", 9)]:
+        cfg, out = _lbvh(oracle, name, worlds)
         st, off = out["state"], out["world_off"]
         n = int(off[-1])
         counts = np.diff(off)
@@ -247,7 +247,7 @@ def test_each_worlds_subtree_is_its_own_radix_tree_with_global_tie_breaks(oracle
             p_prev = -1 if prev is None else 32 - (w ^ prev).bit_length()
             p_next = -1 if nxt is None else 32 - (w ^ nxt).bit_length()
             root = b if p_next > p_prev else a
-            assert st["roots"][w] == root, (index, w)
+            assert st["roots"][w] == root, (name, w)
             assert tuple(st["range"][root]) == (a, b)
             for i in range(a + 1, b):
                 lo, hi = st["range"][i]
@@ -255,7 +255,7 @@ def test_each_worlds_subtree_is_its_own_radix_tree_with_global_tie_breaks(oracle
 
 
 def test_uneven_worlds_and_empty_worlds(oracle):
-    cfg = workloads.config(2, worlds=12)
+    cfg = workloads.make("body_worlds", worlds=12)
     cfg["counts"] = np.array([0, 5, 1, 0, 0, 256, 8, 1, 2, 0, 3, 0], np.int32)
     out = forward(cfg, oracle, outputs=())
     st, off = out["state"], out["world_off"]
