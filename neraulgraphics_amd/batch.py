@@ -8,8 +8,12 @@ Differences from the reference, all deliberate:
     environment maps are not supported (NotImplementedError), and rgb defaults to False here;
   * instances_in_world gives every world its own count of the template's first instances;
   * pose_backward_depth / pose_backward_range give gradients with respect to body poses.
-Errors: ValueError for bad input (the reference's std::invalid_argument), RuntimeError for device failures."""
+Errors: ValueError for bad input (the reference's std::invalid_argument), RuntimeError for device failures.
+Threads: one renderer may be shared; its calls are serialized by a per-renderer lock, and the library call itself
+runs without the GIL (ctypes releases it), as the reference's edge does (python/bindings.cpp:242-251, 287-289)."""
 import ctypes
+import functools
+import threading
 
 import numpy as np
 
@@ -55,6 +59,15 @@ def _as_input(a, what, count=None, dtype=np.float32):
     return arr.ctypes.data_as(ctypes.c_void_p), _lib.NGX_MEM_HOST, arr
 
 
+def _serialized(method):
+    """Runs the method under the renderer's lock: the backend context is not thread-safe."""
+    @functools.wraps(method)
+    def run(self, *args, **kwargs):
+        with self._mutex:
+            return method(self, *args, **kwargs)
+    return run
+
+
 class BatchRenderer:
     """Many worlds rendered at once: thousands of copies of one environment, each seen by its own camera(s).
 
@@ -64,6 +77,7 @@ class BatchRenderer:
 
     def __init__(self, width, height, world_count, cameras_per_world=1, spheres=(), lights=(), floor=None, meshes=(),
                  instances=(), textures=(), device="auto", instances_in_world=None, device_id=None):
+        self._mutex = threading.RLock()
         lib = _lib.load()
         if device not in ("auto", "cuda", "gpu"):
             raise ValueError(f"this backend runs on the GPU; device {device!r} is not one it serves")
@@ -146,6 +160,7 @@ class BatchRenderer:
         _lib.check(self._lib.ngx_set_shading(self._h, ctypes.byref(desc)))
 
     # ---- the RGB picture's lighting (include/neraulgraphics/batch.h:206-212)
+    @_serialized
     def set_ambient(self, ambient):
         old = self._shading["ambient"]
         self._shading["ambient"] = float(ambient)
@@ -159,6 +174,7 @@ class BatchRenderer:
     def ambient(self):
         return self._shading["ambient"]
 
+    @_serialized
     def set_environment(self, environment):
         """The sky gradient (an Environment); maps are not supported."""
         if getattr(environment, "map", None):
@@ -195,9 +211,11 @@ class BatchRenderer:
     def last_kernel_ms(self):
         return float(self._lib.ngx_last_kernel_ms(self._h))
 
+    @_serialized
     def synchronize(self):
         _lib.check(self._lib.ngx_synchronize(self._h))
 
+    @_serialized
     def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000,
                          sort_rays=True):
         """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
@@ -214,6 +232,7 @@ class BatchRenderer:
         _lib.check(self._lib.ngx_get_trace_config(self._h, ctypes.byref(c)))
         return c.as_dict()
 
+    @_serialized
     def autotune(self, cameras, fov=1.0, *, depth=True, normals=False, object_ids=True, reps=3, cache=True):
         """Times every trace configuration worth trying on these cameras and keeps the fastest (see
         ngx_autotune). With cache, the choice is remembered in a JSON file (NGX_TUNE_CACHE, default
@@ -264,6 +283,7 @@ class BatchRenderer:
         return dict(chosen, ms=float(ms.value))
 
     # ---- poses
+    @_serialized
     def set_instance_transforms(self, matrices):
         """(world_count, instances_per_world, 3, 4) float32: the top three rows of each instance's 4x4 transform."""
         n = self._worlds * self._per_world * 12
@@ -271,6 +291,7 @@ class BatchRenderer:
         _lib.check(self._lib.ngx_set_instance_matrices(self._h, p, mem, self._stream_for(mem)))
         self._hold(keep, mem)
 
+    @_serialized
     def bind_bodies(self, body_of_instance, local_pose, scale, bodies_per_world):
         boi = np.ascontiguousarray(np.asarray(body_of_instance, dtype=np.int32).reshape(-1))
         lp = np.ascontiguousarray(np.asarray(local_pose, dtype=np.float32).reshape(-1))
@@ -283,6 +304,7 @@ class BatchRenderer:
                                              lp.ctypes.data_as(ctypes.c_void_p), sc.ctypes.data_as(ctypes.c_void_p)))
         self._bodies = int(bodies_per_world)
 
+    @_serialized
     def set_body_poses(self, body_poses):
         """(world_count, bodies_per_world, 7) float32: x, y, z, then a quaternion x, y, z, w."""
         if self._bodies is None:
@@ -293,6 +315,7 @@ class BatchRenderer:
         self._hold(keep, mem)
 
     # ---- images and rays
+    @_serialized
     def render(self, cameras, fov=1.0, samples_per_pixel=1, *, rgb=False, depth=False, normals=False, object_ids=False,
                frame=0, zero_copy=False):
         """The images asked for, as a dict: 'depth' (views, H, W) float32, 'normals' (views, H, W, 3) float32 and
@@ -313,6 +336,7 @@ class BatchRenderer:
                   "normals": ((v, h, w, 3), np.float32, f.normal), "object_ids": ((v, h, w), np.int32, f.object_id)}
         return self._outputs(shapes, zero_copy)
 
+    @_serialized
     def cast_rays(self, origins, directions, max_distance=float("inf"), zero_copy=None):
         """Rays in every world: origins and directions (world_count, rays_per_world, 3). Returns 'distance'
         (world_count, rays_per_world), 'normal' (..., 3) and 'object_id'. Device tensors in give device tensors
@@ -340,6 +364,7 @@ class BatchRenderer:
         return self._outputs(shapes, zero_copy)
 
     # ---- gradients (no counterpart in the reference)
+    @_serialized
     def pose_backward_depth(self, body_poses, cameras, fov, depth, grad_depth):
         """dL/d(body_poses) given dL/d(depth) of a render from these poses and cameras (all torch device tensors)."""
         t = _torch()
@@ -351,6 +376,7 @@ class BatchRenderer:
                                                      args[2].data_ptr(), args[3].data_ptr(), grad.data_ptr(), _stream()))
         return grad
 
+    @_serialized
     def pose_backward_range(self, body_poses, origins, directions, max_distance, distance, grad_distance):
         t = _torch()
         if self._bodies is None:
@@ -364,9 +390,11 @@ class BatchRenderer:
         return grad
 
     # ---- build read-back, for checking the top-level build
+    @_serialized
     def keep_build_intermediates(self, keep=True):
         _lib.check(self._lib.ngx_debug_keep_build(self._h, int(bool(keep))))
 
+    @_serialized
     def read_build(self, intermediates=False):
         from .layouts import TLAS_NODE
         n = self.instance_count

@@ -232,6 +232,37 @@ def test_zero_copy_views_are_overwritten_by_the_next_render(gpu):
     assert first.data_ptr() == second.data_ptr() and not torch.equal(first, kept)
 
 
+def test_threads_sharing_a_renderer_get_their_own_images(gpu):
+    """python/bindings.cpp:242-251: one renderer used from several threads, each call serialized; every thread's
+    host images are those of its own cameras."""
+    import threading
+    cfg = workloads.make("instanced_worlds", worlds=16)
+    r = workloads.make_renderer(cfg)
+    cams = [cfg["cameras"].copy() for _ in range(4)]
+    for k, c in enumerate(cams):
+        c[:, 3] += 0.25 * k
+    want = [r.render(c, cfg["fov"], depth=True, object_ids=True) for c in cams]
+    got, errors = [[None] * 6 for _ in cams], []
+
+    def work(k):
+        try:
+            for i in range(6):
+                got[k][i] = r.render(cams[k], cfg["fov"], depth=True, object_ids=True)
+        except Exception as e:  # reported below: a thread's exception does not fail the test by itself
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(cams))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not errors and not any(t.is_alive() for t in threads)
+    for k in range(len(cams)):
+        for g in got[k]:
+            assert np.array_equal(g["depth"].view(np.uint32), want[k]["depth"].view(np.uint32))
+            assert np.array_equal(g["object_ids"], want[k]["object_ids"])
+
+
 def test_device_poses_equal_host_poses(gpu):
     import torch
     cfg = workloads.make("body_worlds", worlds=8)
