@@ -120,6 +120,23 @@ __device__ __forceinline__ float box_entry(const float lx, const float ly, const
     return tn <= tf ? tn : NO_HIT;
 }
 
+// The slab test's decision alone, for walks that only need to know whether a box is entered (and, for
+// ordering, its entry distance tn). lim is box_limit(tmax): the lane's closest distance clamped to the
+// largest float below NO_HIT, so that tn <= min(far, lim) holds exactly when box_entry(...) < NO_HIT
+// (tn and far are never NaN: the minimum and maximum return the other operand of a NaN, and the last
+// terms, 0 and tmax, are numbers). Computing lim once per node keeps the per-box work to the slab test.
+constexpr float BELOW_NO_HIT = __builtin_bit_cast(float, __builtin_bit_cast(unsigned, NO_HIT) - 1u);
+__device__ __forceinline__ float box_limit(const float tmax) { return fminf(tmax, BELOW_NO_HIT); }
+__device__ __forceinline__ bool box_hit(const float lx, const float ly, const float lz, const float hx, const float hy,
+                                        const float hz, const float3 o, const float3 inv_d, const float lim, float &tn) {
+    const float tx0 = (lx - o.x) * inv_d.x, tx1 = (hx - o.x) * inv_d.x;
+    const float ty0 = (ly - o.y) * inv_d.y, ty1 = (hy - o.y) * inv_d.y;
+    const float tz0 = (lz - o.z) * inv_d.z, tz1 = (hz - o.z) * inv_d.z;
+    tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), lim));
+    return tn <= tf;
+}
+
 // Two-sided Moller-Trumbore (mesh_dev.h:80-99), triangle given as v0, e1, e2.
 __device__ __forceinline__ bool triangle_hit(const float3 v0, const float3 e1, const float3 e2, const float3 o, const float3 d,
                                              const float tmax, float &t, float &bu, float &bv) {
@@ -152,6 +169,15 @@ __device__ __forceinline__ int popc64(const uint64_t m) { return __popcll(m); }
 // Whether this lane's bit is set in a wave-uniform mask: the mask becomes the execution mask, no vector work.
 __device__ __forceinline__ bool in_mask(const uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 __device__ __forceinline__ int first_lane(const uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+// A copy of v held in vector registers even when v is wave-uniform. A vector instruction reads at most one
+// scalar operand, so a uniform ray origin met with a node's box (scalar too) would otherwise be copied into a
+// vector register before every subtraction; the copy is made once here instead.
+__device__ __forceinline__ float in_vgpr(const float v) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ float3 in_vgpr3(const float3 v) { return make_float3(in_vgpr(v.x), in_vgpr(v.y), in_vgpr(v.z)); }
 
 // Loads through the scalar data cache: the address must be the same in every lane of the wave. The
 // constant address space is how the compiler is told the data is read-only for the whole kernel.

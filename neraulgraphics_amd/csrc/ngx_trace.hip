@@ -194,14 +194,16 @@ __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g
                                     const float3 lo, const float3 ld, const float3 inv_ld, float &tmax, LaneHit &h, Stats &stats) {
     const int lane = lane_id();
     const int octant = (ld.x >= 0.0f ? 1 : 0) | (ld.y >= 0.0f ? 2 : 0) | (ld.z >= 0.0f ? 4 : 0);
-    bool live = in_mask(root_mask);
+    // Lane states are kept as wave masks where they can be (live), and per-lane conditions enter them
+    // through ballots of plain comparisons, which cost one vector compare each.
+    uint64_t live = root_mask;
     int cur = -1, leaf = -1, leaf_count = 0, after_leaf = -1;
     auto first_child = [&](const int a, const int meta) {
         const int code = -meta - 1, axis = code >> 1, flip = code & 1;
         const bool positive = (octant >> axis) & 1;
         return positive != (flip == 1) ? a : a + 1;
     };
-    if (live) {
+    if (in_mask(live)) {
         const SkipNode &r = S.skip[root];
         if (r.meta > 0) {
             leaf = r.a;
@@ -214,17 +216,22 @@ __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g
     while (true) {
         // box steps until every live lane waits at a leaf or has finished
         while (true) {
-            const bool stepping = live && leaf < 0 && cur >= 0;
-            const uint64_t sm = ballot(stepping);
-            if (sm == 0) break;
+            const uint64_t at_node = ballot(leaf < 0);
+            const uint64_t sm = live & at_node & ballot(cur >= 0);
+            if (sm == 0) {
+                live &= ~(at_node & ~sm);  // lanes neither at a leaf nor at a node are finished
+                break;
+            }
             stats.add(ST_BLAS_NODES, 1);
             stats.add(ST_BLAS_LANES, popc64(sm));
-            if (stepping) {
+            const float lim = box_limit(tmax);
+            if (in_mask(sm)) {
                 const float4 *np = reinterpret_cast<const float4 *>(S.skip + cur);
                 const float4 f0 = np[0], f1 = np[1];
                 const int esc = S.skip[cur].escape[octant];
                 const int a = __float_as_int(f0.w), meta = __float_as_int(f1.w);
-                if (box_entry(f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, lo, inv_ld, tmax) < NO_HIT) {
+                float tn;
+                if (box_hit(f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, lo, inv_ld, lim, tn)) {
                     if (meta > 0) {
                         leaf = a;
                         leaf_count = meta;
@@ -236,30 +243,31 @@ __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g
                     cur = esc;
                 }
             }
-            if (live && leaf < 0 && cur < 0) live = false;
+            live &= ~(ballot(leaf < 0) & ballot(cur < 0));
         }
         // the waiting lanes' triangles, as batched pairs
-        const uint64_t waiting = ballot(live && leaf >= 0);
+        const uint64_t waiting = live & ballot(leaf >= 0);
         if (waiting == 0) break;
+        // batches of exactly 64 pairs (the last one fewer), lanes in order: a lane takes what is left of its
+        // leaf up to the batch's free places
         int done = 0;
         while (true) {
-            const bool more = live && leaf >= 0 && done < leaf_count;
-            const uint64_t mm = ballot(more);
+            const uint64_t mm = waiting & ballot(done < leaf_count);
             if (mm == 0) break;
-            const int per = max(1, 64 / popc64(mm));
-            const int take = more ? min(leaf_count - done, per) : 0;
-            const int end = wave_prefix_sum(take);
-            const int total = __builtin_amdgcn_readlane(end, 63);
-            for (int k = 0; k < take; ++k) L->pair[end - take + k] = lane | ((leaf + done + k) << 6);
-            int pairs = total;
+            const int rem = in_mask(mm) ? leaf_count - done : 0;
+            const int incl = wave_prefix_sum(rem);
+            const int before = incl - rem;
+            const int take = max(0, min(rem, 64 - before));
+            for (int k = 0; k < take; ++k) L->pair[before + k] = lane | ((leaf + done + k) << 6);
+            int pairs = min(64, __builtin_amdgcn_readlane(incl, 63));
             flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
             done += take;
         }
-        if (live && leaf >= 0) {
+        if (in_mask(waiting)) {
             leaf = -1;
             cur = after_leaf;
-            if (cur < 0) live = false;
         }
+        live &= ~(waiting & ballot(cur < 0));
     }
 }
 
@@ -278,9 +286,8 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     const float3 inv_d = safe_inverse(ld);
     const float4 *mp = reinterpret_cast<const float4 *>(S.mesh_roots + meta.x);
     const float4 m0 = sload4(mp), m1 = sload4(mp + 1);
-    const int lane = lane_id();
-    const bool in = in_mask(mask);
-    const uint64_t root_mask = ballot(in && box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, tmax) < NO_HIT);
+    float root_t;
+    const uint64_t root_mask = mask & ballot(box_hit(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, box_limit(tmax), root_t));
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     if (LANES && S.mesh_tris[meta.x] >= S.lane_min_tris) {
@@ -301,14 +308,17 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         const Wide w = load_wide(S.blas_wide, cur);
         stats.add(ST_BLAS_NODES, 1);
         stats.add(ST_BLAS_LANES, popc64(cm));
-        const bool on = in_mask(cm);
-        const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], lo, inv_d, tmax) : NO_HIT;
-        const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], lo, inv_d, tmax) : NO_HIT;
-        const uint64_t ml = ballot(dl < NO_HIT), mr = ballot(dr < NO_HIT);
+        // every lane runs the tests (a vector instruction costs the same for 1 lane or 64); the masks keep
+        // the lanes of this node
+        const float lim = box_limit(tmax);
+        float dl, dr;
+        const uint64_t ml = cm & ballot(box_hit(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], lo, inv_d, lim, dl));
+        const uint64_t mr = cm & ballot(box_hit(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], lo, inv_d, lim, dr));
         int next_ref, next_count;
         uint64_t next_mask;
         if (ml != 0 && mr != 0) {
-            const uint64_t near_left = ballot(dl < NO_HIT && dl <= dr);
+            // a lane is nearer the left child when it enters only the left, or the left no later than the right
+            const uint64_t near_left = ml & (~mr | ballot(dl <= dr));
             const bool left_first = 2 * popc64(near_left) >= popc64(ml | mr);
             // a leaf child is pushed as its parent and side, and its run read back when popped
             if (left_first) {
@@ -359,12 +369,13 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
     if (live == 0) return;
     const int root = uniform(S.roots[world]);
     if (root < 0) return;
+    // the origin in vector registers: for camera rays it is the same in every lane
+    const float3 ov = in_vgpr3(o);
     if (root >= S.total - 1) {
-        visit_instance<LANES>(S, L, root - (S.total - 1), live, o, d, tmax, h, stats);
+        visit_instance<LANES>(S, L, root - (S.total - 1), live, ov, d, tmax, h, stats);
         return;
     }
     const float3 inv_d = safe_inverse(d);
-    const int lane = lane_id();
     // One entry is pushed per level at most. Below a world's root every level fixes at least one more bit
     // of a Morton code (30 bits) or, among equal codes, of the leaves' positions (at most ceil(log2 n) + 1
     // bits over n consecutive positions): fewer than 64 levels for any n < 2^31, so 64 entries do.
@@ -375,14 +386,14 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
         const Wide w = load_wide(S.tlas_wide, cur);
         stats.add(ST_TLAS_NODES, 1);
         stats.add(ST_TLAS_LANES, popc64(cm));
-        const bool on = in_mask(cm);
-        const float dl = on ? box_entry(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], o, inv_d, tmax) : NO_HIT;
-        const float dr = on ? box_entry(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], o, inv_d, tmax) : NO_HIT;
-        const uint64_t ml = ballot(dl < NO_HIT), mr = ballot(dr < NO_HIT);
+        const float lim = box_limit(tmax);
+        float dl, dr;
+        const uint64_t ml = cm & ballot(box_hit(w.l_lo[0], w.l_lo[1], w.l_lo[2], w.l_hi[0], w.l_hi[1], w.l_hi[2], ov, inv_d, lim, dl));
+        const uint64_t mr = cm & ballot(box_hit(w.r_lo[0], w.r_lo[1], w.r_lo[2], w.r_hi[0], w.r_hi[1], w.r_hi[2], ov, inv_d, lim, dr));
         int next_ref;
         uint64_t next_mask;
         if (ml != 0 && mr != 0) {
-            const uint64_t near_left = ballot(dl < NO_HIT && dl <= dr);
+            const uint64_t near_left = ml & (~mr | ballot(dl <= dr));
             const bool left_first = 2 * popc64(near_left) >= popc64(ml | mr);
             if (left_first) {
                 st.push(w.right, mr);
@@ -399,7 +410,7 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
             next_ref = 0; next_mask = 0;
         }
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) visit_instance<LANES>(S, L, ~next_ref, next_mask, o, d, tmax, h, stats);
+            if (next_mask != 0) visit_instance<LANES>(S, L, ~next_ref, next_mask, ov, d, tmax, h, stats);
             if (st.sp == 0) return;
             st.pop(next_ref, next_mask);
         }
