@@ -163,7 +163,7 @@ struct ngx_ctx {
     bool timed = false;
     hipStream_t last_stream = nullptr;
     int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000, sort_rays = 1;
-    int persistent_blocks = 0;          // 8 workgroups of 4 wavefronts per compute unit
+    int compute_units = 0;              // sizes the persistent launch
     int *d_tile_counters = nullptr;
 
     ~ngx_ctx() {
@@ -383,8 +383,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
         {
             hipDeviceProp_t prop;
             HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
-            c.persistent_blocks = 8 * std::max(1, prop.multiProcessorCount);
-            c.persistent_blocks = (c.persistent_blocks + 7) / 8 * 8;
+            c.compute_units = std::max(1, prop.multiProcessorCount);
         }
         device_alloc(c.d_tile_counters, 8);
         HIP_CHECK(hipEventCreate(&c.ev0));
@@ -717,7 +716,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         }
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
-        if (a.depth || a.normal || a.object_id) HIP_CHECK(ngx::launch_render(a, s, ctx->persistent ? ctx->persistent_blocks : 0));
+        if (a.depth || a.normal || a.object_id) HIP_CHECK(ngx::launch_render(a, s, ctx->persistent ? ctx->compute_units : 0));
         if (rgb) {
             ngx::RgbArgs g{};
             g.r = a;

@@ -185,9 +185,10 @@ hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &
 // The RGB picture of every view, one 64-pixel tile per wavefront.
 hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream);
 
-// Renders every view; with a.tile_counters set, a fixed grid of `persistent_blocks` workgroups takes tiles
-// from per-XCD counters until none are left (the counters are zeroed by this call first).
-hipError_t launch_render(const RenderArgs &a, hipStream_t stream, int persistent_blocks = 0);
+// Renders every view; with a.tile_counters set and compute_units > 0, a fixed grid of as many workgroups as the
+// device holds at once takes tiles from per-XCD counters until none are left (the counters are zeroed by this
+// call first).
+hipError_t launch_render(const RenderArgs &a, hipStream_t stream, int compute_units = 0);
 hipError_t launch_rays(const RayArgs &a, hipStream_t stream);
 hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream);
 hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream);

@@ -1289,12 +1289,24 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
     default: launch(WALK_PACKET); break; \
     }
 
-hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int persistent_blocks) {
+// The persistent grid: as many workgroups as the device holds at once (the kernel's occupancy times the
+// compute units), a multiple of 8 so every XCD gets the same number.
+template <int WALK>
+static int persistent_grid(const int compute_units) {
+    static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, render_persistent_kernel<WALK>, 256, 0) != hipSuccess || n < 1) n = 1;
+        return n;
+    }();
+    return (per_cu * compute_units + 7) / 8 * 8;
+}
+
+hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int compute_units) {
     if (a.total_tiles <= 0) return hipSuccess;
-    if (a.tile_counters && persistent_blocks > 0) {
-        if (a.total_tiles >= INT32_MAX / 2 || persistent_blocks % 8 != 0) return hipErrorInvalidValue;
+    if (a.tile_counters && compute_units > 0) {
+        if (a.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 64, 0, stream>>>(a.tile_counters);
-#define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_blocks, 256, 0, stream>>>(a)
+#define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_grid<W>(compute_units), 256, 0, stream>>>(a)
         NGX_BY_WALK(a.walk, NGX_LAUNCH)
 #undef NGX_LAUNCH
         return hipGetLastError();

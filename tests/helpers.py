@@ -53,3 +53,21 @@ def world_subtree(nodes, root, n_total):
         if v < n_total - 1:
             stack += [int(nodes[v]["left"]), int(nodes[v]["right"])]
     return np.array(sorted(out), np.int64)
+
+
+def world_subset(cfg, worlds):
+    """The workload restricted to the given worlds (in that order), for checking a full-size batch against the
+    oracle on a sample: every per-world array is indexed, the template and meshes are kept."""
+    worlds = np.asarray(worlds, np.int64)
+    out = dict(cfg)
+    out["worlds"] = len(worlds)
+    cpw = cfg["cameras_per_world"]
+    if cfg.get("counts") is not None:
+        out["counts"] = np.asarray(cfg["counts"])[worlds]
+    for k in ("matrices", "body_poses", "target_poses", "origins", "directions"):
+        if cfg.get(k) is not None:
+            out[k] = np.ascontiguousarray(np.asarray(cfg[k])[worlds])
+    if cfg.get("cameras") is not None:
+        views = (worlds[:, None] * cpw + np.arange(cpw)[None, :]).reshape(-1)
+        out["cameras"] = np.ascontiguousarray(np.asarray(cfg["cameras"]).reshape(-1, 12)[views])
+    return out

@@ -158,3 +158,16 @@ def test_fp64_pose_gradient_equals_central_differences(oracle):
     fd = oracle.pose_grad(1, 1e-6, *args)
     assert np.linalg.norm(cf - fd) <= 1e-7 * np.linalg.norm(fd)
     assert np.count_nonzero(np.abs(cf) > 0) > 20
+
+
+def test_a_sample_of_worlds_renders_as_in_the_full_batch(oracle):
+    """Checking a full-size GPU batch on a sample of its worlds rests on this: the oracle run on some worlds alone
+    gives those worlds' pixels as the run on the whole batch does (equal distances; IDs equal but for ties)."""
+    from tests.helpers import compare_images, world_subset
+    cfg = workloads.make("body_worlds", worlds=10)
+    full = forward(cfg, oracle)
+    sample = np.array([0, 4, 9])
+    part = forward(world_subset(cfg, sample), oracle)
+    got = {k: full[k][sample] for k in ("depth", "object_id", "normal")}
+    rep = compare_images(got, part, "sample")
+    assert rep["depth_bitexact"] == 1.0 and rep["hits"] > 0
