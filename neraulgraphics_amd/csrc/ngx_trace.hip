@@ -187,7 +187,7 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 // wavefront's tile: there each ray of the packet needs its own leaves, so a shared walk visits the union
 // of 64 paths. Each lane follows its own ray through the octant-ordered skip layout (a box test per step;
 // a hit descends to the first child, a miss or a finished leaf goes to the node's escape for the ray's
-// octant), so no stack is kept. Leaves wait until every lane has one or is done, then all waiting lanes'
+// octant), so no stack is kept. Leaves wait until half the walking lanes have one, then all waiting lanes'
 // triangles are tested together as (ray, triangle) pairs. The root's box has been tested already.
 template <class Stats>
 __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g, const uint64_t root_mask, const int root,
@@ -214,14 +214,12 @@ __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g
         }
     }
     while (true) {
-        // box steps until every live lane waits at a leaf or has finished
+        // box steps until half the live lanes (or all) wait at a leaf: waiting for every lane would leave the
+        // early ones idle, testing after each leaf would test few pairs at a time
         while (true) {
-            const uint64_t at_node = ballot(leaf < 0);
-            const uint64_t sm = live & at_node & ballot(cur >= 0);
-            if (sm == 0) {
-                live &= ~(at_node & ~sm);  // lanes neither at a leaf nor at a node are finished
-                break;
-            }
+            const uint64_t at_leaf = live & ballot(leaf >= 0);
+            const uint64_t sm = live & ~at_leaf;  // a live lane is at a leaf or at a node
+            if (sm == 0 || (at_leaf != 0 && 2 * popc64(at_leaf) >= popc64(live))) break;
             stats.add(ST_BLAS_NODES, 1);
             stats.add(ST_BLAS_LANES, popc64(sm));
             const float lim = box_limit(tmax);
