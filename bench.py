@@ -8,14 +8,15 @@ cycled so every step has new poses. Inputs are resident in HBM before timing; th
 HIP graph. Before timing, the trace configuration is autotuned on the benchmark's own cameras (--no-tune keeps
 the defaults).
 
-Also reported, on the same JSON line: LiDAR ray queries (lidar_sweeps: 1024 worlds x 65,536 rays,
+Also reported, on the same JSON line: the same batch with the RGB picture as well (one soft sphere light, a floor);
+LiDAR ray queries (lidar_sweeps: 1024 worlds x 65,536 rays,
 rays and results resident on the device); the forward + pose-backward pass (posed_bodies: 1024 worlds,
 64x64 depth, 32 bodies) and its ratio to the forward alone; the render kernel against the HBM roofline;
 and the reference CPU algorithm (this repository's C restatement of the reference CPU backend: FP64 body
 poses, per-world SAH top level with refit, the same traversal, rows of views in chunks of 4 over threads)
 timed on this host on a bounded sample.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph] [--no-tune] [--no-lidar] [--no-bwd]"""
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph] [--no-tune] [--no-lidar] [--no-rgb] [--no-bwd]"""
 import argparse
 import json
 import os
@@ -157,6 +158,35 @@ def bench_lidar(args, torch):
                 trace_config=r.trace_config)
 
 
+def bench_rgb(args, torch):
+    """The same 4096-world batch with the RGB picture too (the reference's configuration is RGB + depth + object ID):
+    a checker floor under the bodies, one sphere light of radius 0.5 (soft shadows), ambient 0.1, one sample a pixel,
+    new body poses every step. Materials are the defaults (diffuse 0.8, a little specular)."""
+    from neraulgraphics_amd import workloads
+    from neraulgraphics_amd.geometry import Light, Plane
+    cfg = workloads.make("body_worlds")
+    floor = Plane(y=-6.0, half_extent_x=40.0, z_min=-60.0, z_max=20.0)
+    r = workloads.make_renderer(cfg, floor=floor, lights=[Light((0.0, 12.0, -6.0), intensity=1.2, radius=0.5)])
+    r.set_ambient(0.1)
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    rng = np.random.default_rng(7)
+    pose_sets = [torch.from_numpy(cfg["body_poses"]).cuda()]
+    for _ in range(3):
+        p = cfg["body_poses"].copy()
+        p[..., :3] += rng.normal(0, 0.05, p[..., :3].shape).astype(np.float32)
+        pose_sets.append(torch.from_numpy(p).cuda())
+    k = [0]
+
+    def step():
+        r.set_body_poses(pose_sets[k[0] % 4])
+        r.render(cams, cfg["fov"], 1, rgb=True, depth=True, object_ids=True, frame=k[0], zero_copy=True)
+        k[0] += 1
+
+    steps = max(5, args.steps // 2)
+    sec = timed_steps(step, steps, max(2, args.warmup // 2), torch)
+    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3)
+
+
 def bench_posed_bodies(args, torch):
     from neraulgraphics_amd import workloads
     from neraulgraphics_amd.autograd import render_depth
@@ -232,6 +262,7 @@ def main():
     ap.add_argument("--no-xcd", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default trace configuration instead of autotuning")
     ap.add_argument("--no-lidar", action="store_true")
+    ap.add_argument("--no-rgb", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 512 MiB three times with a plain copy kernel, to scale HBM counters")
@@ -266,6 +297,10 @@ def main():
     if not args.no_bwd:
         print("# posed_bodies: forward and forward+backward", file=sys.stderr, flush=True)
         c4 = bench_posed_bodies(args, torch)
+    crgb = None
+    if not args.no_rgb:
+        print("# body_worlds with RGB + depth + ID", file=sys.stderr, flush=True)
+        crgb = bench_rgb(args, torch)
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline", file=sys.stderr, flush=True)
@@ -306,6 +341,8 @@ def main():
     if c4:
         line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])
+    if crgb:
+        line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"])
     line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 

@@ -723,12 +723,12 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
             g.r.depth = nullptr;
             g.r.normal = nullptr;
             g.r.object_id = nullptr;
-            g.r.tile_counters = nullptr;
+            g.r.tile_counters = ctx->persistent ? ctx->d_tile_counters : nullptr;
             g.shade = ctx->shade;
             g.rgb = rgb;
             g.spp = spp;
             g.frame = frame;
-            HIP_CHECK(ngx::launch_rgb(g, s));
+            HIP_CHECK(ngx::launch_rgb(g, s, ctx->persistent ? ctx->compute_units : 0));
         }
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
         ctx->timed = timed;

@@ -183,7 +183,8 @@ hipError_t launch_ray_order(const float *dirs, int rays_per_world, int worlds, u
 // Builds world `world` (instances first .. first+n-1, n > 1) with grid-wide kernels.
 hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &s, int world, int first, int n, hipStream_t stream);
 // The RGB picture of every view, one 64-pixel tile per wavefront.
-hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream);
+// The RGB picture; with a.r.tile_counters set and compute_units > 0, from a persistent grid as launch_render.
+hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, int compute_units = 0);
 
 // Renders every view; with a.tile_counters set and compute_units > 0, a fixed grid of as many workgroups as the
 // device holds at once takes tiles from per-XCD counters until none are left (the counters are zeroed by this

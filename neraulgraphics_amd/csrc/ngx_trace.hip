@@ -27,13 +27,24 @@ struct LaneHit {
 // instrumented kernel built with NGX_TRACE_STATS.
 enum StatIndex { ST_TLAS_NODES, ST_TLAS_LANES, ST_INSTANCES, ST_INSTANCE_LANES, ST_INSTANCE_ROOT_LANES, ST_BLAS_NODES,
                  ST_BLAS_LANES, ST_TRIANGLES, ST_TRIANGLE_LANES, ST_POPS, ST_VALID, ST_COUNT };
+// The same type also says whether the walk is an occlusion test (kAnyHit): then a ray's limit drops below zero
+// at its first hit, so its lane leaves every box after it and the walk ends as soon as every ray is blocked or
+// done; whether a ray is blocked is all that is read (mesh_dev.h:208-227 stops at the first hit too).
 struct NoStats {
+    static constexpr bool kAnyHit = false;
     __device__ __forceinline__ void add(int, int) {}
 };
+struct Occlusion {
+    static constexpr bool kAnyHit = true;
+    __device__ __forceinline__ void add(int, int) {}
+};
+#ifdef NGX_TRACE_STATS
 struct WaveStats {
+    static constexpr bool kAnyHit = false;
     int c[ST_COUNT];
     __device__ __forceinline__ void add(const int k, const int v) { c[k] += v; }
 };
+#endif
 
 struct Wide {
     float l_lo[3], l_hi[3], r_lo[3], r_hi[3];
@@ -140,7 +151,7 @@ __device__ __forceinline__ void flush_pairs(const TraceScene &S, PairLds *L, int
     const unsigned long long b = L->best[lane];
     const unsigned p = (unsigned)b;
     if (p != 0xffffffffu) {
-        tmax = __uint_as_float((unsigned)(b >> 32));
+        tmax = Stats::kAnyHit ? -1.0f : __uint_as_float((unsigned)(b >> 32));
         // the hit is kept in LDS, not registers, until the walk is over
         L->hit_inst[lane] = inst;
         L->hit_tri[lane] = L->pair[p] >> 6;
@@ -189,6 +200,10 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 // a hit descends to the first child, a miss or a finished leaf goes to the node's escape for the ray's
 // octant), so no stack is kept. Leaves wait until half the walking lanes have one, then all waiting lanes'
 // triangles are tested together as (ray, triangle) pairs. The root's box has been tested already.
+// How many quarters of the walking lanes wait at leaves before their triangles are tested.
+#ifndef NGX_LEAF_WAIT_QUARTERS
+#define NGX_LEAF_WAIT_QUARTERS 2
+#endif
 template <class Stats>
 __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g, const uint64_t root_mask, const int root,
                                     const float3 lo, const float3 ld, const float3 inv_ld, float &tmax, LaneHit &h, Stats &stats) {
@@ -219,7 +234,7 @@ __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g
         while (true) {
             const uint64_t at_leaf = live & ballot(leaf >= 0);
             const uint64_t sm = live & ~at_leaf;  // a live lane is at a leaf or at a node
-            if (sm == 0 || (at_leaf != 0 && 2 * popc64(at_leaf) >= popc64(live))) break;
+            if (sm == 0 || (at_leaf != 0 && 4 * popc64(at_leaf) >= NGX_LEAF_WAIT_QUARTERS * popc64(live))) break;
             stats.add(ST_BLAS_NODES, 1);
             stats.add(ST_BLAS_LANES, popc64(sm));
             const float lim = box_limit(tmax);
@@ -573,7 +588,7 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
                     const float4 *tp = S.tris + 3 * (size_t)j;
                     float t, bu, bv;
                     if (triangle_hit(xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), lo, ld, tmax, t, bu, bv)) {
-                        tmax = t;
+                        tmax = Stats::kAnyHit ? -1.0f : t;
                         h.inst = g;
                         h.tri = j;
                         h.u = bu;
@@ -898,12 +913,13 @@ __device__ __forceinline__ bool occluded(const TraceScene &S, PairLds *L, const 
             if (fabsf(pt.x) < S.floor_hx && pt.z > S.floor_zmin && pt.z < S.floor_zmax) blocked = true;
         }
     }
-    // a closest-hit walk limited to max_dist finds a hit exactly when an any-hit walk would (mesh_dev.h:208-227)
+    // the walk limited to max_dist, stopping each ray at its first hit: it finds a hit exactly when the
+    // reference's any-hit walk does (mesh_dev.h:208-227)
     const uint64_t rest = ballot(mine && !blocked);
     if (rest != 0) {
         float best = max_dist;
         LaneHit h;
-        NoStats stats;
+        Occlusion stats;
         if (WALK != WALK_LANES) {
             const int lane = lane_id();
             L->hit_inst[lane] = -1;
@@ -932,13 +948,14 @@ __device__ __forceinline__ float srgb_channel(const float v) {
     return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
 }
 
+// 4 workgroups (16 wavefronts) per compute unit: at most 128 VGPRs
+#ifndef NGX_RGB_MIN_BLOCKS
+#define NGX_RGB_MIN_BLOCKS 4
+#endif
 template <int WALK>
-__global__ __launch_bounds__(256) void rgb_kernel(const RgbArgs G) {
-    NGX_PAIR_LDS(L);
+__device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const long long tile) {
     const RenderArgs &A = G.r;
     const ShadeScene &E = G.shade;
-    const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
-    if (tile >= A.total_tiles) return;
     const int lane = lane_id();
     const int view = (int)(tile / A.tiles_per_view);
     const int tt = (int)(tile % A.tiles_per_view);
@@ -1035,6 +1052,31 @@ __global__ __launch_bounds__(256) void rgb_kernel(const RgbArgs G) {
     out[0] = (uint8_t)(saturate(srgb_channel(c.x)) * 255.0f + 0.5f);
     out[1] = (uint8_t)(saturate(srgb_channel(c.y)) * 255.0f + 0.5f);
     out[2] = (uint8_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
+}
+
+template <int WALK>
+__global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_kernel(const RgbArgs G) {
+    NGX_PAIR_LDS(L);
+    const long long tile = wave_tile(G.r.xcd_remap, G.r.total_tiles);
+    if (tile >= G.r.total_tiles) return;
+    rgb_tile<WALK>(G, L, tile);
+}
+
+// The same tiles from a fixed grid taking PERSISTENT_CHUNK tiles at a time from per-XCD counters, as the
+// persistent depth kernel does.
+template <int WALK>
+__global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_persistent_kernel(const RgbArgs G) {
+    NGX_PAIR_LDS(L);
+    const int xcd = blockIdx.x & 7;
+    const long long first = G.r.total_tiles * xcd / 8, count = G.r.total_tiles * (xcd + 1) / 8 - first;
+    while (true) {
+        int t = 0;
+        if (lane_id() == 0) t = atomicAdd(&G.r.tile_counters[xcd], PERSISTENT_CHUNK);
+        t = __builtin_amdgcn_readlane(t, 0);
+        if (t >= count) break;
+        const int end = (int)min((long long)t + PERSISTENT_CHUNK, count);
+        for (int k = t; k < end; ++k) rgb_tile<WALK>(G, L, first + k);
+    }
 }
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
@@ -1327,9 +1369,27 @@ hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream
 }
 #endif
 
-hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream) {
+template <int WALK>
+static int rgb_persistent_grid(const int compute_units) {
+    static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rgb_persistent_kernel<WALK>, 256, 0) != hipSuccess || n < 1) n = 1;
+        return n;
+    }();
+    return (per_cu * compute_units + 7) / 8 * 8;
+}
+
+hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_units) {
     if (a.r.total_tiles <= 0) return hipSuccess;
     if (a.spp < 1 || a.shade.light_count < 0 || a.shade.light_count > MAX_LIGHTS) return hipErrorInvalidValue;
+    if (a.r.tile_counters && compute_units > 0) {
+        if (a.r.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
+        zero_counters_kernel<<<1, 64, 0, stream>>>(a.r.tile_counters);
+#define NGX_LAUNCH(W) rgb_persistent_kernel<W><<<rgb_persistent_grid<W>(compute_units), 256, 0, stream>>>(a)
+        NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
+#undef NGX_LAUNCH
+        return hipGetLastError();
+    }
     const long long blocks = (a.r.total_tiles + 3) / 4;
 #define NGX_LAUNCH(W) rgb_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
     NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
