@@ -305,6 +305,55 @@ def test_threads_sharing_a_renderer_get_their_own_images(gpu):
             assert np.array_equal(g["object_ids"], want[k]["object_ids"])
 
 
+def test_graph_replays_equal_direct_launches(gpu):
+    """A step (new body poses, then depth + ID, persistent launch with its counter reset) captured in HIP graphs,
+    one graph per pose set, replayed in turn with direct renders and host copies between the replays: every
+    replay gives the images the same step gives launched directly."""
+    import torch
+    cfg = workloads.make("body_worlds", worlds=32)
+    r = workloads.make_renderer(cfg)
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    rng = np.random.default_rng(11)
+    poses = [torch.from_numpy(cfg["body_poses"]).cuda()]
+    for _ in range(2):
+        p = cfg["body_poses"].copy()
+        p[..., :3] += rng.normal(0, 0.3, p[..., :3].shape).astype(np.float32)
+        poses.append(torch.from_numpy(p).cuda())
+
+    def step(k):
+        r.set_body_poses(poses[k])
+        return r.render(cams, cfg["fov"], depth=True, object_ids=True, zero_copy=True)
+
+    want = []
+    for k in range(3):
+        out = step(k)
+        torch.cuda.synchronize()
+        want.append({n: out[n].clone() for n in ("depth", "object_ids")})
+    assert not torch.equal(want[0]["depth"], want[1]["depth"])
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step(0)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graphs = []
+    for k in range(3):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = step(k)
+        graphs.append((g, out))
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for k in (2, 0, 1):
+            g, out = graphs[k]
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out["depth"], want[k]["depth"]), (rep, k)
+            assert torch.equal(out["object_ids"], want[k]["object_ids"]), (rep, k)
+            step((k + 1) % 3)  # direct work between replays
+            torch.cuda.synchronize()
+
+
 def test_device_poses_equal_host_poses(gpu):
     import torch
     cfg = workloads.make("body_worlds", worlds=8)
