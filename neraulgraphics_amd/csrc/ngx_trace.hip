@@ -80,6 +80,10 @@ struct PairLds {
     float2 uv[64];                 // per pair: barycentrics of its hit
     float2 hit_uv[64];             // per ray lane: barycentrics of its closest hit so far
     int hit_inst[64], hit_tri[64]; // per ray lane: instance and triangle of its closest hit so far (-1: none)
+    // the per-lane bottom-level walk's shared work: subtrees given away (ray lane | node << 6), and a ring of
+    // leaves waiting for their triangle tests (first triangle; ray lane | count << 6)
+    int share[128];
+    int leaf_first[128], leaf_ray_count[128];
     // the packet walk's two wave-uniform stacks (top level, bottom level): node reference and lane mask
     int st_ref[2][64];
     uint32_t st_lo[2][64], st_hi[2][64];
@@ -126,7 +130,7 @@ struct LdsStack {
 // Tests the queued pairs of instance inst, whose mesh-space ray is (lo, ld) in each ray's lane.
 template <class Stats>
 __device__ __forceinline__ void flush_pairs(const TraceScene &S, PairLds *L, int &pairs, const int inst, const float3 lo,
-                                            const float3 ld, float &tmax, LaneHit &h, Stats &stats) {
+                                            const float3 ld, float &tmax, Stats &stats) {
     if (pairs == 0) return;
     stats.add(ST_TRIANGLES, 1);
     stats.add(ST_TRIANGLE_LANES, pairs);
@@ -174,7 +178,7 @@ __device__ __forceinline__ void add_leaf(const TraceScene &S, PairLds *L, int &p
     const int fit = 64 / m;
     for (int done = 0; done < count;) {
         const int c = min(count - done, fit);
-        if (pairs + m * c > S.pair_limit) flush_pairs(S, L, pairs, inst, lo, ld, tmax, h, stats);
+        if (pairs + m * c > S.pair_limit) flush_pairs(S, L, pairs, inst, lo, ld, tmax, stats);
         if (in) {
             const int base = pairs + rank * c;
             for (int k = 0; k < c; ++k) L->pair[base + k] = lane | ((first + done + k) << 6);
@@ -195,93 +199,145 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 }
 
 // The bottom level of one instance walked lane by lane, for meshes whose triangles are small next to a
-// wavefront's tile: there each ray of the packet needs its own leaves, so a shared walk visits the union
-// of 64 paths. Each lane follows its own ray through the octant-ordered skip layout (a box test per step;
-// a hit descends to the first child, a miss or a finished leaf goes to the node's escape for the ray's
-// octant), so no stack is kept. Leaves wait until half the walking lanes have one, then all waiting lanes'
-// triangles are tested together as (ray, triangle) pairs. The root's box has been tested already.
-// How many quarters of the walking lanes wait at leaves before their triangles are tested.
-#ifndef NGX_LEAF_WAIT_QUARTERS
-#define NGX_LEAF_WAIT_QUARTERS 2
-#endif
+// wavefront's tile: there each ray of the packet needs its own leaves, so a shared walk visits the union of
+// 64 paths. A lane walks one ray through one subtree of the octant-ordered skip layout (a box test per step;
+// a hit descends to the first child for the ray's octant, a miss or a leaf goes to the node's escape for the
+// octant), so it keeps no stack. The work is shared: a lane whose remaining range is a whole subtree (its
+// node's escape is the range's end) gives the far child's subtree to a shared stack in LDS, while there is
+// room, and keeps the near one; idle lanes take the newest given subtrees, with their rays' mesh-space data
+// read from the rays' own lanes. So the few rays that enter an instance keep the whole wave busy. Leaves are
+// queued, and their triangles are tested in batches of up to 64 (ray, triangle) pairs once 32 leaves or 64
+// pairs wait, and at the end; after each batch every walking lane re-reads its ray's closest distance. Every
+// node a ray enters is walked by exactly one lane, so each ray's closest hit is its own.
+constexpr int SHARE_CAP = 128, LEAF_RING = 128;
+
+template <bool FORCE, class Stats>
+__device__ __forceinline__ void test_queued_leaves(const TraceScene &S, PairLds *L, int &head, int &count, const int g,
+                                                   const float3 lo, const float3 ld, float &tmax, Stats &stats) {
+    const int lane = lane_id();
+    while (count > 0) {
+        wave_lds_order();
+        const int avail = min(count, 64);
+        const bool mine = lane < avail;
+        const int slot = (head + lane) & (LEAF_RING - 1);
+        const int rc = mine ? L->leaf_ray_count[slot] : 0;
+        const int first = mine ? L->leaf_first[slot] : 0;
+        const int c = rc >> 6;
+        const int c0 = __builtin_amdgcn_readlane(c, 0);
+        if (c0 > 64) {
+            // a leaf of more than 64 triangles (the tree's leaves hold 4, unless a mesh has many coincident
+            // triangles): its next 64, the rest stays queued
+            const int ray0 = __builtin_amdgcn_readlane(rc, 0) & 63, first0 = __builtin_amdgcn_readlane(first, 0);
+            wave_lds_order();
+            L->pair[lane] = ray0 | ((first0 + lane) << 6);
+            if (lane == 0) {
+                L->leaf_first[head] = first0 + 64;
+                L->leaf_ray_count[head] = ray0 | ((c0 - 64) << 6);
+            }
+            int pairs = 64;
+            flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
+            continue;
+        }
+        const int incl = wave_prefix_sum(c);
+        const bool take = mine && incl <= 64;
+        const int taken = popc64(ballot(take));  // at least the first leaf, which holds 64 triangles at most
+        const int pairs_total = __builtin_amdgcn_readlane(incl, taken - 1);
+        if (!FORCE && pairs_total < 64 && count < 64) break;
+        if (take) {
+            const int ray = rc & 63;
+            for (int k = 0; k < c; ++k) L->pair[incl - c + k] = ray | ((first + k) << 6);
+        }
+        int pairs = pairs_total;
+        flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
+        head = (head + taken) & (LEAF_RING - 1);
+        count -= taken;
+    }
+}
+
 template <class Stats>
 __device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g, const uint64_t root_mask, const int root,
-                                    const float3 lo, const float3 ld, const float3 inv_ld, float &tmax, LaneHit &h, Stats &stats) {
+                                    const float3 lo, const float3 ld, const float3 inv_ld, float &tmax, Stats &stats) {
     const int lane = lane_id();
-    const int octant = (ld.x >= 0.0f ? 1 : 0) | (ld.y >= 0.0f ? 2 : 0) | (ld.z >= 0.0f ? 4 : 0);
-    // Lane states are kept as wave masks where they can be (live), and per-lane conditions enter them
-    // through ballots of plain comparisons, which cost one vector compare each.
-    uint64_t live = root_mask;
-    int cur = -1, leaf = -1, leaf_count = 0, after_leaf = -1;
-    auto first_child = [&](const int a, const int meta) {
-        const int code = -meta - 1, axis = code >> 1, flip = code & 1;
-        const bool positive = (octant >> axis) & 1;
-        return positive != (flip == 1) ? a : a + 1;
-    };
-    if (in_mask(live)) {
-        const SkipNode &r = S.skip[root];
-        if (r.meta > 0) {
-            leaf = r.a;
-            leaf_count = r.meta;
-            after_leaf = -1;
-        } else {
-            cur = first_child(r.a, r.meta);
-        }
-    }
+    if (in_mask(root_mask)) L->share[rank_in(root_mask)] = lane | (root << 6);
+    int top = popc64(root_mask);
+    int head = 0, queued = 0;
+    int ray = -1, cur = -1, end = -1, oct = 0;
+    float3 wlo = lo, wld = ld, winv = inv_ld;
+    float wlim = tmax;
     while (true) {
-        // box steps until half the live lanes (or all) wait at a leaf: waiting for every lane would leave the
-        // early ones idle, testing after each leaf would test few pairs at a time
-        while (true) {
-            const uint64_t at_leaf = live & ballot(leaf >= 0);
-            const uint64_t sm = live & ~at_leaf;  // a live lane is at a leaf or at a node
-            if (sm == 0 || (at_leaf != 0 && 4 * popc64(at_leaf) >= NGX_LEAF_WAIT_QUARTERS * popc64(live))) break;
-            stats.add(ST_BLAS_NODES, 1);
-            stats.add(ST_BLAS_LANES, popc64(sm));
-            const float lim = box_limit(tmax);
-            if (in_mask(sm)) {
-                const float4 *np = reinterpret_cast<const float4 *>(S.skip + cur);
-                const float4 f0 = np[0], f1 = np[1];
-                const int esc = S.skip[cur].escape[octant];
-                const int a = __float_as_int(f0.w), meta = __float_as_int(f1.w);
-                float tn;
-                if (box_hit(f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, lo, inv_ld, lim, tn)) {
-                    if (meta > 0) {
-                        leaf = a;
-                        leaf_count = meta;
-                        after_leaf = esc;
-                    } else {
-                        cur = first_child(a, meta);
-                    }
-                } else {
-                    cur = esc;
-                }
+        // idle lanes take the newest given subtrees
+        const uint64_t idle = ballot(ray < 0);
+        if (top > 0 && idle != 0) {
+            const int n = min(popc64(idle), top);
+            const int rk = rank_in(idle);
+            const bool take = in_mask(idle) && rk < n;
+            wave_lds_order();
+            int start = 0;
+            if (take) {
+                const int e = L->share[top - 1 - rk];
+                ray = e & 63;
+                start = e >> 6;
             }
-            live &= ~(ballot(leaf < 0) & ballot(cur < 0));
+            top -= n;
+            const int src = ray >= 0 ? ray : lane;
+            wlo = mk3(__shfl(lo.x, src, 64), __shfl(lo.y, src, 64), __shfl(lo.z, src, 64));
+            wld = mk3(__shfl(ld.x, src, 64), __shfl(ld.y, src, 64), __shfl(ld.z, src, 64));
+            winv = mk3(__shfl(inv_ld.x, src, 64), __shfl(inv_ld.y, src, 64), __shfl(inv_ld.z, src, 64));
+            wlim = __shfl(tmax, src, 64);
+            if (take) {
+                oct = (wld.x >= 0.0f ? 1 : 0) | (wld.y >= 0.0f ? 2 : 0) | (wld.z >= 0.0f ? 4 : 0);
+                cur = start;
+                end = S.skip[start].escape[oct];
+            }
         }
-        // the waiting lanes' triangles, as batched pairs
-        const uint64_t waiting = live & ballot(leaf >= 0);
-        if (waiting == 0) break;
-        // batches of exactly 64 pairs (the last one fewer), lanes in order: a lane takes what is left of its
-        // leaf up to the batch's free places
-        int done = 0;
-        while (true) {
-            const uint64_t mm = waiting & ballot(done < leaf_count);
-            if (mm == 0) break;
-            const int rem = in_mask(mm) ? leaf_count - done : 0;
-            const int incl = wave_prefix_sum(rem);
-            const int before = incl - rem;
-            const int take = max(0, min(rem, 64 - before));
-            for (int k = 0; k < take; ++k) L->pair[before + k] = lane | ((leaf + done + k) << 6);
-            int pairs = min(64, __builtin_amdgcn_readlane(incl, 63));
-            flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
-            done += take;
+        const uint64_t busy = ballot(ray >= 0);
+        if (busy == 0) break;
+        stats.add(ST_BLAS_NODES, 1);
+        stats.add(ST_BLAS_LANES, popc64(busy));
+        // One step in every walking lane; the decisions are taken as wave masks, and only the node loads are
+        // confined to the walking lanes (idle lanes see an empty box and their results are masked out).
+        float4 f0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), f1 = f0;
+        int esc = -1;
+        if (in_mask(busy)) {
+            const float4 *np = reinterpret_cast<const float4 *>(S.skip + cur);
+            f0 = np[0];
+            f1 = np[1];
+            esc = S.skip[cur].escape[oct];
         }
-        if (in_mask(waiting)) {
-            leaf = -1;
-            cur = after_leaf;
+        const int a = __float_as_int(f0.w), meta = __float_as_int(f1.w);
+        float tn;
+        const uint64_t hm = busy & ballot(box_hit(f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, wlo, winv, box_limit(wlim), tn));
+        const uint64_t lm = hm & ballot(meta > 0);  // leaves entered
+        const uint64_t im = hm & ~lm;               // interior nodes entered
+        const int code = -meta - 1, axis = (code >> 1) & 3, flip = code & 1;
+        const int near = ((oct >> axis) & 1) != flip ? a : a + 1;
+        const int far = 2 * a + 1 - near;
+        // a lane whose range is this node's subtree (its escape is the range's end) may give the far child away
+        const uint64_t gm = im & ballot(esc == end);
+        if (in_mask(busy)) cur = in_mask(im) ? near : esc;
+        if (gm != 0) {
+            const uint64_t given = gm & ballot(rank_in(gm) < SHARE_CAP - top);
+            if (in_mask(given)) {
+                L->share[top + rank_in(given)] = ray | (far << 6);
+                end = far;  // after the near subtree the walk would reach the far one: stop there
+            }
+            top += popc64(given);
         }
-        live &= ~(waiting & ballot(cur < 0));
+        if (lm != 0) {
+            if (in_mask(lm)) {
+                const int slot = (head + queued + rank_in(lm)) & (LEAF_RING - 1);
+                L->leaf_first[slot] = a;
+                L->leaf_ray_count[slot] = ray | (meta << 6);
+            }
+            queued += popc64(lm);
+        }
+        if (in_mask(busy) && cur == end) ray = -1;
+        if (queued >= 32) {
+            test_queued_leaves<false>(S, L, head, queued, g, lo, ld, tmax, stats);
+            wlim = __shfl(tmax, ray >= 0 ? ray : lane, 64);
+        }
     }
+    test_queued_leaves<true>(S, L, head, queued, g, lo, ld, tmax, stats);
 }
 
 // One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
@@ -304,14 +360,14 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     if (LANES && S.mesh_tris[meta.x] >= S.lane_min_tris) {
-        walk_instance_lanes(S, L, g, root_mask, S.mesh_skip_root[meta.x], lo, ld, inv_d, tmax, h, stats);
+        walk_instance_lanes(S, L, g, root_mask, S.mesh_skip_root[meta.x], lo, ld, inv_d, tmax, stats);
         return;
     }
     const int root_ref = __float_as_int(m0.w);
     int pairs = 0;
     if (root_ref < 0) {
         add_leaf(S, L, pairs, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h, stats);
-        flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
+        flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
         return;
     }
     LdsStack st(L, 1);
@@ -371,7 +427,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         cur = next_ref;
         cm = next_mask;
     }
-    flush_pairs(S, L, pairs, g, lo, ld, tmax, h, stats);
+    flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
 }
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
@@ -771,7 +827,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
 
 // One tile per wavefront, tiles dealt by the dispatcher.
 template <int WALK>
-__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 8 : WALK == WALK_HYBRID ? 6 : 1) void render_kernel(const RenderArgs A) {
+__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? 6 : 1) void render_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
     if (tile >= A.total_tiles) return;

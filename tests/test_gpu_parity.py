@@ -211,6 +211,27 @@ def test_full_lidar_batch_on_a_sample_of_worlds(gpu, oracle):
     assert rep["hits"] > 0.1 * rep["pixels"]
 
 
+@pytest.mark.parametrize("walk", list(WALKS))
+def test_a_mesh_of_coincident_triangles(gpu, oracle, walk):
+    """A mesh of 150 copies of one triangle: every ray that meets it meets 150 surfaces at one distance, spread
+    over many leaves that all overlap; the depth is exact and the ID the instance's whichever copy is kept."""
+    import neraulgraphics_amd as ng
+    _say(f"coincident triangles, {walk} walk")
+    v = np.array([[-3.0, -3.0, 0.0], [3.0, -3.0, 0.0], [0.0, 3.0, 0.0]], np.float32)
+    mesh = ng.Mesh(np.tile(v, (150, 1)), np.arange(450, dtype=np.int32))
+    cfg = workloads.scattered(4, 6, width=32, height=32, spread=0.5)
+    cfg["meshes"] = [mesh, cfg["meshes"][1]]
+    r = workloads.make_renderer(cfg)
+    r.set_trace_config(**WALKS[walk])
+    if walk == "hybrid":
+        r.set_trace_config(lane_min_triangles=0)
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    ref = forward(cfg, oracle)
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "coincident")
+    _say(rep)
+    assert rep["hits"] > 100
+
+
 def test_worlds_do_not_see_one_another(gpu):
     """tests/test_batch.cpp:185-214: boxes at z=-8 and z=-3 in two worlds, seen from the origin."""
     import neraulgraphics_amd as ng
