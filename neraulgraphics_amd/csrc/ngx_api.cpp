@@ -111,7 +111,7 @@ struct ngx_ctx {
     int *d_mesh_skip_root = nullptr, *d_mesh_tris = nullptr;
     int max_mesh_tris = 0;
     ngx::MeshRoot *d_mesh_roots = nullptr;
-    float4 *d_tris = nullptr;
+    float *d_tris = nullptr;            // 9 floats per triangle
     float4 *d_attrs = nullptr;
     float4 *d_spheres = nullptr;
 
@@ -403,7 +403,14 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             upload(c.d_mesh_skip_root, built.roots.data(), built.roots.size());
             upload(c.d_mesh_tris, built.tri_counts.data(), built.tri_counts.size());
             for (const int t : built.tri_counts) c.max_mesh_tris = std::max(c.max_mesh_tris, t);
-            upload(c.d_tris, reinterpret_cast<const float4 *>(built.triangles.data()), built.triangles.size() / 4);
+            {
+                // v0, e1, e2 without the padding float of each: 36 bytes a triangle on the device
+                std::vector<float> packed;
+                packed.reserve(built.triangles.size() / 12 * 9);
+                for (size_t t = 0; t + 12 <= built.triangles.size(); t += 12)
+                    for (int k = 0; k < 3; ++k) packed.insert(packed.end(), &built.triangles[t + 4 * k], &built.triangles[t + 4 * k + 3]);
+                upload(c.d_tris, packed.data(), packed.size());
+            }
             if (!built.attributes.empty())
                 upload(c.d_attrs, reinterpret_cast<const float4 *>(built.attributes.data()), built.attributes.size() / 4);
             std::vector<float> mesh_bounds;

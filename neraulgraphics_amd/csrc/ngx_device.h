@@ -137,6 +137,14 @@ __device__ __forceinline__ bool box_hit(const float lx, const float ly, const fl
     return tn <= tf;
 }
 
+// Triangle j of the packed array (v0, e1, e2: 9 floats).
+__device__ __forceinline__ void load_tri(const float *tris, const size_t j, float3 &v0, float3 &e1, float3 &e2) {
+    const float *p = tris + 9 * j;
+    v0 = make_float3(p[0], p[1], p[2]);
+    e1 = make_float3(p[3], p[4], p[5]);
+    e2 = make_float3(p[6], p[7], p[8]);
+}
+
 // Two-sided Moller-Trumbore (mesh_dev.h:80-99), triangle given as v0, e1, e2.
 __device__ __forceinline__ bool triangle_hit(const float3 v0, const float3 e1, const float3 e2, const float3 o, const float3 d,
                                              const float tmax, float &t, float &bu, float &bv) {

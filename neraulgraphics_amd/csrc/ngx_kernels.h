@@ -51,7 +51,7 @@ struct TraceScene {
     const InstanceRec *inst;
     const MeshRoot *mesh_roots;
     const WideNode *blas_wide;
-    const float4 *tris;        // 3 per triangle: v0, e1, e2
+    const float *tris;         // 9 floats per triangle: v0, e1, e2 (packed: the walks fetch 36 bytes a triangle)
     const float4 *attrs;       // 4 per triangle (normals + flags) or null
     int sphere_count;
     const float4 *spheres;     // centre xyz, radius

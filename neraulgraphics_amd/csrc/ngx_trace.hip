@@ -144,9 +144,10 @@ __device__ __forceinline__ void flush_pairs(const TraceScene &S, PairLds *L, int
     const float3 pd = mk3(__shfl(ld.x, r, 64), __shfl(ld.y, r, 64), __shfl(ld.z, r, 64));
     const float pt = __shfl(tmax, r, 64);
     if (act) {
-        const float4 *tp = S.tris + 3 * (size_t)(e >> 6);
+        float3 v0, e1, e2;
+        load_tri(S.tris, (size_t)(e >> 6), v0, e1, e2);
         float t, bu, bv;
-        if (triangle_hit(xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), po, pd, pt, t, bu, bv)) {
+        if (triangle_hit(v0, e1, e2, po, pd, pt, t, bu, bv)) {
             atomicMin(&L->best[r], ((unsigned long long)__float_as_uint(t) << 32) | (unsigned)lane);
             L->uv[lane] = make_float2(bu, bv);
         }
@@ -641,9 +642,10 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
             if (tris) {
                 const int first = ~leaf_ref;
                 for (int j = first; j < first + leaf_count; ++j) {
-                    const float4 *tp = S.tris + 3 * (size_t)j;
+                    float3 v0, e1, e2;
+                    load_tri(S.tris, (size_t)j, v0, e1, e2);
                     float t, bu, bv;
-                    if (triangle_hit(xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), lo, ld, tmax, t, bu, bv)) {
+                    if (triangle_hit(v0, e1, e2, lo, ld, tmax, t, bu, bv)) {
                         tmax = Stats::kAnyHit ? -1.0f : t;
                         h.inst = g;
                         h.tri = j;
@@ -722,8 +724,9 @@ __device__ __forceinline__ void surface(const TraceScene &S, const int kind, con
     if (kind == OBJECT_INSTANCE) {
         const InstanceRec &rec = S.inst[h.inst];
         const float3 r0 = xyz(rec.inv[0]), r1 = xyz(rec.inv[1]), r2 = xyz(rec.inv[2]);
-        const float4 e1 = S.tris[3 * (size_t)h.tri + 1], e2 = S.tris[3 * (size_t)h.tri + 2];
-        const float3 nl = cross3(xyz(e1), xyz(e2));
+        float3 v0, e1, e2;
+        load_tri(S.tris, (size_t)h.tri, v0, e1, e2);
+        const float3 nl = cross3(e1, e2);
         float3 geometric = normalize3(add3(add3(mul3(r0, nl.x), mul3(r1, nl.y)), mul3(r2, nl.z)));
         if (dot3(geometric, d) > 0.0f) geometric = neg3(geometric);
         normal = geometric;
@@ -825,9 +828,14 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
     if (A.object_id) A.object_id[idx] = object;
 }
 
+// Workgroups per compute unit the hybrid render kernels are built for (their registers follow: 6 gives 80 VGPRs).
+#ifndef NGX_HYBRID_MIN_BLOCKS
+#define NGX_HYBRID_MIN_BLOCKS 6
+#endif
+
 // One tile per wavefront, tiles dealt by the dispatcher.
 template <int WALK>
-__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? 6 : 1) void render_kernel(const RenderArgs A) {
+__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? NGX_HYBRID_MIN_BLOCKS : 1) void render_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
     if (tile >= A.total_tiles) return;
@@ -842,7 +850,7 @@ __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID 
 constexpr int PERSISTENT_CHUNK = 8;
 
 template <int WALK>
-__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? 6 : 1) void render_persistent_kernel(const RenderArgs A) {
+__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? NGX_HYBRID_MIN_BLOCKS : 1) void render_persistent_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const int xcd = blockIdx.x & 7;
     const long long first = A.total_tiles * xcd / 8, count = A.total_tiles * (xcd + 1) / 8 - first;
@@ -1216,15 +1224,16 @@ __device__ __forceinline__ bool hit_pose_terms(const TraceScene &S, const int wo
     body = body_of_instance[rec.tmpl];
     if (body < 0) return false;
     const float3 r0 = xyz(rec.inv[0]), r1 = xyz(rec.inv[1]), r2 = xyz(rec.inv[2]);
-    const float4 v0 = S.tris[3 * (size_t)h.tri], e1 = S.tris[3 * (size_t)h.tri + 1], e2 = S.tris[3 * (size_t)h.tri + 2];
-    const float3 nl = cross3(xyz(e1), xyz(e2));
+    float3 v0, e1, e2;
+    load_tri(S.tris, (size_t)h.tri, v0, e1, e2);
+    const float3 nl = cross3(e1, e2);
     const float3 n = add3(add3(mul3(r0, nl.x), mul3(r1, nl.y)), mul3(r2, nl.z));
     const float nd = dot3(n, d);
     if (nd == 0.0f) return false;
     const float s = dl_dt / nd;
     const float3 a = mul3(n, s);
     // the hit's point in mesh space, then in the body's frame: y = Rl diag(s) x + pl
-    const float3 x = add3(add3(xyz(v0), mul3(xyz(e1), h.u)), mul3(xyz(e2), h.v));
+    const float3 x = add3(add3(v0, mul3(e1, h.u)), mul3(e2, h.v));
     const float *L = local_rot_scale + 12 * (size_t)rec.tmpl;
     const float3 y = mk3(L[0] * x.x + L[1] * x.y + L[2] * x.z + L[3], L[4] * x.x + L[5] * x.y + L[6] * x.z + L[7],
                          L[8] * x.x + L[9] * x.y + L[10] * x.z + L[11]);
