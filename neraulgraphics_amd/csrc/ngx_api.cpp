@@ -202,7 +202,7 @@ struct ngx_ctx {
         s.floor_zmin = floor[2];
         s.floor_zmax = floor[3];
         s.pair_limit = pair_limit;
-        s.skip = d_skip;
+        s.skip = d_skip ? d_skip + 1 : nullptr;
         s.mesh_skip_root = d_mesh_skip_root;
         s.mesh_tris = d_mesh_tris;
         s.lane_min_tris = lane_min_tris;
@@ -399,7 +399,13 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             upload(c.d_blas_parent, built.wide_parent.data(), built.wide_parent.size());
             if (!c.d_blas_parent) device_alloc(c.d_blas_parent, 1);
             upload(c.d_mesh_roots, built.mesh_roots.data(), built.mesh_roots.size());
-            upload(c.d_skip, built.skip.data(), built.skip.size());
+            {
+                // one leading unused record: node i then sits at 64 (i + 1) bytes, so the child pairs (2k + 1, 2k + 2)
+                // of a mesh whose nodes start at an even index share one 128-byte cache line
+                std::vector<ngx::SkipNode> shifted(built.skip.size() + 1);
+                std::copy(built.skip.begin(), built.skip.end(), shifted.begin() + 1);
+                upload(c.d_skip, shifted.data(), shifted.size());
+            }
             upload(c.d_mesh_skip_root, built.roots.data(), built.roots.size());
             upload(c.d_mesh_tris, built.tri_counts.data(), built.tri_counts.size());
             for (const int t : built.tri_counts) c.max_mesh_tris = std::max(c.max_mesh_tris, t);
