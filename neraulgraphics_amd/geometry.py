@@ -30,6 +30,10 @@ class Mesh:
     def triangle_count(self):
         return self.indices.size // 3
 
+    @property
+    def vertex_count(self):
+        return len(self.vertices)
+
 
 @dataclass
 class Transform:
