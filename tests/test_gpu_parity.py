@@ -6,6 +6,7 @@ matrices bit-exact. Images and rays: see helpers.compare_images."""
 import numpy as np
 import pytest
 
+import neraulgraphics_amd as ng
 from neraulgraphics_amd import OBJECT_INSTANCE, workloads
 from oracle.oracle import forward
 
@@ -230,6 +231,38 @@ def test_a_mesh_of_coincident_triangles(gpu, oracle, walk):
     rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "coincident")
     _say(rep)
     assert rep["hits"] > 100
+
+
+@pytest.mark.parametrize("walk", list(WALKS))
+def test_several_cameras_per_world_and_a_ragged_frame(gpu, oracle, walk):
+    """Three cameras per world (view v sees world v // 3), on a 45 x 29 frame whose last tiles are partly outside
+    it; depth, normal and ID, and the RGB picture."""
+    _say(f"3 cameras per world, 45x29, {walk} walk")
+    cfg = workloads.make("instanced_worlds", worlds=6)
+    rng = np.random.default_rng(21)
+    cams = []
+    for w in range(cfg["worlds"]):
+        target = cfg["positions"][w].mean(axis=0)
+        for k in range(3):
+            cams.append(ng.look_at(target + rng.uniform(-6, 6, size=3) + np.array([0.0, 0.0, 8.0]), target).reshape(12))
+    cfg.update(cameras_per_world=3, cameras=np.array(cams, np.float32), width=45, height=29)
+    r = workloads.make_renderer(cfg)
+    r.set_trace_config(**WALKS[walk])
+    out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+    assert out["depth"].shape == (18, 29, 45)
+    ref = forward(cfg, oracle)
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "3 cameras")
+    _say(rep)
+    assert rep["hits"] > 0.1 * rep["pixels"]
+    c = dict(cfg)
+    c["shading"] = ng.shading_arrays(cfg["instances"], [], ng.Plane(), [ng.Light((0.0, 10.0, -5.0), radius=0.3)], 0.1, None)
+    r2 = workloads.make_renderer(cfg, lights=[ng.Light((0.0, 10.0, -5.0), radius=0.3)])
+    r2.set_trace_config(**WALKS[walk])
+    r2.set_ambient(0.1)
+    got = r2.render(cfg["cameras"], cfg["fov"], 1, rgb=True, frame=2)["rgb"].astype(np.int32)
+    c["spp"], c["frame"] = 1, 2
+    want = forward(c, oracle, outputs=("rgb",))["rgb"].astype(np.int32)
+    assert np.abs(got - want).mean() < 0.05
 
 
 def test_worlds_do_not_see_one_another(gpu):
