@@ -107,6 +107,7 @@ struct ngx_ctx {
     float *d_mesh_bounds = nullptr;
     ngx::WideNode *d_blas_wide = nullptr;
     int *d_blas_parent = nullptr;
+    int *d_blas_wide_node = nullptr;
     ngx::SkipNode *d_skip = nullptr;
     int *d_mesh_skip_root = nullptr, *d_mesh_tris = nullptr;
     int max_mesh_tris = 0;
@@ -167,7 +168,7 @@ struct ngx_ctx {
     int *d_tile_counters = nullptr;
 
     ~ngx_ctx() {
-        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_wide_parent,
+        int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_blas_wide_node, d_wide_parent,
                        d_tile_counters, d_mesh_skip_root, d_mesh_tris};
         for (int *p : ints) if (p) (void)hipFree(p);
         float *floats[] = {d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
@@ -188,6 +189,7 @@ struct ngx_ctx {
         s.tlas_wide = d_wide;
         s.tlas_parent = d_wide_parent;
         s.blas_parent = d_blas_parent;
+        s.blas_wide_node = d_blas_wide_node;
         s.roots = d_roots;
         s.total = total;
         s.inst = d_inst;
@@ -398,6 +400,8 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             if (!c.d_blas_wide) device_alloc(c.d_blas_wide, 1);
             upload(c.d_blas_parent, built.wide_parent.data(), built.wide_parent.size());
             if (!c.d_blas_parent) device_alloc(c.d_blas_parent, 1);
+            upload(c.d_blas_wide_node, built.wide_node.data(), built.wide_node.size());
+            if (!c.d_blas_wide_node) device_alloc(c.d_blas_wide_node, 1);
             upload(c.d_mesh_roots, built.mesh_roots.data(), built.mesh_roots.size());
             {
                 // one leading unused record: node i then sits at 64 (i + 1) bytes, so the child pairs (2k + 1, 2k + 2)

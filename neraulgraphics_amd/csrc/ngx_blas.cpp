@@ -206,6 +206,9 @@ void build_meshes(const MeshInput &in, BuiltMeshes &out) {
     for (size_t i = 0; i < out.nodes.size(); ++i)
         if (out.nodes[i].count == 0) wide_index[i] = wide_count++;
     out.wide.resize(static_cast<size_t>(wide_count));
+    out.wide_node.assign(static_cast<size_t>(wide_count), -1);
+    for (size_t i = 0; i < out.nodes.size(); ++i)
+        if (wide_index[i] >= 0) out.wide_node[wide_index[i]] = static_cast<int>(i);
     auto child_ref = [&](const BlasNode &c, const int index, int32_t &ref, int32_t &count) {
         if (c.count > 0) {
             ref = ~c.a;

@@ -29,6 +29,7 @@ struct BuiltMeshes {
     std::vector<Box> bounds;          // per mesh, in its own space
     std::vector<WideNode> wide;       // the traversal layout of nodes' interior nodes
     std::vector<int> wide_parent;     // per wide node: its parent wide node, -1 at a mesh's root
+    std::vector<int> wide_node;       // per wide node: its node in `nodes` (and `skip`)
     std::vector<MeshRoot> mesh_roots; // per mesh, for the tracer
     std::vector<SkipNode> skip;       // per node of `nodes`: the per-lane walk's layout
     std::vector<int> tri_counts;      // per mesh

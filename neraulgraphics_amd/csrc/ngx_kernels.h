@@ -46,6 +46,7 @@ struct TraceScene {
     const WideNode *tlas_wide;
     const int *tlas_parent;    // per wide node, -1 at a world's root
     const int *blas_parent;    // per bottom-level wide node, -1 at a mesh's root
+    const int *blas_wide_node; // per bottom-level wide node, its node in the reference layout (and the skip layout)
     const int *roots;
     int total;
     const InstanceRec *inst;

@@ -211,6 +211,11 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 // pairs wait, and at the end; after each batch every walking lane re-reads its ray's closest distance. Every
 // node a ray enters is walked by exactly one lane, so each ray's closest hit is its own.
 constexpr int SHARE_CAP = 128, LEAF_RING = 128;
+// In the hybrid walk, a bottom-level subtree entered by at most this many of the packet's rays is walked by the
+// shared per-lane walk instead of the packet (0: never).
+#ifndef NGX_HANDOFF_LANES
+#define NGX_HANDOFF_LANES 16
+#endif
 
 template <bool FORCE, class Stats>
 __device__ __forceinline__ void test_queued_leaves(const TraceScene &S, PairLds *L, int &head, int &count, const int g,
@@ -255,12 +260,17 @@ __device__ __forceinline__ void test_queued_leaves(const TraceScene &S, PairLds 
     }
 }
 
+// Gives the subtree of skip node `node` to the shared stack for every ray in mask (the caller has checked the room).
+__device__ __forceinline__ void share_subtree(PairLds *L, int &top, const uint64_t mask, const int node) {
+    if (in_mask(mask)) L->share[top + rank_in(mask)] = lane_id() | (node << 6);
+    top += popc64(mask);
+}
+
+// Walks the `top` subtrees on the shared stack (rays of instance g, whose mesh-space rays are in their own lanes).
 template <class Stats>
-__device__ void walk_instance_lanes(const TraceScene &S, PairLds *L, const int g, const uint64_t root_mask, const int root,
-                                    const float3 lo, const float3 ld, const float3 inv_ld, float &tmax, Stats &stats) {
+__device__ void walk_shared(const TraceScene &S, PairLds *L, const int g, int top, const float3 lo, const float3 ld,
+                            const float3 inv_ld, float &tmax, Stats &stats) {
     const int lane = lane_id();
-    if (in_mask(root_mask)) L->share[rank_in(root_mask)] = lane | (root << 6);
-    int top = popc64(root_mask);
     int head = 0, queued = 0;
     int ray = -1, cur = -1, end = -1, oct = 0;
     float3 wlo = lo, wld = ld, winv = inv_ld;
@@ -360,8 +370,10 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     const uint64_t root_mask = mask & ballot(box_hit(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_d, box_limit(tmax), root_t));
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
+    int shared = 0;  // subtrees given to the shared per-lane walk
     if (LANES && S.mesh_tris[meta.x] >= S.lane_min_tris) {
-        walk_instance_lanes(S, L, g, root_mask, S.mesh_skip_root[meta.x], lo, ld, inv_d, tmax, stats);
+        share_subtree(L, shared, root_mask, S.mesh_skip_root[meta.x]);
+        walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
         return;
     }
     const int root_ref = __float_as_int(m0.w);
@@ -405,10 +417,17 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         } else {
             next_ref = 0; next_count = 0; next_mask = 0;
         }
-        // Leaves are queued where they are reached; then the walk resumes from the stack.
+        // Leaves are queued where they are reached; a subtree entered by few rays is given to the shared per-lane
+        // walk (hybrid walk only); then the walk resumes from the stack.
         bool finished = false;
-        while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) add_leaf(S, L, pairs, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
+        while (true) {
+            if (next_mask != 0 && next_ref >= 0) {
+                const int n = popc64(next_mask);
+                if (!LANES || n > NGX_HANDOFF_LANES || shared + n > SHARE_CAP) break;
+                share_subtree(L, shared, next_mask, S.blas_wide_node[next_ref]);
+            } else if (next_mask != 0) {
+                add_leaf(S, L, pairs, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
+            }
             if (st.sp == 0) {
                 finished = true;
                 break;
@@ -429,6 +448,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         cm = next_mask;
     }
     flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
+    if (LANES && shared > 0) walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
 }
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
