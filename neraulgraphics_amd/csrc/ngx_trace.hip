@@ -30,17 +30,23 @@ enum StatIndex { ST_TLAS_NODES, ST_TLAS_LANES, ST_INSTANCES, ST_INSTANCE_LANES, 
 // The same type also says whether the walk is an occlusion test (kAnyHit): then a ray's limit drops below zero
 // at its first hit, so its lane leaves every box after it and the walk ends as soon as every ray is blocked or
 // done; whether a ray is blocked is all that is read (mesh_dev.h:208-227 stops at the first hit too).
+// kHandoff says whether the hybrid walk may hand thin subtrees to the shared per-lane walk; the RGB kernel, whose
+// registers are already spent on shading, walks without it.
 struct NoStats {
-    static constexpr bool kAnyHit = false;
+    static constexpr bool kAnyHit = false, kHandoff = true;
+    __device__ __forceinline__ void add(int, int) {}
+};
+struct ShadeRays {
+    static constexpr bool kAnyHit = false, kHandoff = false;
     __device__ __forceinline__ void add(int, int) {}
 };
 struct Occlusion {
-    static constexpr bool kAnyHit = true;
+    static constexpr bool kAnyHit = true, kHandoff = false;
     __device__ __forceinline__ void add(int, int) {}
 };
 #ifdef NGX_TRACE_STATS
 struct WaveStats {
-    static constexpr bool kAnyHit = false;
+    static constexpr bool kAnyHit = false, kHandoff = true;
     int c[ST_COUNT];
     __device__ __forceinline__ void add(const int k, const int v) { c[k] += v; }
 };
@@ -423,7 +429,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         while (true) {
             if (next_mask != 0 && next_ref >= 0) {
                 const int n = popc64(next_mask);
-                if (!LANES || n > NGX_HANDOFF_LANES || shared + n > SHARE_CAP) break;
+                if (!LANES || !Stats::kHandoff || n > NGX_HANDOFF_LANES || shared + n > SHARE_CAP) break;
                 share_subtree(L, shared, next_mask, S.blas_wide_node[next_ref]);
             } else if (next_mask != 0) {
                 add_leaf(S, L, pairs, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
@@ -1060,7 +1066,7 @@ __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const lon
         for (int depth = 0; ballot(active); ++depth) {
             float best;
             LaneHit h;
-            NoStats stats;
+            ShadeRays stats;
             int kind = OBJECT_NONE;
             if (depth <= MAX_BOUNCES) kind = scene_hit<WALK>(A.s, L, world, o, d, ballot(active), best, h, stats);
             const bool hit = active && kind != OBJECT_NONE;
