@@ -170,21 +170,23 @@ def test_rays_of_uneven_worlds_per_wave(gpu, oracle, walk):
     compare_rays(out, ref, "37 rays per world")
 
 
-def test_full_body_worlds_batch_on_a_sample_of_worlds(gpu, oracle):
-    """The benchmark's batch at full size (4096 worlds at 128x128, 534,919 instances from body poses) rendered in one
-    launch with the default (tuned) configuration; 16 worlds spread over the batch, including the first and last,
-    are checked against the oracle run on those worlds alone. A world's tree and its pixels depend only on its
-    own instances, so the sample is checked as the whole batch would be."""
+@pytest.mark.parametrize("name", ["body_worlds", "instanced_worlds"])
+def test_full_batch_on_a_sample_of_worlds(gpu, oracle, name):
+    """A benchmark batch at full size (body_worlds: 4096 worlds at 128x128, 534,919 instances from body poses;
+    instanced_worlds: 4096 worlds at 64x64, 131,072 instances from matrices) rendered in one launch with the default
+    configuration; 16 worlds spread over the batch, including the first and last, are checked against the oracle
+    run on those worlds alone. A world's tree and its pixels depend only on its own instances, so the sample is
+    checked as the whole batch would be."""
     import torch
-    _say("full-size parity: body_worlds, 4096 worlds 128x128, 16 sampled worlds")
-    cfg = workloads.make("body_worlds")
+    _say(f"full-size parity: {name}, 4096 worlds, 16 sampled worlds")
+    cfg = workloads.make(name)
     r = workloads.make_renderer(cfg)
     out = r.render(torch.from_numpy(cfg["cameras"]).cuda(), cfg["fov"], depth=True, normals=True, object_ids=True, zero_copy=True)
     torch.cuda.synchronize()
     sample = np.unique(np.concatenate([[0, cfg["worlds"] - 1], np.random.default_rng(5).choice(cfg["worlds"], 14, replace=False)]))
     got = {k: out[k][torch.from_numpy(sample).cuda()].cpu().numpy() for k in ("depth", "normals", "object_ids")}
     ref = forward(world_subset(cfg, sample), oracle)
-    rep = compare_images(dict(depth=got["depth"], object_id=got["object_ids"], normal=got["normals"]), ref, "body_worlds sample")
+    rep = compare_images(dict(depth=got["depth"], object_id=got["object_ids"], normal=got["normals"]), ref, f"{name} sample")
     _say(rep)
     assert rep["hits"] > 0.3 * rep["pixels"]
 
