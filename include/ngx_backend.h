@@ -26,6 +26,7 @@
  *    template; the reference has the same list in every world (include/neraulgraphics/batch.h:7-9);
  *  - ngx_pose_backward_depth / ngx_pose_backward_range: gradients of depth and range with respect to
  *    body poses;
+ *  - ngx_set_mesh_vertices: deforming meshes, refitted on the device;
  *  - ngx_debug_*: the top-level build's intermediates, for checking it against the reference algorithm.
  */
 #ifndef NGX_BACKEND_H
@@ -159,6 +160,12 @@ ngx_status ngx_bind_bodies(ngx_ctx *ctx, int32_t bodies_per_world, const int32_t
 /* Backend::set_body_poses (src/backend.h:65); checks of src/batch.cpp:131-144.
  * worlds x bodies_per_world x 7 floats. Composed in FP64 as src/instances.cpp:137-164 does. */
 ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memory, void *stream);
+
+/* No counterpart in the reference (its bottom levels are built once): moves the vertices of mesh `mesh` (as many
+ * as it was created with, 3 floats each) and refits its bottom level on the device, keeping the tree. The top
+ * level holds the mesh's old box until the next ngx_set_instance_matrices or ngx_set_body_poses, and renders and
+ * ray queries return NGX_BAD_STATE until then. Meshes with vertex normals are refused (NGX_UNSUPPORTED). */
+ngx_status ngx_set_mesh_vertices(ngx_ctx *ctx, int32_t mesh, const float *vertices, ngx_memory memory, void *stream);
 
 /* Materials, lights, ambient and sky for the RGB picture: the reference takes them with the scene
  * (src/backend.h:19-34, :74-80); here they are set after creation, before the first RGB render

@@ -132,6 +132,7 @@ class BatchRenderer:
         self._width, self._height = width, height
         self._worlds, self._cpw = world_count, cameras_per_world
         self._per_world = len(instances)
+        self._mesh_vertex_counts = [len(m.vertices) for m in meshes]
         self._mesh_signature = ",".join(str(len(m.indices) // 3 if hasattr(m, "indices") else 0) for m in meshes)
         self._bodies = None
         self._device_id = int(device_id)
@@ -312,6 +313,18 @@ class BatchRenderer:
         n = self._worlds * self._bodies * 7
         p, mem, keep = _as_input(body_poses, "body poses (world_count, bodies_per_world, 7)", n)
         _lib.check(self._lib.ngx_set_body_poses(self._h, p, mem, self._stream_for(mem)))
+        self._hold(keep, mem)
+
+    @_serialized
+    def set_mesh_vertices(self, mesh, vertices):
+        """Moves mesh `mesh`'s vertices ((vertex_count, 3) float32, NumPy or a GPU tensor) and refits its bottom level on
+        the GPU, keeping its tree (no counterpart in the reference, whose meshes are fixed). Set the instance
+        transforms or body poses again before the next render: the top level still holds the mesh's old box."""
+        if not 0 <= int(mesh) < len(self._mesh_vertex_counts):
+            raise ValueError(f"mesh {mesh} is not one of the scene's {len(self._mesh_vertex_counts)} meshes")
+        n = 3 * self._mesh_vertex_counts[int(mesh)]
+        p, mem, keep = _as_input(vertices, f"vertices of mesh {mesh} ({n // 3}, 3)", n)
+        _lib.check(self._lib.ngx_set_mesh_vertices(self._h, int(mesh), p, mem, self._stream_for(mem)))
         self._hold(keep, mem)
 
     # ---- images and rays

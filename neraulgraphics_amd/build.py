@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libngx.so")
-SOURCES = ["ngx_tlas.hip", "ngx_trace.hip", "ngx_selftest.hip", "ngx_api.cpp", "ngx_blas.cpp"]
+SOURCES = ["ngx_tlas.hip", "ngx_trace.hip", "ngx_refit.hip", "ngx_selftest.hip", "ngx_api.cpp", "ngx_blas.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",

@@ -108,6 +108,12 @@ struct ngx_ctx {
     ngx::WideNode *d_blas_wide = nullptr;
     int *d_blas_parent = nullptr;
     int *d_blas_wide_node = nullptr;
+    // deforming meshes (ngx_set_mesh_vertices)
+    float *d_vertices = nullptr;        // every mesh's vertices, 3 floats each
+    int *d_tri_verts = nullptr, *d_node_parent = nullptr, *d_node_wide = nullptr, *d_node_arrivals = nullptr;
+    std::vector<int> mesh_vertex_base, mesh_vertex_count, mesh_node_base, mesh_node_count, mesh_tri_base, mesh_tri_count;
+    std::vector<uint8_t> mesh_has_normals;
+    bool top_level_stale = false;       // a mesh moved since the last pose update
     ngx::SkipNode *d_skip = nullptr;
     int *d_mesh_skip_root = nullptr, *d_mesh_tris = nullptr;
     int max_mesh_tris = 0;
@@ -169,9 +175,10 @@ struct ngx_ctx {
 
     ~ngx_ctx() {
         int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_blas_wide_node, d_wide_parent,
+                       d_tri_verts, d_node_parent, d_node_wide, d_node_arrivals,
                        d_tile_counters, d_mesh_skip_root, d_mesh_tris};
         for (int *p : ints) if (p) (void)hipFree(p);
-        float *floats[] = {d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
+        float *floats[] = {d_vertices, d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
                            d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
                            d_dbg_world_bounds};
         for (float *p : floats) if (p) (void)hipFree(p);
@@ -285,6 +292,7 @@ struct ngx_ctx {
         a.matrices_in = matrices_wp12;
         build(a, stream);
         last_body_poses = nullptr;
+        top_level_stale = false;
     }
 
     void build_from_bodies(const float *poses, hipStream_t stream) {
@@ -294,6 +302,7 @@ struct ngx_ctx {
         a.body_poses = poses;
         build(a, stream);
         last_body_poses = poses;
+        top_level_stale = false;
     }
 };
 
@@ -411,6 +420,20 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
                 upload(c.d_skip, shifted.data(), shifted.size());
             }
             upload(c.d_mesh_skip_root, built.roots.data(), built.roots.size());
+            upload(c.d_vertices, s.vertices, 3 * (size_t)s.vertex_offsets[s.mesh_count]);
+            upload(c.d_tri_verts, built.tri_verts.data(), built.tri_verts.size());
+            upload(c.d_node_parent, built.node_parent.data(), built.node_parent.size());
+            upload(c.d_node_wide, built.node_wide.data(), built.node_wide.size());
+            device_alloc(c.d_node_arrivals, built.nodes.size());
+            for (int m = 0; m < s.mesh_count; ++m) {
+                c.mesh_vertex_base.push_back(s.vertex_offsets[m]);
+                c.mesh_vertex_count.push_back(s.vertex_offsets[m + 1] - s.vertex_offsets[m]);
+                c.mesh_has_normals.push_back(s.normals && s.mesh_has_normals && s.mesh_has_normals[m] ? 1 : 0);
+            }
+            c.mesh_node_base = built.node_base;
+            c.mesh_node_count = built.node_count;
+            c.mesh_tri_base = built.tri_base;
+            c.mesh_tri_count = built.tri_counts;
             upload(c.d_mesh_tris, built.tri_counts.data(), built.tri_counts.size());
             for (const int t : built.tri_counts) c.max_mesh_tris = std::max(c.max_mesh_tris, t);
             {
@@ -601,6 +624,45 @@ void pack_material(const ngx_material &m, float4 out[2]) {
 }
 }  // namespace
 
+ngx_status ngx_set_mesh_vertices(ngx_ctx *ctx, const int32_t mesh, const float *vertices, const ngx_memory memory, void *stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(mesh >= 0 && mesh < (int)ctx->mesh_vertex_count.size(),
+                "mesh " + std::to_string(mesh) + " is not one of the scene's " + std::to_string(ctx->mesh_vertex_count.size()) +
+                    " meshes (a scene with no instances keeps none)");
+        if (ctx->mesh_has_normals[mesh])
+            fail(NGX_UNSUPPORTED, "mesh " + std::to_string(mesh) + " has vertex normals, which a refit does not move");
+        const size_t n = 3 * (size_t)ctx->mesh_vertex_count[mesh];
+        require(n == 0 || vertices != nullptr, "vertices must not be null");
+        if (memory == NGX_MEM_HOST) require(finite_all(vertices, n), "mesh " + std::to_string(mesh) + " has a NaN or infinite vertex");
+        if (ctx->mesh_tri_count[mesh] == 0) return;
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        float *dst = ctx->d_vertices + 3 * (size_t)ctx->mesh_vertex_base[mesh];
+        HIP_CHECK(hipMemcpyAsync(dst, vertices, n * sizeof(float),
+                                 memory == NGX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+        ngx::MeshRefitArgs a{};
+        a.vertices = dst;
+        a.vertex_base = ctx->mesh_vertex_base[mesh];
+        a.tri_verts = ctx->d_tri_verts;
+        a.tri_base = ctx->mesh_tri_base[mesh];
+        a.tri_count = ctx->mesh_tri_count[mesh];
+        a.tris = ctx->d_tris;
+        a.skip = ctx->d_skip + 1;
+        a.node_parent = ctx->d_node_parent;
+        a.node_wide = ctx->d_node_wide;
+        a.wide = ctx->d_blas_wide;
+        a.node_base = ctx->mesh_node_base[mesh];
+        a.node_count = ctx->mesh_node_count[mesh];
+        a.root = ctx->mesh_node_base[mesh];
+        a.arrivals = ctx->d_node_arrivals;
+        a.mesh_root = ctx->d_mesh_roots + mesh;
+        a.mesh_bounds = ctx->d_mesh_bounds + 6 * (size_t)mesh;
+        HIP_CHECK(ngx::launch_mesh_refit(a, s));
+        ctx->top_level_stale = true;
+    });
+}
+
 ngx_status ngx_set_shading(ngx_ctx *ctx, const ngx_shading *shading) {
     return guarded([&] {
         require(ctx != nullptr && shading != nullptr, "context and shading must not be null");
@@ -694,6 +756,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         require(frame >= 0, "frame must be 0 or more, got " + std::to_string(frame));
         require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
         require(cams12 != nullptr, "camera_matrices must not be null");
+        require_state(!ctx->top_level_stale, "a mesh moved (ngx_set_mesh_vertices): set the instance matrices or body poses again first");
         if (memory == NGX_MEM_HOST) require(finite_all(cams12, 12 * (size_t)ctx->views), "a camera has a NaN or infinite value");
         if (outputs & NGX_OUTPUT_RGB)
             require_state(ctx->shading_set, "the RGB picture needs materials and lights: call ngx_set_shading first");
@@ -763,6 +826,7 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
         require(count <= INT32_MAX / 3, "too many rays: " + std::to_string(count));
         require(count == 0 || (origins != nullptr && directions != nullptr), "ray origins and directions must not be null");
         require(max_distance > 0.0f, "max_distance must be positive, got " + std::to_string(max_distance));
+        require_state(!ctx->top_level_stale, "a mesh moved (ngx_set_mesh_vertices): set the instance matrices or body poses again first");
         if (memory == NGX_MEM_HOST) require(finite_all(origins, 3 * (size_t)count), "a ray origin has a NaN or infinite value");
         ctx->set_device();
         if (count == 0) {

@@ -174,12 +174,16 @@ void build_meshes(const MeshInput &in, BuiltMeshes &out) {
         const int node_base = static_cast<int>(out.nodes.size());
         const int tri_base = static_cast<int>(out.triangles.size() / 12);
         out.roots.push_back(node_base);
+        out.node_base.push_back(node_base);
+        out.node_count.push_back(static_cast<int>(tree.size()));
+        out.tri_base.push_back(tri_base);
         out.bounds.push_back(Box{{tree[0].lo[0], tree[0].lo[1], tree[0].lo[2]}, {tree[0].hi[0], tree[0].hi[1], tree[0].hi[2]}});
         for (BlasNode node : tree) {
             node.a += node.count > 0 ? tri_base : node_base;
             out.nodes.push_back(node);
         }
         for (const uint32_t t : order) {
+            for (int k = 0; k < 3; ++k) out.tri_verts.push_back(in.vertex_offsets[m] + static_cast<int>(ix[3 * t + k]));
             const float *a = v + 3 * static_cast<size_t>(ix[3 * t]);
             const float *b = v + 3 * static_cast<size_t>(ix[3 * t + 1]);
             const float *c = v + 3 * static_cast<size_t>(ix[3 * t + 2]);
@@ -206,6 +210,10 @@ void build_meshes(const MeshInput &in, BuiltMeshes &out) {
     for (size_t i = 0; i < out.nodes.size(); ++i)
         if (out.nodes[i].count == 0) wide_index[i] = wide_count++;
     out.wide.resize(static_cast<size_t>(wide_count));
+    out.node_wide = wide_index;
+    out.node_parent.assign(out.nodes.size(), -1);
+    for (size_t i = 0; i < out.nodes.size(); ++i)
+        if (out.nodes[i].count == 0) out.node_parent[out.nodes[i].a] = out.node_parent[out.nodes[i].a + 1] = static_cast<int>(i);
     out.wide_node.assign(static_cast<size_t>(wide_count), -1);
     for (size_t i = 0; i < out.nodes.size(); ++i)
         if (wide_index[i] >= 0) out.wide_node[wide_index[i]] = static_cast<int>(i);

@@ -33,6 +33,11 @@ struct BuiltMeshes {
     std::vector<MeshRoot> mesh_roots; // per mesh, for the tracer
     std::vector<SkipNode> skip;       // per node of `nodes`: the per-lane walk's layout
     std::vector<int> tri_counts;      // per mesh
+    // for refitting a mesh to new vertices on the device (ngx_set_mesh_vertices)
+    std::vector<int> tri_verts;       // per packed triangle: its three vertices, global indices
+    std::vector<int> node_parent;     // per node of `nodes`: its parent, -1 at a mesh's root
+    std::vector<int> node_wide;       // per node of `nodes`: its wide node, -1 for a leaf
+    std::vector<int> node_base, node_count, tri_base;  // per mesh
 };
 
 // Binned SAH build (src/bvh.cpp:46-174). Returns the node count.

@@ -184,6 +184,24 @@ hipError_t launch_ray_order(const float *dirs, int rays_per_world, int worlds, u
 // Builds world `world` (instances first .. first+n-1, n > 1) with grid-wide kernels.
 hipError_t launch_tlas_build_big(const TlasBuildArgs &a, const BigBuildScratch &s, int world, int first, int n, hipStream_t stream);
 // The RGB picture of every view, one 64-pixel tile per wavefront.
+// A mesh's bottom level refitted to new vertices (ngx_refit.hip).
+struct MeshRefitArgs {
+    const float *vertices;    // the mesh's vertices, 3 floats each
+    int vertex_base;          // global index of its first vertex (tri_verts holds global indices)
+    const int *tri_verts;     // per packed triangle: its three vertices
+    int tri_base, tri_count;  // the mesh's packed triangles
+    float *tris;              // packed triangles, 9 floats each
+    SkipNode *skip;           // per node: the box written here first
+    const int *node_parent;   // per node, -1 at the root
+    const int *node_wide;     // per node: its wide node, -1 for a leaf
+    WideNode *wide;
+    int node_base, node_count, root;
+    int *arrivals;            // per node, zeroed by the launch
+    MeshRoot *mesh_root;      // the mesh's
+    float *mesh_bounds;       // the mesh's 6 floats (lo, hi): the top-level build's local box
+};
+hipError_t launch_mesh_refit(const MeshRefitArgs &a, hipStream_t stream);
+
 // The RGB picture; with a.r.tile_counters set and compute_units > 0, from a persistent grid as launch_render.
 hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, int compute_units = 0);
 
