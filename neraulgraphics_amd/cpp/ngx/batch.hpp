@@ -44,7 +44,7 @@ private:
     ngx_status status_;
 };
 
-// Something the reference does that this backend does not (the RGB picture).
+// Something the reference does that this backend does not (textured meshes, environment maps, the path tracer).
 class Unsupported : public std::runtime_error {
 public:
     using std::runtime_error::runtime_error;
@@ -265,6 +265,12 @@ public:
     // world_count x bodies_per_world x 7 floats: x, y, z, qx, qy, qz, qw.
     void set_body_poses(const float *poses, const Memory memory = Memory::Host) {
         check(ngx_set_body_poses(live(), poses, mem(memory), nullptr));
+        check(ngx_synchronize(ctx_));
+    }
+    // No counterpart in the reference: moves mesh `mesh`'s vertices (as many as it was made with, 3 floats each) and refits
+    // its bottom level on the device. Set the instance transforms or body poses again before the next render.
+    void set_mesh_vertices(const int mesh, const float *vertices, const Memory memory = Memory::Host) {
+        check(ngx_set_mesh_vertices(live(), mesh, vertices, mem(memory), nullptr));
         check(ngx_synchronize(ctx_));
     }
     // view_count x 12 camera-to-world floats; results stay in the renderer's buffers until the next render.

@@ -178,6 +178,27 @@ void gpu_checks() {
         check(std::fabs(f.depth[centre] - 8.0f) < 0.02f && std::fabs(f.depth[world1 + centre] - 3.0f) < 0.02f,
               "a body's pose, the instance's offset, rotation and scale compose (front faces at 8 and 3)");
     }
+
+    std::puts("a deforming mesh");
+    {
+        ngx::Mesh m = box(1, 1, 1);
+        ngx::BatchRenderer b(W, H, 1, 1, {}, {}, {}, {m}, {ngx::Instance{}});
+        std::vector<float> poses;
+        add_translation(poses, 0, 0, -8);
+        b.set_instance_transforms(poses.data());
+        std::vector<float> cams;
+        add_camera(cams);
+        std::vector<float> moved = m.vertices;
+        for (size_t i = 2; i < moved.size(); i += 3) moved[i] *= 3.0f;  // the box three times as deep: front face at z = 3
+        b.set_mesh_vertices(0, moved.data());
+        check(throws<ngx::DeviceError>([&] { b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH); }),
+              "a render between a mesh's move and the next pose update is refused");
+        b.set_instance_transforms(poses.data());
+        const size_t centre = (size_t)(H / 2) * W + W / 2;
+        check(std::fabs(b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH).depth[centre] - 5.0f) < 0.02f,
+              "after the refit and a pose update the box's front face is at 5");
+        check(throws<std::invalid_argument>([&] { b.set_mesh_vertices(1, moved.data()); }), "a mesh that is not there is refused");
+    }
 }
 
 }  // namespace

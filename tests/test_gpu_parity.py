@@ -59,6 +59,23 @@ def test_top_level_build_matches_the_reference_algorithm(gpu, oracle, name, worl
     _check_build(r, ref, ref["world_off"])
 
 
+@pytest.mark.parametrize("per_world", [200, 3000])
+def test_the_build_sort_is_a_library_stable_sort(gpu, per_world):
+    """The top-level build's radix sort (per world in LDS, or device-wide above 1,024 instances) gives the
+    permutation a library sort gives: torch.sort(stable=True) on the GPU (rocPRIM underneath) of the same keys."""
+    import torch
+    _say(f"sort cross-check: 6 worlds of up to {per_world} instances")
+    cfg = workloads.scattered(6, per_world, counts=[per_world, per_world // 2, 1, 0, per_world - 1, 17], width=16, height=16)
+    r = workloads.make_renderer(cfg)
+    r.keep_build_intermediates(True)
+    r.set_instance_transforms(cfg["matrices"])
+    b = r.read_build(intermediates=True)
+    keys = torch.from_numpy(b["keys"].view(np.int64)).cuda()
+    _, order = torch.sort(keys, stable=True)
+    assert np.array_equal(order.cpu().numpy().astype(np.int32), b["sorted_ids"])
+    assert np.array_equal(np.sort(b["keys"]), b["sorted_keys"])
+
+
 def test_large_worlds_build_and_render_as_the_reference_algorithm(gpu, oracle):
     """Worlds above 1024 instances take the device-wide build; 1024 and below the workgroup build. Both must give
     the reference algorithm's keys, permutation and trees, and the same images."""
