@@ -307,10 +307,12 @@ def main():
         cpu = cpu_baseline(args.cpu_seconds)
     kernel = "render_persistent_kernel" if c2["trace_config"].get("persistent") else "render_kernel"
     prof = os.path.join(ROOT, "profiles", f"{kernel}_body_worlds_traffic.json")
-    traffic = None
+    traffic, valu_busy = None, None
     if os.path.exists(prof):
         with open(prof) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            summary = json.load(f)
+        traffic = summary.get("hbm_bytes_per_launch")
+        valu_busy = summary.get("kernels", {}).get("render_kernel_sq", {}).get("valu_busy")
     line = {
         "metric": METRIC,
         "value": c2["views"] / sec,
@@ -333,7 +335,9 @@ def main():
         "render_ms": c2["render_ms"],
         "hit_fraction": c2["hit_frac"],
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": rb},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": rb,
+                     # the bound that holds: the fraction of cycles every SIMD issues a vector instruction (rocprof, same kernel)
+                     "valu_busy": valu_busy},
         "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
     }
     if c3:
