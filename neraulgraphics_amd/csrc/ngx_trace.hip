@@ -213,10 +213,14 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 // node's escape is the range's end) gives the far child's subtree to a shared stack in LDS, while there is
 // room, and keeps the near one; idle lanes take the newest given subtrees, with their rays' mesh-space data
 // read from the rays' own lanes. So the few rays that enter an instance keep the whole wave busy. Leaves are
-// queued, and their triangles are tested in batches of up to 64 (ray, triangle) pairs once 32 leaves or 64
+// queued, and their triangles are tested in batches of up to 64 (ray, triangle) pairs once 64 leaves or 64
 // pairs wait, and at the end; after each batch every walking lane re-reads its ray's closest distance. Every
 // node a ray enters is walked by exactly one lane, so each ray's closest hit is its own.
 constexpr int SHARE_CAP = 128, LEAF_RING = 128;
+// Queued leaves that trigger a batch of triangle tests (at most 64 pairs a batch; fewer leaves wait for more).
+#ifndef NGX_LEAF_BATCH
+#define NGX_LEAF_BATCH 64
+#endif
 // In the hybrid walk, a bottom-level subtree entered by at most this many of the packet's rays is walked by the
 // shared per-lane walk instead of the packet (0: never).
 #ifndef NGX_HANDOFF_LANES
@@ -349,7 +353,7 @@ __device__ void walk_shared(const TraceScene &S, PairLds *L, const int g, int to
             queued += popc64(lm);
         }
         if (in_mask(busy) && cur == end) ray = -1;
-        if (queued >= 32) {
+        if (queued >= NGX_LEAF_BATCH) {
             test_queued_leaves<false>(S, L, head, queued, g, lo, ld, tmax, stats);
             wlim = __shfl(tmax, ray >= 0 ? ray : lane, 64);
         }
@@ -871,9 +875,12 @@ __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID 
 // Persistent: a fixed grid whose wavefronts take tiles until none are left. The tiles are cut into eight
 // contiguous ranges, one per XCD (the dispatcher deals workgroups round-robin over the XCDs, so workgroup b
 // runs on XCD b mod 8); each range has its own counter, so the worlds of a range stay in one XCD's L2. A
-// wavefront takes PERSISTENT_CHUNK consecutive tiles per counter increment, so cheap tiles do not queue on
+// wavefront takes PERSISTENT_CHUNK (2) consecutive tiles per counter increment, so cheap tiles do not queue on
 // the counter. Every wavefront leaves the loop once its range's counter passes the range's end.
-constexpr int PERSISTENT_CHUNK = 8;
+#ifndef NGX_PERSISTENT_CHUNK
+#define NGX_PERSISTENT_CHUNK 2
+#endif
+constexpr int PERSISTENT_CHUNK = NGX_PERSISTENT_CHUNK;
 
 template <int WALK>
 __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? NGX_HYBRID_MIN_BLOCKS : 1) void render_persistent_kernel(const RenderArgs A) {
