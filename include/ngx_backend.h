@@ -27,6 +27,8 @@
  *  - ngx_pose_backward_depth / ngx_pose_backward_range: gradients of depth and range with respect to
  *    body poses;
  *  - ngx_set_mesh_vertices: deforming meshes, refitted on the device;
+ *  - ngx_bind_geoms / ngx_set_geom_frames: the MuJoCo Warp adapter's compose step (a Warp kernel and a
+ *    separate instance update in the reference) fused into the top-level build;
  *  - ngx_debug_*: the top-level build's intermediates, for checking it against the reference algorithm.
  */
 #ifndef NGX_BACKEND_H
@@ -160,6 +162,18 @@ ngx_status ngx_bind_bodies(ngx_ctx *ctx, int32_t bodies_per_world, const int32_t
 /* Backend::set_body_poses (src/backend.h:65); checks of src/batch.cpp:131-144.
  * worlds x bodies_per_world x 7 floats. Composed in FP64 as src/instances.cpp:137-164 does. */
 ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memory, void *stream);
+
+/* Simulator geoms, the MuJoCo Warp flow (python/neraulgraphics/mujoco.py:177-302): binds each template
+ * instance to geom geom_of_instance[i] (0..geoms_per_world-1) of its world, drawn with scale3[i] along the
+ * mesh's own axes. Host arrays; checked before anything changes. Replaces the adapter's geom list and scales
+ * (mujoco.py:239, :257); independent of the body binding. */
+ngx_status ngx_bind_geoms(ngx_ctx *ctx, int32_t geoms_per_world, const int32_t *geom_of_instance, const float *scale3);
+
+/* The adapter's per-step update (mujoco.py:276-302: a Warp kernel composing R diag(s) | p, then
+ * set_instance_transforms) as one build: each instance's matrix is composed in FP32 from its geom's world
+ * position (xpos3: worlds x geoms_per_world x 3 floats, MjData.geom_xpos) and row-major rotation (xmat9:
+ * worlds x geoms_per_world x 9 floats, MjData.geom_xmat), both in `memory`, then the top level is rebuilt. */
+ngx_status ngx_set_geom_frames(ngx_ctx *ctx, const float *xpos3, const float *xmat9, ngx_memory memory, void *stream);
 
 /* No counterpart in the reference (its bottom levels are built once): moves the vertices of mesh `mesh` (as many
  * as it was created with, 3 floats each) and refits its bottom level on the device, keeping the tree. The top

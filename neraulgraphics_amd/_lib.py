@@ -83,6 +83,8 @@ _SIGNATURES = {
     "ngx_set_instance_matrices": (ctypes.c_int32, [_VP, _VP, ctypes.c_int32, _VP]),
     "ngx_bind_bodies": (ctypes.c_int32, [_VP, ctypes.c_int32, _VP, _VP, _VP]),
     "ngx_set_body_poses": (ctypes.c_int32, [_VP, _VP, ctypes.c_int32, _VP]),
+    "ngx_bind_geoms": (ctypes.c_int32, [_VP, ctypes.c_int32, _VP, _VP]),
+    "ngx_set_geom_frames": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
     "ngx_set_mesh_vertices": (ctypes.c_int32, [_VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP]),
     "ngx_render_batch": (ctypes.c_int32, [_VP, _VP, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_uint32,
                                           ctypes.c_int32, _VP, ctypes.POINTER(BatchFrame)]),

@@ -135,6 +135,7 @@ class BatchRenderer:
         self._mesh_vertex_counts = [len(m.vertices) for m in meshes]
         self._mesh_signature = ",".join(str(len(m.indices) // 3 if hasattr(m, "indices") else 0) for m in meshes)
         self._bodies = None
+        self._geoms = None
         self._device_id = int(device_id)
         self._keep = []
         self._shading = shading_arrays(instances, spheres, floor, list(lights))
@@ -314,6 +315,35 @@ class BatchRenderer:
         p, mem, keep = _as_input(body_poses, "body poses (world_count, bodies_per_world, 7)", n)
         _lib.check(self._lib.ngx_set_body_poses(self._h, p, mem, self._stream_for(mem)))
         self._hold(keep, mem)
+
+    @_serialized
+    def bind_geoms(self, geom_of_instance, scale, geoms_per_world):
+        """Binds each template instance to a simulator geom of its world (the MuJoCo Warp flow,
+        python/neraulgraphics/mujoco.py:177-302): instance i is drawn at geom geom_of_instance[i]'s world frame,
+        scaled by scale[i] along the mesh's own axes. Independent of bind_bodies."""
+        goi = np.ascontiguousarray(np.asarray(geom_of_instance, dtype=np.int32).reshape(-1))
+        sc = np.ascontiguousarray(np.asarray(scale, dtype=np.float32).reshape(-1))
+        n = self._per_world
+        if goi.size != n or sc.size != 3 * n:
+            raise ValueError(f"a geom binding needs one geom index and 3 scale floats for each of the {n} instances of a world")
+        _lib.check(self._lib.ngx_bind_geoms(self._h, int(geoms_per_world), goi.ctypes.data_as(ctypes.c_void_p),
+                                            sc.ctypes.data_as(ctypes.c_void_p)))
+        self._geoms = int(geoms_per_world)
+
+    @_serialized
+    def set_geom_frames(self, geom_xpos, geom_xmat):
+        """Places every instance from its geom's world frame and rebuilds the top level, one kernel: geom_xpos
+        (world_count, geoms_per_world, 3) and geom_xmat (world_count, geoms_per_world, 3, 3) or (..., 9), float32,
+        row-major, as MuJoCo Warp's data.geom_xpos and data.geom_xmat hold them. Both NumPy or both GPU arrays."""
+        if self._geoms is None:
+            raise ValueError("set_geom_frames needs bind_geoms to be called first")
+        n = self._worlds * self._geoms
+        p, mem, keep = _as_input(geom_xpos, "geom positions (world_count, geoms_per_world, 3)", 3 * n)
+        r, mem_r, keep_r = _as_input(geom_xmat, "geom rotations (world_count, geoms_per_world, 3, 3)", 9 * n)
+        if mem != mem_r:
+            raise ValueError("geom positions and rotations must both be on the host or both on the GPU")
+        _lib.check(self._lib.ngx_set_geom_frames(self._h, p, r, mem, self._stream_for(mem)))
+        self._hold((keep, keep_r), mem)
 
     @_serialized
     def set_mesh_vertices(self, mesh, vertices):

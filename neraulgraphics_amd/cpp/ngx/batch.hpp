@@ -267,6 +267,20 @@ public:
         check(ngx_set_body_poses(live(), poses, mem(memory), nullptr));
         check(ngx_synchronize(ctx_));
     }
+    // Simulator geoms (the MuJoCo Warp flow): instance i of every world is drawn at geom geom_of_instance[i]'s world
+    // frame, scaled by scale[3 i .. 3 i + 2] along the mesh's own axes. Independent of bind_bodies.
+    void bind_geoms(const int geoms_per_world, const std::vector<int32_t> &geom_of_instance, const std::vector<float> &scale) {
+        const size_t n = (size_t)instances_per_world();
+        if (geom_of_instance.size() != n || scale.size() != 3 * n)
+            throw std::invalid_argument("a geom binding needs one geom index and 3 scale floats for each of the " +
+                                        std::to_string(n) + " instances of a world");
+        check(ngx_bind_geoms(live(), geoms_per_world, geom_of_instance.data(), scale.data()));
+    }
+    // world_count x geoms_per_world x 3 positions and x 9 row-major rotations (MjData.geom_xpos, geom_xmat in FP32).
+    void set_geom_frames(const float *xpos, const float *xmat, const Memory memory = Memory::Host) {
+        check(ngx_set_geom_frames(live(), xpos, xmat, mem(memory), nullptr));
+        check(ngx_synchronize(ctx_));
+    }
     // No counterpart in the reference: moves mesh `mesh`'s vertices (as many as it was made with, 3 floats each) and refits
     // its bottom level on the device. Set the instance transforms or body poses again before the next render.
     void set_mesh_vertices(const int mesh, const float *vertices, const Memory memory = Memory::Host) {

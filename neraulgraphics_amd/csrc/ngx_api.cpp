@@ -139,6 +139,12 @@ struct ngx_ctx {
     float *d_body_stage = nullptr;
     const float *last_body_poses = nullptr;
 
+    // simulator geoms: each template instance drawn at a geom's world frame, scaled
+    bool geoms_bound = false;
+    int geoms = 0;
+    int *d_geom_of_instance = nullptr;
+    float *d_geom_scale = nullptr, *d_geom_stage = nullptr;  // stage: [W][G][3] positions, then [W][G][9] rotations
+
     // shading of the RGB picture
     bool shading_set = false;
     ngx::ShadeScene shade{};
@@ -176,10 +182,10 @@ struct ngx_ctx {
     ~ngx_ctx() {
         int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_blas_wide_node, d_wide_parent,
                        d_tri_verts, d_node_parent, d_node_wide, d_node_arrivals,
-                       d_tile_counters, d_mesh_skip_root, d_mesh_tris};
+                       d_geom_of_instance, d_tile_counters, d_mesh_skip_root, d_mesh_tris};
         for (int *p : ints) if (p) (void)hipFree(p);
         float *floats[] = {d_vertices, d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
-                           d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
+                           d_geom_scale, d_geom_stage, d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
                            d_dbg_world_bounds};
         for (float *p : floats) if (p) (void)hipFree(p);
         void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
@@ -229,6 +235,9 @@ struct ngx_ctx {
         a.body_of_instance = d_body_of_instance;
         a.local_pose = d_local_pose;
         a.scale = d_scale;
+        a.geoms_per_world = geoms;
+        a.geom_of_instance = d_geom_of_instance;
+        a.geom_scale = d_geom_scale;
         a.matrices_out = d_matrices;
         a.tmpl_mesh = d_tmpl_mesh;
         a.mesh_bounds = d_mesh_bounds;
@@ -288,7 +297,7 @@ struct ngx_ctx {
     void build_from_matrices(const float *matrices_wp12, hipStream_t stream) {
         if (total == 0) return;
         ngx::TlasBuildArgs a = build_args();
-        a.from_bodies = 0;
+        a.source = ngx::SOURCE_MATRICES;
         a.matrices_in = matrices_wp12;
         build(a, stream);
         last_body_poses = nullptr;
@@ -298,10 +307,21 @@ struct ngx_ctx {
     void build_from_bodies(const float *poses, hipStream_t stream) {
         if (total == 0) return;
         ngx::TlasBuildArgs a = build_args();
-        a.from_bodies = 1;
+        a.source = ngx::SOURCE_BODIES;
         a.body_poses = poses;
         build(a, stream);
         last_body_poses = poses;
+        top_level_stale = false;
+    }
+
+    void build_from_geoms(const float *pos3, const float *mat9, hipStream_t stream) {
+        if (total == 0) return;
+        ngx::TlasBuildArgs a = build_args();
+        a.source = ngx::SOURCE_GEOMS;
+        a.geom_pos = pos3;
+        a.geom_mat = mat9;
+        build(a, stream);
+        last_body_poses = nullptr;
         top_level_stale = false;
     }
 };
@@ -608,6 +628,55 @@ ngx_status ngx_set_body_poses(ngx_ctx *ctx, const float *poses7, ngx_memory memo
         }
         ctx->last_stream = s;
         ctx->build_from_bodies(src, s);
+    });
+}
+
+ngx_status ngx_bind_geoms(ngx_ctx *ctx, int32_t geoms_per_world, const int32_t *geom_of_instance, const float *scale3) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        const int n = ctx->per_world;
+        require(geoms_per_world >= 0, "geoms_per_world must be 0 or more");
+        require(n == 0 || (geom_of_instance && scale3),
+                "a geom binding needs one geom index and 3 scale floats for each of the " + std::to_string(n) +
+                    " instances of a world");
+        for (int i = 0; i < n; ++i) {
+            const int g = geom_of_instance[i];
+            require(g >= 0 && g < geoms_per_world, "instance " + std::to_string(i) + " is drawn at geom " + std::to_string(g) +
+                                                       ", but there are " + std::to_string(geoms_per_world) + " geoms per world");
+            require(finite_all(scale3 + 3 * i, 3), "instance " + std::to_string(i) + " has a scale with a NaN or infinite value");
+        }
+        ctx->set_device();
+        HIP_CHECK(hipDeviceSynchronize());
+        upload(ctx->d_geom_of_instance, geom_of_instance, (size_t)n);
+        upload(ctx->d_geom_scale, scale3, 3 * (size_t)n);
+        device_alloc(ctx->d_geom_stage, 12 * (size_t)ctx->worlds * geoms_per_world);
+        ctx->geoms = geoms_per_world;
+        ctx->geoms_bound = true;
+    });
+}
+
+ngx_status ngx_set_geom_frames(ngx_ctx *ctx, const float *xpos3, const float *xmat9, ngx_memory memory, void *stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(ctx->geoms_bound, "set_geom_frames needs bind_geoms to be called first");
+        const size_t frames = (size_t)ctx->worlds * ctx->geoms;
+        require(frames == 0 || (xpos3 != nullptr && xmat9 != nullptr), "geom positions and rotations must not be null");
+        if (memory == NGX_MEM_HOST)
+            for (size_t g = 0; g < frames; ++g)
+                require(finite_all(xpos3 + 3 * g, 3) && finite_all(xmat9 + 9 * g, 9),
+                        "geom frame " + std::to_string(g) + " has a NaN or infinite value");
+        if (ctx->per_world == 0) return;
+        ctx->set_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const float *pos = xpos3, *mat = xmat9;
+        if (memory == NGX_MEM_HOST && frames > 0) {
+            HIP_CHECK(hipMemcpyAsync(ctx->d_geom_stage, xpos3, 3 * sizeof(float) * frames, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(ctx->d_geom_stage + 3 * frames, xmat9, 9 * sizeof(float) * frames, hipMemcpyHostToDevice, s));
+            pos = ctx->d_geom_stage;
+            mat = ctx->d_geom_stage + 3 * frames;
+        }
+        ctx->last_stream = s;
+        ctx->build_from_geoms(pos, mat, s);
     });
 }
 

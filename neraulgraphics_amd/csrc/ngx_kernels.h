@@ -9,13 +9,17 @@ namespace ngx {
 
 // The instances of a batch are world-major: world w holds template entries 0 .. count(w)-1 at global
 // positions world_off[w] .. world_off[w+1]-1. The reference has count(w) equal for every world.
+// Where the build takes instance placements from: given matrices, body poses composed with a body binding,
+// or simulator geom frames (position and rotation matrix per geom) scaled by a geom binding.
+enum { SOURCE_MATRICES = 0, SOURCE_BODIES = 1, SOURCE_GEOMS = 2 };
+
 struct TlasBuildArgs {
     int worlds;
     int total;                 // N = world_off[worlds]
     const int *world_off;      // [worlds + 1]
     const uint8_t *root_at_end; // [worlds]: the world's subtree root is node world_off[w+1]-1, not world_off[w]
 
-    int from_bodies;           // 0: matrices_in; 1: body poses composed with the binding
+    int source;                // SOURCE_MATRICES, SOURCE_BODIES or SOURCE_GEOMS
     const float *matrices_in;  // world w's instance i at 12 * (w * matrices_stride + i)
     int matrices_stride;       // instances per world in matrices_in (the template size)
     const float *body_poses;   // [worlds][bodies_per_world][7]
@@ -23,6 +27,11 @@ struct TlasBuildArgs {
     const int *body_of_instance; // [P]
     const float *local_pose;   // [P][7]
     const float *scale;        // [P][3]
+    const float *geom_pos;     // [worlds][geoms_per_world][3]: each geom's world position
+    const float *geom_mat;     // [worlds][geoms_per_world][9]: each geom's world rotation, row-major
+    int geoms_per_world;
+    const int *geom_of_instance; // [P]
+    const float *geom_scale;   // [P][3]
     float *matrices_out;       // [N][12]: every instance's matrix as placed
 
     const int *tmpl_mesh;      // [P]

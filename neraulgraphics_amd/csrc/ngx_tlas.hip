@@ -125,10 +125,19 @@ __device__ void block_exclusive_scan(int *data, const int count, int *wave_tot) 
 // inverse the tracer reads, and its world box (the 8 corners of the mesh box, cuda_tlas.cuh:69-96).
 __device__ void place_instance(const TlasBuildArgs &A, const int w, const int i, const int g, float lo[3], float hi[3]) {
     float m[12];
-    if (A.from_bodies) {
+    if (A.source == SOURCE_BODIES) {
         const int b = A.body_of_instance[i];
         const float *body = b < 0 ? nullptr : A.body_poses + 7 * ((size_t)w * A.bodies_per_world + b);
         compose_body_pose(body, A.local_pose + 7 * i, A.scale + 3 * i, m);
+    } else if (A.source == SOURCE_GEOMS) {
+        // R diag(s) | p from FP32 inputs, one FP32 product an element, as the reference's MuJoCo Warp compose
+        // kernel does (python/neraulgraphics/mujoco.py:276-287).
+        const size_t gw = (size_t)w * A.geoms_per_world + A.geom_of_instance[i];
+        const float *p = A.geom_pos + 3 * gw, *r = A.geom_mat + 9 * gw, *sc = A.geom_scale + 3 * i;
+        for (int row = 0; row < 3; ++row) {
+            for (int col = 0; col < 3; ++col) m[4 * row + col] = r[3 * row + col] * sc[col];
+            m[4 * row + 3] = p[row];
+        }
     } else {
         const float4 *mi = reinterpret_cast<const float4 *>(A.matrices_in + 12 * ((size_t)w * A.matrices_stride + i));
         const float4 r0 = mi[0], r1 = mi[1], r2 = mi[2];

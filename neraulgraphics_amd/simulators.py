@@ -15,10 +15,10 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .batch import BatchRenderer
-from .geometry import Instance, Material, Mesh, make_box, make_icosphere
-from .shapes import capsule, cone, cylinder, plane
+from .geometry import OBJECT_INSTANCE, Instance, Light, Material, Mesh, make_box, make_icosphere
+from .shapes import capsule, cone, cylinder, look_at, plane
 
-__all__ = ["Shape", "SimScene", "SimBatchRenderer", "build_scene"]
+__all__ = ["Shape", "SimScene", "SimBatchRenderer", "MJWarpBatchRenderer", "build_scene", "mujoco_shapes", "scene_bounds"]
 
 KINDS = ("sphere", "box", "ellipsoid", "cylinder", "cone", "capsule", "plane", "mesh")
 UNBOUNDED_PLANE = 1000.0  # the size the Newton viewer gives a plane of size 0
@@ -127,3 +127,166 @@ class SimBatchRenderer:
 
     def cast_rays(self, origins, directions, max_distance=float("inf"), **kw):
         return self.renderer.cast_rays(origins, directions, max_distance, **kw)
+
+
+# MuJoCo's geom types (mjtGeom), the codes MjModel.geom_type holds.
+MJ_PLANE, MJ_HFIELD, MJ_SPHERE, MJ_CAPSULE, MJ_ELLIPSOID, MJ_CYLINDER, MJ_BOX, MJ_MESH = range(8)
+_MJ_DEFAULT_RGBA = np.array([0.5, 0.5, 0.5, 1.0], np.float32)
+
+
+def mujoco_shapes(model, geom_groups=(0, 1, 2)):
+    """The drawn geoms of a MuJoCo model as Shapes, and their geom indices. Reads only the model's arrays (ngeom,
+    geom_type, geom_size, geom_rgba, geom_group, geom_matid, mat_rgba, geom_dataid and the mesh_* arrays), so an
+    MjModel or anything holding the same fields will do. The selection and sizes of the reference's MuJoCo adapter
+    (python/neraulgraphics/mujoco.py:68-88, :104-136): geoms of the groups asked for with a visible colour; planes,
+    spheres, capsules, ellipsoids, cylinders, boxes and meshes (heightfields and SDFs are not drawn); a geom with a
+    material and the default colour takes its material's colour. Planes have their colour, not the reference's
+    checkerboard texture (textures are not drawn by this backend)."""
+    shapes, geoms, meshes = [], [], {}
+    for g in range(int(model.ngeom)):
+        kind = int(model.geom_type[g])
+        rgba = np.asarray(model.geom_rgba[g], np.float32)
+        if int(model.geom_group[g]) not in geom_groups or rgba[3] <= 0.0:
+            continue
+        size = [float(v) for v in model.geom_size[g]]
+        if kind == MJ_SPHERE:
+            shape = Shape("sphere", (size[0],))
+        elif kind == MJ_ELLIPSOID:
+            shape = Shape("ellipsoid", tuple(size[:3]))
+        elif kind == MJ_BOX:
+            shape = Shape("box", tuple(size[:3]))
+        elif kind == MJ_CYLINDER:
+            shape = Shape("cylinder", (size[0], size[1]))
+        elif kind == MJ_CAPSULE:
+            shape = Shape("capsule", (size[0], size[1]))
+        elif kind == MJ_PLANE:  # MuJoCo sizes a plane by its half-extents; 0 extends it forever
+            shape = Shape("plane", (2.0 * size[0] if size[0] > 0.0 else 0.0, 2.0 * size[1] if size[1] > 0.0 else 0.0))
+        elif kind == MJ_MESH:
+            m = int(model.geom_dataid[g])
+            if m not in meshes:  # one Mesh per MuJoCo mesh, shared by the geoms that draw it
+                v0, nv = int(model.mesh_vertadr[m]), int(model.mesh_vertnum[m])
+                f0, nf = int(model.mesh_faceadr[m]), int(model.mesh_facenum[m])
+                meshes[m] = Mesh(np.asarray(model.mesh_vert[v0:v0 + nv], np.float32),
+                                 np.asarray(model.mesh_face[f0:f0 + nf], np.uint32))
+            shape = Shape("mesh", (1.0, 1.0, 1.0), mesh=meshes[m])
+        else:
+            continue
+        matid = int(model.geom_matid[g])
+        if matid >= 0 and np.allclose(rgba, _MJ_DEFAULT_RGBA):
+            rgba = np.asarray(model.mat_rgba[matid], np.float32)
+        shape.color = tuple(float(c) for c in rgba[:3])
+        shapes.append(shape)
+        geoms.append(g)
+    return shapes, np.asarray(geoms, np.int32)
+
+
+def _reach(shape: Shape, scale) -> float:
+    """Radius about the shape's origin that holds it (planes: 0, they are the world)."""
+    if shape.kind == "plane":
+        return 0.0
+    if shape.kind == "capsule":
+        return float(shape.size[0] + shape.size[1])
+    if shape.kind == "mesh":
+        return float(np.linalg.norm(shape.mesh.vertices, axis=1).max() * np.abs(scale).max())
+    unit = {"sphere": 1.0, "ellipsoid": 1.0, "box": np.sqrt(3.0), "cylinder": np.sqrt(2.0), "cone": np.sqrt(2.0)}[shape.kind]
+    return float(unit * np.abs(scale).max())
+
+
+def scene_bounds(positions, reach):
+    """Centre and half-size of spheres of radius `reach` at `positions` (none: a unit scene at the origin), as the
+    reference's adapters frame a scene (python/neraulgraphics/_adapter.py:71-76)."""
+    positions, reach = np.asarray(positions, np.float64).reshape(-1, 3), np.asarray(reach, np.float64).reshape(-1)
+    if len(positions) == 0:
+        return np.zeros(3), 1.0
+    lo, hi = (positions - reach[:, None]).min(axis=0), (positions + reach[:, None]).max(axis=0)
+    return (lo + hi) / 2.0, float(np.linalg.norm(hi - lo)) / 2.0
+
+
+class MJWarpBatchRenderer:
+    """Draws MuJoCo Warp worlds, thousands at once, each with its own cameras: the reference's MJWarpBatchRenderer
+    (python/neraulgraphics/mujoco.py:177-323). The model's geoms become instances once; each render reads the
+    data's geom_xpos (world_count, ngeom, 3) and geom_xmat (world_count, ngeom, 3, 3) as float32 GPU tensors (or
+    anything torch.from_dlpack takes, or NumPy arrays) and places every geom of every world while rebuilding the top
+    levels, one kernel (ngx_set_geom_frames); the reference composes the matrices in a Warp kernel first.
+
+    The reference frames its default camera and lights from the model after mj_forward; here `data` at construction
+    (anything with the model's geom_xpos, for one world or all) plays that part, and without it the frame is a
+    unit scene at the origin."""
+
+    def __init__(self, model, world_count, width=128, height=128, cameras_per_world=1, *, geom_groups=(0, 1, 2),
+                 lights=None, fov=1.0, ambient=0.2, data=None, device_id=None):
+        shapes, self._geoms = mujoco_shapes(model, geom_groups)
+        if not shapes:
+            raise ValueError("the model has no visible geoms to draw")
+        self.scene = build_scene(shapes)
+        self._worlds, self._cameras_per_world = int(world_count), int(cameras_per_world)
+        self.width, self.height, self.fov = int(width), int(height), float(fov)
+        self.up = np.array([0.0, 0.0, 1.0])
+        reach = np.array([_reach(sh, sc) for sh, sc in zip(shapes, self.scene.scale)])
+        if data is not None:
+            xpos = np.asarray(_to_numpy(data.geom_xpos), np.float64).reshape(-1, int(model.ngeom), 3)[0][self._geoms]
+            solid = reach > 0
+            self.center, extent = scene_bounds(xpos[solid], reach[solid])
+        else:
+            self.center, extent = np.zeros(3), 1.0
+        self.extent = max(extent, 1e-3)
+        if lights is None:
+            lights = self.default_lights()
+        self.renderer = BatchRenderer(self.width, self.height, self._worlds, self._cameras_per_world, lights=list(lights),
+                                      meshes=self.scene.meshes, instances=self.scene.instances, device_id=device_id)
+        self.renderer.set_ambient(ambient)
+        self.renderer.bind_geoms(self._geoms, self.scene.scale, int(model.ngeom))
+
+    @property
+    def view_count(self) -> int:
+        return self._worlds * self._cameras_per_world
+
+    def default_lights(self):
+        """A key light high and to one side and a dimmer fill from the other, scaled to the scene, as the
+        reference's adapters place them (python/neraulgraphics/_adapter.py:128-134), for Z up."""
+        side, across = np.array([0.0, 1.0, 0.0]), np.array([-1.0, 0.0, 0.0])
+        key = self.center + self.up * 3.0 * self.extent + side * 1.5 * self.extent + across * 1.0 * self.extent
+        fill = self.center + self.up * 2.0 * self.extent - side * 2.0 * self.extent - across * 1.5 * self.extent
+        return [Light(tuple(key), intensity=1.0, radius=0.0), Light(tuple(fill), intensity=0.4, radius=0.0)]
+
+    def default_cameras(self):
+        """One camera per view, framing the scene from the side, slightly above: (views, 3, 4)."""
+        eye = self.center - np.array([0.0, 1.0, 0.0]) * 2.6 * self.extent + self.up * 0.7 * self.extent
+        return np.tile(look_at(eye, self.center, self.up), (self.view_count, 1, 1))
+
+    def update(self, data) -> None:
+        """Places every geom of every world from data.geom_xpos and data.geom_xmat (step the simulation first)."""
+        self.renderer.set_geom_frames(_as_frames(data.geom_xpos), _as_frames(data.geom_xmat))
+
+    def render(self, data, cameras=None, *, fov=None, rgb=True, depth=False, normals=False, object_ids=False, zero_copy=False,
+               frame=0, samples_per_pixel=1):
+        """update(data), then the images asked for as a dict (see BatchRenderer.render). GPU cameras need zero_copy."""
+        self.update(data)
+        return self.renderer.render(self.default_cameras() if cameras is None else cameras, self.fov if fov is None else fov,
+                                    samples_per_pixel, rgb=rgb, depth=depth, normals=normals, object_ids=object_ids,
+                                    frame=frame, zero_copy=zero_copy)
+
+    @property
+    def object_id_of_geom(self) -> dict:
+        """Which object ID each drawn MuJoCo geom index has in an object_ids image or a ray query."""
+        return {int(g): OBJECT_INSTANCE + i for i, g in enumerate(self._geoms)}
+
+    def cast_rays(self, origins, directions, max_distance=float("inf"), **kw):
+        """Ray queries in every world: (world_count, rays_per_world, 3) arrays (see BatchRenderer.cast_rays)."""
+        return self.renderer.cast_rays(origins, directions, max_distance, **kw)
+
+
+def _as_frames(a):
+    """A GPU tensor, a NumPy array, or anything torch.from_dlpack takes (a Warp array), as the renderer's input."""
+    if isinstance(a, np.ndarray) or not hasattr(a, "__dlpack__"):
+        return np.asarray(a, np.float32)
+    import torch
+    return a if isinstance(a, torch.Tensor) else torch.from_dlpack(a)
+
+
+def _to_numpy(a):
+    if isinstance(a, np.ndarray) or not hasattr(a, "__dlpack__"):
+        return np.asarray(a)
+    import torch
+    t = a if isinstance(a, torch.Tensor) else torch.from_dlpack(a)
+    return t.detach().cpu().numpy()

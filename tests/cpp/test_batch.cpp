@@ -179,6 +179,24 @@ void gpu_checks() {
               "a body's pose, the instance's offset, rotation and scale compose (front faces at 8 and 3)");
     }
 
+    std::puts("poses from geom frames");
+    {
+        // one geom per world holding the box scaled 2 along z and turned 90 degrees about y (row-major rotation)
+        ngx::BatchRenderer b(W, H, 2, 1, {}, {}, {}, {box(1, 1, 1)}, {ngx::Instance{}});
+        b.bind_geoms(1, {0}, {1, 1, 2});
+        const std::vector<float> xpos = {0, 0, -9, 0, 0, -4};
+        const std::vector<float> xmat = {0, 0, 1, 0, 1, 0, -1, 0, 0, 0, 0, 1, 0, 1, 0, -1, 0, 0};
+        b.set_geom_frames(xpos.data(), xmat.data());
+        std::vector<float> cams;
+        add_camera(cams);
+        add_camera(cams);
+        const ngx::HostBatchFrame f = b.render_host(cams.data(), FOV, 1, ngx::OUTPUT_DEPTH);
+        const size_t centre = (size_t)(H / 2) * W + W / 2, world1 = (size_t)W * H;
+        check(std::fabs(f.depth[centre] - 8.0f) < 0.02f && std::fabs(f.depth[world1 + centre] - 3.0f) < 0.02f,
+              "a geom's frame and the binding's scale compose (front faces at 8 and 3)");
+        check(throws<std::invalid_argument>([&] { b.bind_geoms(1, {1}, {1, 1, 1}); }), "a geom index past the last geom is refused");
+    }
+
     std::puts("a deforming mesh");
     {
         ngx::Mesh m = box(1, 1, 1);

@@ -1,13 +1,17 @@
 """The simulator front end (neraulgraphics_amd/simulators.py): shapes to meshes, instances and a body binding, as the
-reference's adapters build them (python/neraulgraphics/newton.py:48-89), and the per-step flow of body poses.
-Newton, MuJoCo and Warp are not installed, so the simulators themselves are not exercised; the pose math is checked
-against the oracle's FP64 body composition (the reference host's, src/instances.cpp:126-164)."""
+reference's adapters build them (python/neraulgraphics/newton.py:48-89), and the per-step flow of body poses; MuJoCo
+models to shapes and geom frames to instance matrices (python/neraulgraphics/mujoco.py:68-136, :177-302).
+Newton, MuJoCo and Warp are not installed, so the simulators themselves are not exercised: a MuJoCo model is stood for
+by an object holding the MjModel fields the adapter reads. The pose math is checked against the oracle's FP64 body
+composition (the reference host's, src/instances.cpp:126-164)."""
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 
 import neraulgraphics_amd as ng
 from neraulgraphics_amd import shapes
-from neraulgraphics_amd.simulators import Shape, SimBatchRenderer, build_scene
+from neraulgraphics_amd.simulators import MJWarpBatchRenderer, Shape, SimBatchRenderer, build_scene, mujoco_shapes, scene_bounds
 from oracle.oracle import forward
 
 
@@ -89,3 +93,117 @@ def test_a_simulated_batch_renders_as_the_oracle(gpu, oracle):
     ref = forward(cfg, oracle)
     rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "simulated batch")
     assert rep["hits"] > 0.3 * rep["pixels"]
+
+
+def _mujoco_model():
+    """The MjModel fields the MuJoCo adapter reads, for nine geoms: a plane, a sphere taking its material's colour,
+    a capsule, a box in group 3 (hidden), an invisible ellipsoid, a cylinder, a mesh, a heightfield (not drawn) and the
+    same mesh again."""
+    ico = ng.make_icosphere(0.25, 1)
+    grey = [0.5, 0.5, 0.5, 1.0]
+    return SimpleNamespace(
+        ngeom=9,
+        geom_type=np.array([0, 2, 3, 6, 4, 5, 7, 1, 7]),
+        geom_size=np.array([[2.0, 3.0, 0.1], [0.3, 0, 0], [0.1, 0.25, 0], [0.2, 0.3, 0.1], [0.1, 0.2, 0.3], [0.15, 0.2, 0],
+                            [0, 0, 0], [1, 1, 1], [0, 0, 0]]),
+        geom_rgba=np.array([[0.3, 0.3, 0.3, 1], grey, [0.1, 0.8, 0.1, 1], grey, [0.2, 0.2, 0.9, 0.0], [0.7, 0.7, 0.1, 1],
+                            [0.2, 0.4, 0.9, 1], grey, [0.9, 0.5, 0.1, 1]], np.float32),
+        geom_group=np.array([0, 0, 1, 3, 0, 2, 0, 0, 2]),
+        geom_matid=np.array([-1, 0, -1, -1, -1, -1, -1, -1, -1]),
+        mat_rgba=np.array([[0.9, 0.2, 0.2, 1.0]], np.float32),
+        geom_dataid=np.array([-1, -1, -1, -1, -1, -1, 0, 0, 0]),
+        mesh_vertadr=np.array([0]), mesh_vertnum=np.array([len(ico.vertices)]),
+        mesh_faceadr=np.array([0]), mesh_facenum=np.array([len(ico.indices) // 3]),
+        mesh_vert=ico.vertices, mesh_face=ico.indices.reshape(-1, 3))
+
+
+def test_mujoco_geoms_become_shapes():
+    model = _mujoco_model()
+    shp, geoms = mujoco_shapes(model)
+    assert geoms.tolist() == [0, 1, 2, 5, 6, 8]
+    assert [s.kind for s in shp] == ["plane", "sphere", "capsule", "cylinder", "mesh", "mesh"]
+    assert shp[0].size == (4.0, 6.0)                      # half-extents to full sizes
+    assert shp[1].size == (0.3,) and shp[2].size == (0.1, 0.25) and shp[3].size == (0.15, 0.2)
+    assert np.allclose(shp[1].color, (0.9, 0.2, 0.2))     # the default colour gives way to the material's
+    assert np.allclose(shp[2].color, (0.1, 0.8, 0.1))
+    assert shp[4].mesh is shp[5].mesh                     # one Mesh per MuJoCo mesh
+    shp3, geoms3 = mujoco_shapes(model, geom_groups=(3,))
+    assert geoms3.tolist() == [3] and shp3[0].kind == "box" and shp3[0].size == (0.2, 0.3, 0.1)
+    sc = build_scene(shp)
+    assert np.allclose(sc.scale[[1, 3]], [[0.3] * 3, [0.15, 0.15, 0.2]])
+
+
+def test_scene_bounds():
+    c, e = scene_bounds([[0, 0, 0], [2, 0, 0]], [1.0, 1.0])
+    assert np.allclose(c, [1, 0, 0]) and np.isclose(e, np.sqrt(16 + 4 + 4) / 2)
+    c, e = scene_bounds(np.zeros((0, 3)), np.zeros(0))
+    assert np.allclose(c, 0) and e == 1.0
+
+
+def _geom_frames(worlds, ngeom, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(worlds * ngeom, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    xmat = shapes.quat_to_matrix(q).astype(np.float32).reshape(worlds, ngeom, 3, 3)
+    xpos = rng.uniform(-0.8, 0.8, size=(worlds, ngeom, 3)).astype(np.float32)
+    xpos[:, 0] = (0.0, 0.0, -1.0)                     # the plane, below
+    xmat[:, 0] = np.eye(3, dtype=np.float32)
+    return xpos, xmat, q.reshape(worlds, ngeom, 4)
+
+
+@pytest.mark.gpu
+def test_geom_frames_place_instances_and_render_as_the_oracle(gpu, oracle):
+    """ngx_set_geom_frames composes R diag(s) | p in FP32 (the Warp compose kernel's arithmetic, bit for bit); the
+    same placement from the quaternions through the oracle's FP64 body composition agrees to rounding (a transposed
+    rotation or a scale on the wrong axis would not); and the images equal the oracle's for the placed matrices."""
+    import torch
+    from tests.helpers import compare_images
+    model, worlds = _mujoco_model(), 3
+    xpos, xmat, q = _geom_frames(worlds, model.ngeom, 11)
+    sim = MJWarpBatchRenderer(model, worlds, 48, 40, data=SimpleNamespace(geom_xpos=xpos))
+    cams = sim.default_cameras().reshape(worlds, 12)
+    host = sim.render(SimpleNamespace(geom_xpos=xpos, geom_xmat=xmat), cams, rgb=False, depth=True, normals=True,
+                      object_ids=True)
+    m_host = sim.renderer.read_build()["matrices"].reshape(worlds, -1, 3, 4)
+    dev = SimpleNamespace(geom_xpos=torch.from_numpy(xpos).to(gpu), geom_xmat=torch.from_numpy(xmat).to(gpu))
+    out = sim.render(dev, cams, rgb=False, depth=True, normals=True, object_ids=True)
+    torch.cuda.synchronize()
+    m_dev = sim.renderer.read_build()["matrices"].reshape(worlds, -1, 3, 4)
+    g, sc = sim._geoms, sim.scene.scale
+    want = np.empty_like(m_host)
+    want[..., :3] = xmat[:, g] * sc[None, :, None, :]
+    want[..., 3] = xpos[:, g]
+    assert np.array_equal(m_host.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(m_dev.view(np.uint32), want.view(np.uint32))
+    for k in ("depth", "object_ids", "normals"):
+        assert np.array_equal(out[k], host[k]), k
+    # the same placement from the quaternions, composed in FP64 as a body pose with an identity local pose
+    n = len(g)
+    poses = np.concatenate([xpos[:, g], q[:, g]], axis=-1).astype(np.float32).reshape(worlds, n, 7)
+    poses[:, 0, 3:] = (0.0, 0.0, 0.0, 1.0)
+    local = np.tile(np.array([0, 0, 0, 0, 0, 0, 1], np.float32), (n, 1))
+    world_off = np.arange(worlds + 1, dtype=np.int32) * n
+    via_q = oracle.body_poses_to_matrices(world_off, n, np.arange(n, dtype=np.int32), local, sc, poses)
+    assert np.allclose(via_q.reshape(want.shape), want, atol=2e-6)
+    cfg = dict(width=48, height=40, worlds=worlds, cameras_per_world=1, meshes=sim.scene.meshes, instances=sim.scene.instances,
+               counts=None, mode="matrices", matrices=m_host.reshape(worlds, n, 12), cameras=cams, fov=1.0)
+    ref = forward(cfg, oracle)
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "mujoco geoms")
+    assert rep["hits"] > 0.3 * rep["pixels"]
+    seen = set(np.unique(out["object_ids"]).tolist()) - {ng.OBJECT_NONE}
+    assert seen <= set(sim.object_id_of_geom.values()) and len(seen) >= 3
+
+
+@pytest.mark.gpu
+def test_geom_binding_is_checked(gpu):
+    model = _mujoco_model()
+    sim = MJWarpBatchRenderer(model, 2, 16, 16)
+    with pytest.raises(ValueError, match="geom"):
+        sim.renderer.bind_geoms(np.full(6, 9), sim.scene.scale, model.ngeom)   # geom 9 of 9
+    bad = np.zeros((2, model.ngeom, 3), np.float32)
+    bad[1, 4, 2] = np.nan
+    with pytest.raises(ValueError, match="NaN"):
+        sim.renderer.set_geom_frames(bad, np.tile(np.eye(3, dtype=np.float32), (2, model.ngeom, 1, 1)))
+    r = ng.BatchRenderer(16, 16, 1, meshes=[ng.make_box((1, 1, 1))], instances=[ng.Instance(0)])
+    with pytest.raises(ValueError, match="bind_geoms"):
+        r.set_geom_frames(np.zeros((1, 1, 3), np.float32), np.zeros((1, 1, 9), np.float32))
