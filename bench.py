@@ -187,6 +187,49 @@ def bench_rgb(args, torch):
     return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3)
 
 
+# The reference's own published batch-renderer timings on this scene (bench/results/run2/summary.txt:20-22, :41-43:
+# neraulgraphics 0.5.0 on an RTX 5090, 4096 worlds, RGB only, ms per frame), keyed by (bodies per world, resolution).
+REFERENCE_SCENE_MS = {(8, 64): 1.4055, (8, 128): 4.0413, (32, 64): 3.9754, (32, 128): 12.0280}
+
+
+def bench_falling_bodies(args, torch):
+    """The reference benchmark's own scene (bench/scene.py, bench/run_ours.py) through MJWarpBatchRenderer: 4096 worlds
+    of 8 or 32 falling balls and boxes, per frame new geom frames (GPU arrays) composed into the top-level build, then
+    the RGB picture (one hard-shadow light, one sample a pixel) into the renderer's buffer. Timed as the reference times
+    it (bench/timing.py:43-72): warm-up, then blocks of frames of about 50 ms between two syncs; the median block."""
+    from types import SimpleNamespace
+    from neraulgraphics_amd import workloads
+    from neraulgraphics_amd.geometry import Light
+    from neraulgraphics_amd.simulators import MJWarpBatchRenderer
+    out = {}
+    for nbody in (8, 32):
+        fb = workloads.falling_bodies(nbody, 4096)
+        xpos = torch.from_numpy(fb["geom_xpos"]).cuda()
+        xmat = torch.from_numpy(fb["geom_xmat"]).cuda()
+        frames = [SimpleNamespace(geom_xpos=xpos[f], geom_xmat=xmat[f]) for f in range(xpos.shape[0])]
+        cams = torch.from_numpy(np.tile(fb["camera"].reshape(1, 12), (4096, 1))).cuda()
+        light = Light(fb["light"], intensity=1.0, radius=0.0)
+        for res in (64, 128):
+            view = MJWarpBatchRenderer(fb["model"], 4096, res, res, lights=[light], fov=fb["fov"])
+            k = [0]
+
+            def frame():
+                view.render(frames[k[0] % len(frames)], cams, rgb=True, zero_copy=True)
+                k[0] += 1
+
+            per = timed_steps(frame, 10, 10, torch)
+            block = int(min(2000, max(1, np.ceil(0.05 / per))))
+            reps = sorted(timed_steps(frame, block, 0, torch) for _ in range(args.scene_reps))
+            ms = reps[len(reps) // 2] * 1e3
+            ref = REFERENCE_SCENE_MS[(nbody, res)]
+            out[f"{nbody}_bodies_{res}"] = dict(ms=ms, images_per_s=4096 / ms * 1e3, reference_rtx5090_ms=ref,
+                                                speedup_vs_reference=ref / ms)
+            print(f"# falling_bodies {nbody} bodies {res}x{res}: {ms:.4f} ms/frame (reference on an RTX 5090: {ref} ms)",
+                  file=sys.stderr, flush=True)
+            del view
+    return out
+
+
 def bench_posed_bodies(args, torch):
     from neraulgraphics_amd import workloads
     from neraulgraphics_amd.autograd import render_depth
@@ -263,6 +306,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default trace configuration instead of autotuning")
     ap.add_argument("--no-lidar", action="store_true")
     ap.add_argument("--no-rgb", action="store_true")
+    ap.add_argument("--no-scene", action="store_true", help="skip the reference benchmark's falling-bodies scene")
+    ap.add_argument("--scene-reps", type=int, default=15)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 512 MiB three times with a plain copy kernel, to scale HBM counters")
@@ -301,6 +346,10 @@ def main():
     if not args.no_rgb:
         print("# body_worlds with RGB + depth + ID", file=sys.stderr, flush=True)
         crgb = bench_rgb(args, torch)
+    cscene = None
+    if not args.no_scene:
+        print("# falling_bodies: the reference benchmark's scene, RGB", file=sys.stderr, flush=True)
+        cscene = bench_falling_bodies(args, torch)
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline", file=sys.stderr, flush=True)
@@ -347,6 +396,8 @@ def main():
                     fwd_bwd_over_fwd=c4["ratio"])
     if crgb:
         line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"])
+    if cscene:
+        line["falling_bodies_rgb"] = cscene
     line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 

@@ -416,7 +416,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
             c.compute_units = std::max(1, prop.multiProcessorCount);
         }
-        device_alloc(c.d_tile_counters, 8);
+        device_alloc(c.d_tile_counters, (size_t)ngx::TILE_COUNTER_INTS);
         HIP_CHECK(hipEventCreate(&c.ev0));
         HIP_CHECK(hipEventCreate(&c.ev1));
 

@@ -13,6 +13,15 @@ namespace ngx {
 // or simulator geom frames (position and rotation matrix per geom) scaled by a geom binding.
 enum { SOURCE_MATRICES = 0, SOURCE_BODIES = 1, SOURCE_GEOMS = 2 };
 
+// The persistent launches' tile counters: each XCD's range of tiles is cut into COUNTER_SLICES slices with a
+// counter each, every counter on a 256-byte line of its own (COUNTER_STRIDE ints apart), so the counter updates
+// of the whole chip spread over 8 x COUNTER_SLICES lines instead of queueing on one.
+#ifndef NGX_COUNTER_SLICES
+#define NGX_COUNTER_SLICES 8
+#endif
+constexpr int COUNTER_SLICES = NGX_COUNTER_SLICES, COUNTER_STRIDE = 64;
+constexpr int TILE_COUNTER_INTS = 8 * COUNTER_SLICES * COUNTER_STRIDE;
+
 struct TlasBuildArgs {
     int worlds;
     int total;                 // N = world_off[worlds]
@@ -86,7 +95,7 @@ struct RenderArgs {
     int32_t *object_id;        // [views][H][W] or null
     int xcd_remap;             // 1: consecutive tiles stay on one XCD
     int walk;                  // 0 per-lane, 1 packet, 2 packet with fine meshes lane by lane
-    int *tile_counters;        // persistent mode (non-null): 8 counters, zeroed before the launch, one per XCD
+    int *tile_counters;        // persistent mode (non-null): TILE_COUNTER_INTS ints, zeroed by the launch
 };
 
 // The RGB picture's shading inputs, on the device: materials as two float4 (diffuse rgb, diffuse albedo;

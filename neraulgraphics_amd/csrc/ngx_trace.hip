@@ -874,31 +874,44 @@ __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID 
 
 // Persistent: a fixed grid whose wavefronts take tiles until none are left. The tiles are cut into eight
 // contiguous ranges, one per XCD (the dispatcher deals workgroups round-robin over the XCDs, so workgroup b
-// runs on XCD b mod 8); each range has its own counter, so the worlds of a range stay in one XCD's L2. A
-// wavefront takes PERSISTENT_CHUNK (2) consecutive tiles per counter increment, so cheap tiles do not queue on
-// the counter. Every wavefront leaves the loop once its range's counter passes the range's end.
+// runs on XCD b mod 8), so the worlds of a range stay in one XCD's L2; each range is cut into COUNTER_SLICES
+// slices with a counter each on its own cache line. A wavefront starts at the slice its index within the XCD
+// picks, takes PERSISTENT_CHUNK (1) tile per counter increment until the slice is used up, then
+// moves on to the XCD's next slice; it leaves once it has found every slice of its range used up, so every
+// wavefront ends after at most COUNTER_SLICES empty increments.
 #ifndef NGX_PERSISTENT_CHUNK
-#define NGX_PERSISTENT_CHUNK 2
+#define NGX_PERSISTENT_CHUNK 1
 #endif
 constexpr int PERSISTENT_CHUNK = NGX_PERSISTENT_CHUNK;
+
+template <class Work>
+__device__ __forceinline__ void take_tiles(int *counters, const long long total, Work &&work) {
+    const int xcd = blockIdx.x & 7;
+    const long long first = total * xcd / 8, count = total * (xcd + 1) / 8 - first;
+    const int home = uniform((int)((blockIdx.x >> 3) * (blockDim.x >> 6) + (threadIdx.x >> 6)) % COUNTER_SLICES);
+    for (int i = 0; i < COUNTER_SLICES; ++i) {
+        const int s = (home + i) % COUNTER_SLICES;
+        const long long lo = count * s / COUNTER_SLICES, n = count * (s + 1) / COUNTER_SLICES - lo;
+        int *c = counters + (xcd * COUNTER_SLICES + s) * COUNTER_STRIDE;
+        while (true) {
+            int t = 0;
+            if (lane_id() == 0) t = atomicAdd(c, PERSISTENT_CHUNK);
+            t = __builtin_amdgcn_readlane(t, 0);
+            if (t >= n) break;
+            const int end = (int)min((long long)t + PERSISTENT_CHUNK, n);
+            for (int k = t; k < end; ++k) work(first + lo + k);
+        }
+    }
+}
 
 template <int WALK>
 __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? NGX_HYBRID_MIN_BLOCKS : 1) void render_persistent_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
-    const int xcd = blockIdx.x & 7;
-    const long long first = A.total_tiles * xcd / 8, count = A.total_tiles * (xcd + 1) / 8 - first;
-    while (true) {
-        int t = 0;
-        if (lane_id() == 0) t = atomicAdd(&A.tile_counters[xcd], PERSISTENT_CHUNK);
-        t = __builtin_amdgcn_readlane(t, 0);
-        if (t >= count) break;
-        const int end = (int)min((long long)t + PERSISTENT_CHUNK, count);
-        for (int k = t; k < end; ++k) render_tile<WALK>(A, L, first + k);
-    }
+    take_tiles(A.tile_counters, A.total_tiles, [&](const long long tile) { render_tile<WALK>(A, L, tile); });
 }
 
 __global__ void zero_counters_kernel(int *c) {
-    if (threadIdx.x < 8) c[threadIdx.x] = 0;
+    if (threadIdx.x < 8 * COUNTER_SLICES) c[threadIdx.x * COUNTER_STRIDE] = 0;
 }
 
 #ifdef NGX_TRACE_STATS
@@ -1159,21 +1172,12 @@ __global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_kernel(const RgbA
     rgb_tile<WALK>(G, L, tile);
 }
 
-// The same tiles from a fixed grid taking PERSISTENT_CHUNK tiles at a time from per-XCD counters, as the
-// persistent depth kernel does.
+// The same tiles from a fixed grid taking them from the per-XCD slice counters, as the persistent depth
+// kernel does.
 template <int WALK>
 __global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_persistent_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
-    const int xcd = blockIdx.x & 7;
-    const long long first = G.r.total_tiles * xcd / 8, count = G.r.total_tiles * (xcd + 1) / 8 - first;
-    while (true) {
-        int t = 0;
-        if (lane_id() == 0) t = atomicAdd(&G.r.tile_counters[xcd], PERSISTENT_CHUNK);
-        t = __builtin_amdgcn_readlane(t, 0);
-        if (t >= count) break;
-        const int end = (int)min((long long)t + PERSISTENT_CHUNK, count);
-        for (int k = t; k < end; ++k) rgb_tile<WALK>(G, L, first + k);
-    }
+    take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_tile<WALK>(G, L, tile); });
 }
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
@@ -1443,7 +1447,7 @@ hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int comp
     if (a.total_tiles <= 0) return hipSuccess;
     if (a.tile_counters && compute_units > 0) {
         if (a.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
-        zero_counters_kernel<<<1, 64, 0, stream>>>(a.tile_counters);
+        zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.tile_counters);
 #define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_grid<W>(compute_units), 256, 0, stream>>>(a)
         NGX_BY_WALK(a.walk, NGX_LAUNCH)
 #undef NGX_LAUNCH
@@ -1482,7 +1486,7 @@ hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_un
     if (a.spp < 1 || a.shade.light_count < 0 || a.shade.light_count > MAX_LIGHTS) return hipErrorInvalidValue;
     if (a.r.tile_counters && compute_units > 0) {
         if (a.r.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
-        zero_counters_kernel<<<1, 64, 0, stream>>>(a.r.tile_counters);
+        zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
 #define NGX_LAUNCH(W) rgb_persistent_kernel<W><<<rgb_persistent_grid<W>(compute_units), 256, 0, stream>>>(a)
         NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
 #undef NGX_LAUNCH

@@ -16,7 +16,15 @@
       its own body; the target is rendered from poses perturbed by N(0, 0.01) in translation and 1 degree
       about a random axis. Seed 1004.
 
-Quaternions are Gaussian four-vectors normalized. Everything is float32."""
+Quaternions are Gaussian four-vectors normalized. Everything is float32.
+
+falling_bodies(nbody, ...) is the scene of the reference's own batch-renderer benchmark (bench/scene.py,
+bench/run_ours.py): per world `nbody` free bodies in rows of 8, even ones a ball (icosphere of 2 subdivisions,
+radius 0.12, 320 triangles), odd ones a box (half-size 0.1, 12 triangles), no ground; one camera framing the rows;
+one point light at (0, 0, 4) with no radius; RGB, one sample a pixel. The reference replays poses recorded from
+MuJoCo Warp physics; MuJoCo is absent here, so the bodies fall under gravity from the same starting layout (nudged
+and spun as bench/gen_poses.py:33-40 does) to rest on the ground, turning as they go, over the same 24 frames
+20 ms apart."""
 import numpy as np
 
 from .geometry import Instance, Transform, look_at, make_box, make_grid, make_icosphere, make_transforms
@@ -188,3 +196,49 @@ def make_renderer(cfg, **kw):
     else:
         r.set_instance_transforms(cfg["matrices"])
     return r
+
+
+def falling_bodies(nbody=8, worlds=4096, frames=24, seed=1010):
+    """The reference benchmark's scene (see the module docstring): a MuJoCo-like model object (the MjModel fields
+    neraulgraphics_amd.simulators.mujoco_shapes reads), per-frame geom frames geom_xpos (frames, worlds, nbody, 3)
+    and geom_xmat (frames, worlds, nbody, 9), the camera (3 x 4, camera-to-world), the vertical fov in radians and
+    the light position."""
+    from types import SimpleNamespace
+    from .shapes import look_at, quat_to_matrix
+    rng = np.random.default_rng(seed)
+    columns, spacing = 8, 0.3
+    ball, box = make_icosphere(0.12, 2), make_box((0.1, 0.1, 0.1))
+    model = SimpleNamespace(
+        ngeom=nbody, geom_type=np.full(nbody, 7), geom_size=np.zeros((nbody, 3)),
+        geom_rgba=np.array([[0.8, 0.2, 0.2, 1.0] if i % 2 == 0 else [0.2, 0.4, 0.8, 1.0] for i in range(nbody)], np.float32),
+        geom_group=np.zeros(nbody, np.int32), geom_matid=np.full(nbody, -1), mat_rgba=np.zeros((0, 4), np.float32),
+        geom_dataid=np.arange(nbody) % 2,
+        mesh_vertadr=np.array([0, len(ball.vertices)]), mesh_vertnum=np.array([len(ball.vertices), len(box.vertices)]),
+        mesh_faceadr=np.array([0, len(ball.indices) // 3]), mesh_facenum=np.array([len(ball.indices) // 3, len(box.indices) // 3]),
+        mesh_vert=np.concatenate([ball.vertices, box.vertices]),
+        mesh_face=np.concatenate([ball.indices.reshape(-1, 3), box.indices.reshape(-1, 3)]))
+    i = np.arange(nbody)
+    start = np.stack([(i % columns - (columns - 1) / 2) * spacing, (i // columns) * spacing - 0.2, 0.3 + 0.02 * i], axis=1)
+    p0 = np.broadcast_to(start, (worlds, nbody, 3)).copy()
+    p0[..., :2] += rng.uniform(-0.04, 0.04, (worlds, nbody, 2))
+    p0[..., 2] += rng.uniform(0.0, 0.08, (worlds, nbody))
+    q0 = rng.normal(size=(worlds, nbody, 4))
+    q0 /= np.linalg.norm(q0, axis=-1, keepdims=True)
+    axis = rng.normal(size=(worlds, nbody, 3))
+    axis /= np.linalg.norm(axis, axis=-1, keepdims=True)
+    spin = rng.uniform(0.0, 3.0, (worlds, nbody))
+    rest = np.where(i % 2 == 0, 0.12, 0.1)
+    xpos = np.empty((frames, worlds, nbody, 3), np.float32)
+    xmat = np.empty((frames, worlds, nbody, 9), np.float32)
+    for f in range(frames):
+        t = 0.02 * (f + 1)
+        p = p0.copy()
+        p[..., 2] = np.maximum(rest, p0[..., 2] - 4.905 * t * t)
+        half = 0.5 * spin * t
+        dq = np.concatenate([axis * np.sin(half)[..., None], np.cos(half)[..., None]], axis=-1)
+        xpos[f] = p
+        xmat[f] = quat_to_matrix(_qmul(dq, q0).reshape(-1, 4)).reshape(worlds, nbody, 9)
+    rows = (nbody + columns - 1) // columns
+    camera = look_at([0.0, -1.9, 1.1], [0.0, (rows - 1) * spacing / 2 - 0.2, 0.15])[0]
+    return dict(model=model, geom_xpos=xpos, geom_xmat=xmat, camera=camera, fov=float(np.radians(45.0)),
+                light=(0.0, 0.0, 4.0), worlds=worlds, nbody=nbody)

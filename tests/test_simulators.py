@@ -207,3 +207,34 @@ def test_geom_binding_is_checked(gpu):
     r = ng.BatchRenderer(16, 16, 1, meshes=[ng.make_box((1, 1, 1))], instances=[ng.Instance(0)])
     with pytest.raises(ValueError, match="bind_geoms"):
         r.set_geom_frames(np.zeros((1, 1, 3), np.float32), np.zeros((1, 1, 9), np.float32))
+
+
+@pytest.mark.gpu
+def test_the_reference_benchmark_scene_renders_as_the_oracle(gpu, oracle):
+    """The falling-bodies scene of the reference's batch-renderer benchmark (bench/scene.py), drawn through
+    MJWarpBatchRenderer as bench/run_ours.py draws it (geom frames from the GPU, one hard-shadow light, RGB, one
+    sample), against the oracle: depth and IDs to the usual bar, the picture to tests/test_batch.cpp:177's bar of
+    0.05 of a level mean per channel."""
+    import torch
+    from tests.helpers import compare_images
+    from neraulgraphics_amd import workloads
+    fb = workloads.falling_bodies(8, worlds=6, frames=3)
+    light = ng.Light(fb["light"], intensity=1.0, radius=0.0)
+    sim = MJWarpBatchRenderer(fb["model"], 6, 48, 48, lights=[light], fov=fb["fov"])
+    cams = np.tile(fb["camera"].reshape(1, 12), (6, 1))
+    dev_cams = torch.from_numpy(cams).to(gpu)
+    for f in (0, 2):
+        data = SimpleNamespace(geom_xpos=torch.from_numpy(fb["geom_xpos"][f]).to(gpu),
+                               geom_xmat=torch.from_numpy(fb["geom_xmat"][f]).to(gpu))
+        out = sim.render(data, dev_cams, rgb=True, depth=True, object_ids=True, zero_copy=True)
+        got = {k: v.cpu().numpy() for k, v in out.items()}
+        mats = sim.renderer.read_build()["matrices"].reshape(6, 8, 12)
+        sc = sim.scene
+        cfg = dict(width=48, height=48, worlds=6, cameras_per_world=1, meshes=sc.meshes, instances=sc.instances, counts=None,
+                   mode="matrices", matrices=mats, cameras=cams, fov=fb["fov"], floor=np.zeros(4, np.float32),
+                   shading=ng.shading_arrays(sc.instances, [], ng.Plane(), [light], 0.2, None), spp=1, frame=0)
+        ref = forward(cfg, oracle, outputs=("depth", "normal", "id", "rgb"))
+        rep = compare_images(dict(depth=got["depth"], object_id=got["object_ids"]), ref, f"falling bodies frame {f}")
+        assert rep["hits"] > 0.03 * rep["pixels"]
+        diff = np.abs(got["rgb"].astype(np.int32) - ref["rgb"].astype(np.int32))
+        assert float(diff.mean()) < 0.05, float(diff.mean())
