@@ -190,6 +190,9 @@ def bench_rgb(args, torch):
 # The reference's own published batch-renderer timings on this scene (bench/results/run2/summary.txt:20-22, :41-43:
 # neraulgraphics 0.5.0 on an RTX 5090, 4096 worlds, RGB only, ms per frame), keyed by (bodies per world, resolution).
 REFERENCE_SCENE_MS = {(8, 64): 1.4055, (8, 128): 4.0413, (32, 64): 3.9754, (32, 128): 12.0280}
+# The reference's README (README.md:93-95): 4096 worlds at 128x128, RGB, depth and object IDs, new poses every step,
+# "about 6 ms" on an RTX 5090.
+REFERENCE_README_RGB_DEPTH_ID_MS = 6.0
 
 
 def bench_falling_bodies(args, torch):
@@ -209,23 +212,22 @@ def bench_falling_bodies(args, torch):
         frames = [SimpleNamespace(geom_xpos=xpos[f], geom_xmat=xmat[f]) for f in range(xpos.shape[0])]
         cams = torch.from_numpy(np.tile(fb["camera"].reshape(1, 12), (4096, 1))).cuda()
         light = Light(fb["light"], intensity=1.0, radius=0.0)
-        for res in (64, 128):
+        for res, aov in ((64, False), (128, False)) + (((128, True),) if nbody == 8 else ()):
             view = MJWarpBatchRenderer(fb["model"], 4096, res, res, lights=[light], fov=fb["fov"])
             k = [0]
 
             def frame():
-                view.render(frames[k[0] % len(frames)], cams, rgb=True, zero_copy=True)
+                view.render(frames[k[0] % len(frames)], cams, rgb=True, depth=aov, object_ids=aov, zero_copy=True)
                 k[0] += 1
 
             per = timed_steps(frame, 10, 10, torch)
             block = int(min(2000, max(1, np.ceil(0.05 / per))))
             reps = sorted(timed_steps(frame, block, 0, torch) for _ in range(args.scene_reps))
             ms = reps[len(reps) // 2] * 1e3
-            ref = REFERENCE_SCENE_MS[(nbody, res)]
-            out[f"{nbody}_bodies_{res}"] = dict(ms=ms, images_per_s=4096 / ms * 1e3, reference_rtx5090_ms=ref,
-                                                speedup_vs_reference=ref / ms)
-            print(f"# falling_bodies {nbody} bodies {res}x{res}: {ms:.4f} ms/frame (reference on an RTX 5090: {ref} ms)",
-                  file=sys.stderr, flush=True)
+            ref = REFERENCE_README_RGB_DEPTH_ID_MS if aov else REFERENCE_SCENE_MS[(nbody, res)]
+            key = f"{nbody}_bodies_{res}" + ("_rgb_depth_id" if aov else "")
+            out[key] = dict(ms=ms, images_per_s=4096 / ms * 1e3, reference_rtx5090_ms=ref, speedup_vs_reference=ref / ms)
+            print(f"# falling_bodies {key}: {ms:.4f} ms/frame (reference on an RTX 5090: {ref} ms)", file=sys.stderr, flush=True)
             del view
     return out
 

@@ -133,6 +133,7 @@ struct RayArgs {
     int32_t *object_id;
     int walk;
     const int *order;          // null, or per world the order to trace its rays in (indices within the world)
+    int xcd_remap;             // 1: each XCD takes a contiguous range of the waves
 };
 
 // The pose backward of depth: dL/d(body pose) for every world, through each pixel's fixed hit triangle.

@@ -976,11 +976,15 @@ __device__ __forceinline__ float3 sky(const ShadeScene &E, const float3 dir) {
     const float t = saturate(dot3(dir, E.sky_up) * 0.5f + 0.5f);
     return mul3(add3(E.sky_color_down, mul3(sub3(E.sky_color_up, E.sky_color_down), t)), E.sky_intensity);
 }
-// geom_dev.h:141-152: a uniform direction, then a point on the light's sphere
+// geom_dev.h:141-152: a uniform direction, then a point on the light's sphere. The two random numbers are drawn
+// whatever the radius; a point light (radius 0, the same for every lane) is its centre, which is what adding the
+// finite unit vector times 0 gives, so its sine and cosine are not evaluated.
 __device__ __forceinline__ float3 sample_sphere_light(const float3 c, const float radius, uint32_t &rng) {
-    const float z = 1.0f - 2.0f * rand01(rng);
+    const float u = rand01(rng), v = rand01(rng);
+    if (radius == 0.0f) return c;
+    const float z = 1.0f - 2.0f * u;
     const float rad = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    const float phi = 6.28318530718f * rand01(rng);
+    const float phi = 6.28318530718f * v;
     return add3(c, mul3(mk3(rad * cosf(phi), rad * sinf(phi), z), radius));
 }
 // device_scene.h:98-111
@@ -1180,12 +1184,13 @@ __global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_persistent_kernel
     take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_tile<WALK>(G, L, tile); });
 }
 
-// Ray queries (render_core.h:331-349): 64 consecutive rays a wave; a wave whose rays fall in several
-// worlds walks each world's rays as its own packet.
+// Ray queries (render_core.h:331-349): 64 consecutive rays a wave (with xcd_remap, each XCD takes a
+// contiguous range of the waves, so the rays an XCD holds at once are neighbours in the direction order); a
+// wave whose rays fall in several worlds walks each world's rays as its own packet.
 template <int WALK>
 __global__ __launch_bounds__(256) void ray_kernel(const RayArgs A) {
     NGX_PAIR_LDS(L);
-    const long long base = ((long long)blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6)) * 64;
+    const long long base = wave_tile(A.xcd_remap, 0) * 64;
     if (base >= A.count) return;
     const int lane = lane_id();
     const long long slot = base + lane;
