@@ -1062,10 +1062,17 @@ __device__ __forceinline__ float srgb_channel(const float v) {
     return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
 }
 
-// 4 workgroups (16 wavefronts) per compute unit: at most 128 VGPRs
+// Workgroups per compute unit the RGB kernels are compiled for: 4 (16 wavefronts, at most 128 VGPRs) for the
+// packet and per-lane walks, 5 (at most 102 VGPRs, a few more spills) for the hybrid walk, whose scenes have
+// more to hide (body_worlds RGB 27.9 -> 26.2 ms; the 8-body falling-bodies scene, packet walk, is 3.45 ms at 4
+// and 3.87 ms at 5).
 #ifndef NGX_RGB_MIN_BLOCKS
 #define NGX_RGB_MIN_BLOCKS 4
 #endif
+#ifndef NGX_RGB_MIN_BLOCKS_HYBRID
+#define NGX_RGB_MIN_BLOCKS_HYBRID 5
+#endif
+#define NGX_RGB_BLOCKS(W) ((W) == WALK_HYBRID ? NGX_RGB_MIN_BLOCKS_HYBRID : NGX_RGB_MIN_BLOCKS)
 template <int WALK>
 __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const long long tile) {
     const RenderArgs &A = G.r;
@@ -1169,7 +1176,7 @@ __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const lon
 }
 
 template <int WALK>
-__global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_kernel(const RgbArgs G) {
+__global__ __launch_bounds__(256, NGX_RGB_BLOCKS(WALK)) void rgb_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(G.r.xcd_remap, G.r.total_tiles);
     if (tile >= G.r.total_tiles) return;
@@ -1179,7 +1186,7 @@ __global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_kernel(const RgbA
 // The same tiles from a fixed grid taking them from the per-XCD slice counters, as the persistent depth
 // kernel does.
 template <int WALK>
-__global__ __launch_bounds__(256, NGX_RGB_MIN_BLOCKS) void rgb_persistent_kernel(const RgbArgs G) {
+__global__ __launch_bounds__(256, NGX_RGB_BLOCKS(WALK)) void rgb_persistent_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
     take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_tile<WALK>(G, L, tile); });
 }

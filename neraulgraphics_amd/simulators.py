@@ -16,9 +16,10 @@ import numpy as np
 
 from .batch import BatchRenderer
 from .geometry import OBJECT_INSTANCE, Instance, Light, Material, Mesh, make_box, make_icosphere
-from .shapes import capsule, cone, cylinder, look_at, plane
+from .shapes import capsule, compose_poses, cone, cylinder, look_at, plane
 
-__all__ = ["Shape", "SimScene", "SimBatchRenderer", "MJWarpBatchRenderer", "build_scene", "mujoco_shapes", "scene_bounds"]
+__all__ = ["Shape", "SimScene", "SimBatchRenderer", "MJWarpBatchRenderer", "NewtonBatchRenderer", "build_scene", "mujoco_shapes",
+           "newton_shapes", "scene_bounds"]
 
 KINDS = ("sphere", "box", "ellipsoid", "cylinder", "cone", "capsule", "plane", "mesh")
 UNBOUNDED_PLANE = 1000.0  # the size the Newton viewer gives a plane of size 0
@@ -290,3 +291,163 @@ def _to_numpy(a):
     import torch
     t = a if isinstance(a, torch.Tensor) else torch.from_dlpack(a)
     return t.detach().cpu().numpy()
+
+
+# ---- Newton
+
+_NEWTON_UP = {0: (1.0, 0.0, 0.0), 1: (0.0, 1.0, 0.0), 2: (0.0, 0.0, 1.0)}
+
+
+def _newton_enums(geo_types, visible_flag):
+    """Newton's GeoType and the VISIBLE shape flag: from `newton` when it is importable, else as given."""
+    if geo_types is None or visible_flag is None:
+        try:
+            import newton
+        except ImportError as e:
+            raise ImportError("newton is not installed: pass geo_types (an object with Newton's GeoType names as "
+                              "attributes) and visible_flag") from e
+        geo_types = newton.GeoType if geo_types is None else geo_types
+        visible_flag = int(newton.ShapeFlags.VISIBLE) if visible_flag is None else visible_flag
+    return geo_types, int(visible_flag)
+
+
+def _host(a):
+    """A Warp array (.numpy()), a tensor or a NumPy array as NumPy."""
+    if hasattr(a, "numpy") and not isinstance(a, np.ndarray):
+        a = a.detach().cpu() if hasattr(a, "detach") else a
+        return np.asarray(a.numpy())
+    return np.asarray(a)
+
+
+def newton_shapes(model, geo_types=None, visible_flag=None):
+    """One world's template of a batched Newton model, as the reference's NewtonBatchRenderer reads it
+    (python/neraulgraphics/newton.py:250-296): the visible shapes that belong to no world (a ground plane), then those
+    of world 0, each on its body's index within the world; every world must hold the same shapes on the same bodies,
+    as Newton's replicate() makes them (checked). Returns (shapes, bodies_per_world). Reads world_count, body_count,
+    body_world, shape_type, shape_scale, shape_flags, shape_world, shape_body, shape_transform (x, y, z, qx, qy, qz,
+    qw), shape_color and shape_source. Sizes are Newton's shape_scale, the convention Shape takes; planes have their
+    colour, not the reference's checkerboard texture."""
+    geo, visible = _newton_enums(geo_types, visible_flag)
+    worlds, bodies = int(model.world_count), int(model.body_count)
+    if bodies % worlds:
+        raise ValueError(f"the model has {bodies} bodies in {worlds} worlds, which is not the same number in each")
+    per_world = bodies // worlds
+    if bodies and not np.array_equal(_host(model.body_world), np.repeat(np.arange(worlds), per_world)):
+        raise ValueError("the bodies are not stored world by world (all of world 0's, then world 1's, and so on)")
+    shape_type, shape_scale = _host(model.shape_type), _host(model.shape_scale)
+    flags, shape_world = _host(model.shape_flags), _host(model.shape_world)
+    shape_body, transform = _host(model.shape_body), _host(model.shape_transform)
+    colors = _host(model.shape_color) if getattr(model, "shape_color", None) is not None else None
+    kinds = {int(geo.SPHERE): "sphere", int(geo.BOX): "box", int(geo.ELLIPSOID): "ellipsoid", int(geo.CYLINDER): "cylinder",
+             int(geo.CONE): "cone", int(geo.CAPSULE): "capsule", int(geo.PLANE): "plane", int(geo.MESH): "mesh"}
+    if hasattr(geo, "CONVEX_MESH"):
+        kinds[int(geo.CONVEX_MESH)] = "mesh"
+    entries, signatures, meshes = {}, {}, {}
+    for i in range(len(shape_type)):
+        if not int(flags[i]) & visible:
+            continue
+        kind = kinds.get(int(shape_type[i]))
+        source = model.shape_source[i] if kind == "mesh" else None
+        if kind is None or (kind == "mesh" and source is None):
+            continue
+        w, body = int(shape_world[i]), int(shape_body[i])
+        local_body = body - w * per_world if (body >= 0 and w >= 0) else body
+        color = tuple(float(c) for c in colors[i][:3]) if colors is not None else _FALLBACK_COLOR
+        mesh = None
+        if kind == "mesh":  # one Mesh per source, shared by the shapes that draw it
+            if id(source) not in meshes:
+                meshes[id(source)] = Mesh(np.asarray(source.vertices, np.float32), np.asarray(source.indices, np.uint32))
+            mesh = meshes[id(source)]
+        size = tuple(float(v) for v in shape_scale[i])
+        tf = [float(v) for v in transform[i][:7]]
+        entries.setdefault(w, []).append(Shape(kind, size, tuple(tf[:3]), tuple(tf[3:7]), local_body, color, mesh))
+        signatures.setdefault(w, []).append((kind, tuple(np.round(size, 5)), local_body, tuple(np.round(tf, 5)),
+                                             tuple(np.round(color, 4)), None if mesh is None else id(source)))
+    template = signatures.get(0, [])
+    for w in range(1, worlds):
+        if signatures.get(w, []) != template:
+            raise ValueError(f"world {w} differs from world 0 (its shapes, sizes, colours, bodies or offsets), "
+                             "but a batch needs identical worlds, as Newton's replicate() makes them")
+    shapes = entries.get(-1, []) + entries.get(0, [])
+    if not shapes:
+        raise ValueError("the model has no visible shapes to draw")
+    return shapes, per_world
+
+
+class NewtonBatchRenderer:
+    """Draws a batched Newton simulation, each world seen by its own cameras: the reference's NewtonBatchRenderer
+    (python/neraulgraphics/newton.py:235-370). The shapes of one world become instances bound to their bodies once;
+    each render hands state.body_q ((world_count x bodies_per_world) poses x, y, z, qx, qy, qz, qw; a GPU tensor,
+    a Warp array through DLPack, or NumPy) to set_body_poses, one kernel. The default cameras and lights frame world
+    0's shapes in the model's initial pose (model.body_q), as the reference does."""
+
+    def __init__(self, model, width=128, height=128, cameras_per_world=1, *, lights=None, fov=1.0, ambient=0.2,
+                 geo_types=None, visible_flag=None, device_id=None):
+        shapes, per_world = newton_shapes(model, geo_types, visible_flag)
+        self._worlds, self._cameras_per_world = int(model.world_count), int(cameras_per_world)
+        self._bodies = per_world
+        self.width, self.height, self.fov = int(width), int(height), float(fov)
+        self.up = np.asarray(_NEWTON_UP[int(getattr(model, "up_axis", 2))], np.float64)
+        scene = build_scene(shapes)
+        initial = _host(model.body_q).reshape(-1, 7)[:per_world] if per_world else np.zeros((0, 7))
+        positions, reach = [], []
+        for sh, sc in zip(shapes, scene.scale):
+            p = np.asarray(sh.position, np.float64)[None]
+            if sh.body >= 0:
+                p, _ = compose_poses(initial[sh.body:sh.body + 1, :3], initial[sh.body:sh.body + 1, 3:7], p,
+                                     np.asarray(sh.rotation, np.float64)[None])
+            positions.append(p[0])
+            reach.append(_reach(sh, sc))
+        reach = np.asarray(reach)
+        solid = reach > 0
+        self.center, extent = scene_bounds(np.asarray(positions).reshape(-1, 3)[solid], reach[solid])
+        self.extent = max(extent, 1e-3)
+        self._initial_body_q = initial
+        self.sim = SimBatchRenderer(shapes, self._worlds, per_world, self.width, self.height, self._cameras_per_world,
+                                    lights=self.default_lights() if lights is None else lights, ambient=ambient,
+                                    device_id=device_id)
+        self.renderer = self.sim.renderer
+        self.scene = self.sim.scene
+
+    @property
+    def view_count(self) -> int:
+        return self._worlds * self._cameras_per_world
+
+    def default_lights(self):
+        """A key light high and to one side and a dimmer fill from the other, scaled to the scene
+        (python/neraulgraphics/_adapter.py:128-134)."""
+        side = _horizontal(self.up)
+        across = np.cross(self.up, side)
+        key = self.center + self.up * 3.0 * self.extent + side * 1.5 * self.extent + across * 1.0 * self.extent
+        fill = self.center + self.up * 2.0 * self.extent - side * 2.0 * self.extent - across * 1.5 * self.extent
+        return [Light(tuple(key), intensity=1.0, radius=0.0), Light(tuple(fill), intensity=0.4, radius=0.0)]
+
+    def default_cameras(self):
+        """One camera per view, framing world 0's shapes from the side, slightly above: (views, 3, 4)."""
+        eye = self.center - _horizontal(self.up) * 2.6 * self.extent + self.up * 0.7 * self.extent
+        return np.tile(look_at(eye, self.center, self.up), (self.view_count, 1, 1))
+
+    def update(self, state=None) -> None:
+        """Places every instance of every world from state.body_q (default: the model's initial pose)."""
+        if not self._bodies:
+            return
+        body_q = _as_frames(state.body_q) if state is not None else np.tile(self._initial_body_q, (self._worlds, 1))
+        self.renderer.set_body_poses(body_q)
+
+    def render(self, state, cameras=None, *, fov=None, rgb=True, depth=False, normals=False, object_ids=False,
+               zero_copy=False, frame=0, samples_per_pixel=1):
+        """update(state), then the images asked for as a dict (see BatchRenderer.render)."""
+        self.update(state)
+        return self.renderer.render(self.default_cameras() if cameras is None else cameras, self.fov if fov is None else fov,
+                                    samples_per_pixel, rgb=rgb, depth=depth, normals=normals, object_ids=object_ids,
+                                    frame=frame, zero_copy=zero_copy)
+
+    def cast_rays(self, origins, directions, max_distance=float("inf"), **kw):
+        return self.renderer.cast_rays(origins, directions, max_distance, **kw)
+
+
+def _horizontal(up):
+    """A unit vector perpendicular to `up`, the direction the default camera looks along (_adapter.py:79-83)."""
+    probe = np.array([0.0, 1.0, 0.0]) if abs(up[1]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    h = np.cross(up, np.cross(probe, up))
+    return h / np.linalg.norm(h)

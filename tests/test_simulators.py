@@ -11,7 +11,8 @@ import pytest
 
 import neraulgraphics_amd as ng
 from neraulgraphics_amd import shapes
-from neraulgraphics_amd.simulators import MJWarpBatchRenderer, Shape, SimBatchRenderer, build_scene, mujoco_shapes, scene_bounds
+from neraulgraphics_amd.simulators import (MJWarpBatchRenderer, NewtonBatchRenderer, Shape, SimBatchRenderer, build_scene, mujoco_shapes,
+                                          newton_shapes, scene_bounds)
 from oracle.oracle import forward
 
 
@@ -238,3 +239,65 @@ def test_the_reference_benchmark_scene_renders_as_the_oracle(gpu, oracle):
         assert rep["hits"] > 0.03 * rep["pixels"]
         diff = np.abs(got["rgb"].astype(np.int32) - ref["rgb"].astype(np.int32))
         assert float(diff.mean()) < 0.05, float(diff.mean())
+
+
+# Newton's GeoType names with codes of this test's own (Newton is not installed; the adapter reads the names).
+_GEO = SimpleNamespace(PLANE=0, SPHERE=2, CAPSULE=3, ELLIPSOID=4, CYLINDER=5, BOX=6, MESH=7, CONE=9, CONVEX_MESH=10)
+
+
+def _newton_model(worlds=3, break_world=None):
+    """The Newton Model fields the adapter reads: a ground plane in no world, then per world a sphere on body 0, a box
+    and a capsule on body 1 (with an offset and a turn) and an invisible box."""
+    q = np.array([0.0, 0.38268343, 0.0, 0.92387953])
+    types, scales, flags, sworld, sbody, tfs, colors = [0], [[0.0, 0.0, 0.0]], [1], [-1], [-1], [[0, 0, -0.3, 0, 0, 0, 1]], [[0.3] * 3]
+    for w in range(worlds):
+        box = [0.1, 0.2, 0.3] if w != break_world else [0.1, 0.2, 0.35]
+        for t, sc, fl, b, tf, c in [(2, [0.2, 0, 0], 1, 0, [0, 0, 0.1, 0, 0, 0, 1], [0.9, 0.1, 0.1]),
+                                    (6, box, 1, 1, [0.1, 0, 0, *q], [0.1, 0.9, 0.1]),
+                                    (3, [0.05, 0.2, 0], 1, 1, [0, 0.2, 0, 0, 0, 0, 1], [0.1, 0.1, 0.9]),
+                                    (6, [1, 1, 1], 0, 1, [0, 0, 0, 0, 0, 0, 1], [1, 1, 1])]:
+            types.append(t); scales.append(sc); flags.append(fl); sworld.append(w); sbody.append(2 * w + b); tfs.append(tf)
+            colors.append(c)
+    rng = np.random.default_rng(4)
+    qb = rng.normal(size=(2 * worlds, 4))
+    body_q = np.concatenate([rng.uniform(-0.5, 0.5, (2 * worlds, 3)) + [0, 0, -3.0], qb / np.linalg.norm(qb, axis=1, keepdims=True)], 1)
+    return SimpleNamespace(world_count=worlds, body_count=2 * worlds, body_world=np.repeat(np.arange(worlds), 2),
+                           shape_type=np.array(types), shape_scale=np.array(scales, np.float32), shape_flags=np.array(flags),
+                           shape_world=np.array(sworld), shape_body=np.array(sbody), shape_transform=np.array(tfs, np.float32),
+                           shape_color=np.array(colors, np.float32), shape_source=[None] * len(types), up_axis=2,
+                           body_q=body_q.astype(np.float32))
+
+
+def test_newton_models_become_a_template_on_bodies():
+    shp, per_world = newton_shapes(_newton_model(), _GEO, 1)
+    assert per_world == 2
+    assert [s.kind for s in shp] == ["plane", "sphere", "box", "capsule"]
+    assert [s.body for s in shp] == [-1, 0, 1, 1]
+    assert shp[2].size == pytest.approx((0.1, 0.2, 0.3)) and shp[2].position == pytest.approx((0.1, 0.0, 0.0))
+    assert np.allclose(shp[1].color, (0.9, 0.1, 0.1))
+    with pytest.raises(ValueError, match="world 2 differs"):
+        newton_shapes(_newton_model(break_world=2), _GEO, 1)
+    m = _newton_model()
+    m.body_world = np.array([0, 1, 0, 1, 2, 2])
+    with pytest.raises(ValueError, match="world by world"):
+        newton_shapes(m, _GEO, 1)
+
+
+@pytest.mark.gpu
+def test_a_newton_batch_renders_as_the_oracle(gpu, oracle):
+    """NewtonBatchRenderer with state.body_q as a GPU tensor of all worlds' bodies, against the oracle's FP64 body
+    composition and render of the same template and poses."""
+    import torch
+    from tests.helpers import compare_images
+    model = _newton_model()
+    view = NewtonBatchRenderer(model, 40, 32, geo_types=_GEO, visible_flag=1)
+    cams = view.default_cameras().reshape(-1, 12)
+    state = SimpleNamespace(body_q=torch.from_numpy(model.body_q).to(gpu))
+    out = view.render(state, cams, rgb=False, depth=True, normals=True, object_ids=True)
+    sc = view.scene
+    cfg = dict(width=40, height=32, worlds=3, cameras_per_world=1, meshes=sc.meshes, instances=sc.instances, counts=None,
+               mode="bodies", bodies=2, body_of_instance=sc.body_of_instance, local_pose=sc.local_pose, scale=sc.scale,
+               body_poses=model.body_q.reshape(3, 2, 7), cameras=cams, fov=1.0)
+    ref = forward(cfg, oracle)
+    rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "newton batch")
+    assert rep["hits"] > 0.1 * rep["pixels"]
