@@ -878,7 +878,8 @@ __global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID 
 // slices with a counter each on its own cache line. A wavefront starts at the slice its index within the XCD
 // picks, takes PERSISTENT_CHUNK (1) tile per counter increment until the slice is used up, then
 // moves on to the XCD's next slice; it leaves once it has found every slice of its range used up, so every
-// wavefront ends after at most COUNTER_SLICES empty increments.
+// wavefront ends after at most COUNTER_SLICES empty increments. The grid is a multiple of 8 workgroups, so every
+// XCD's range has wavefronts to take it.
 #ifndef NGX_PERSISTENT_CHUNK
 #define NGX_PERSISTENT_CHUNK 1
 #endif
@@ -893,6 +894,13 @@ __device__ __forceinline__ void take_tiles(int *counters, const long long total,
         const int s = (home + i) % COUNTER_SLICES;
         const long long lo = count * s / COUNTER_SLICES, n = count * (s + 1) / COUNTER_SLICES - lo;
         int *c = counters + (xcd * COUNTER_SLICES + s) * COUNTER_STRIDE;
+        if (i > 0) {
+            // a slice another wavefront has used up is passed over with a load rather than an increment, so the
+            // end of the launch costs each wavefront one increment instead of one per slice
+            int seen = 0;
+            if (lane_id() == 0) seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__builtin_amdgcn_readlane(seen, 0) >= n) continue;
+        }
         while (true) {
             int t = 0;
             if (lane_id() == 0) t = atomicAdd(c, PERSISTENT_CHUNK);
@@ -1445,14 +1453,20 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
 
 // The persistent grid: as many workgroups as the device holds at once (the kernel's occupancy times the
 // compute units), a multiple of 8 so every XCD gets the same number.
+// No more workgroups than the tiles need (4 wavefronts each), and at least 8, one per XCD.
+static int capped_grid(const int resident, const long long tiles) {
+    const long long need = ((tiles + 3) / 4 + 7) / 8 * 8;
+    return (int)(need < resident ? need : resident);
+}
+
 template <int WALK>
-static int persistent_grid(const int compute_units) {
+static int persistent_grid(const int compute_units, const long long tiles) {
     static const int per_cu = [] {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, render_persistent_kernel<WALK>, 256, 0) != hipSuccess || n < 1) n = 1;
         return n;
     }();
-    return (per_cu * compute_units + 7) / 8 * 8;
+    return capped_grid((per_cu * compute_units + 7) / 8 * 8, tiles);
 }
 
 hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int compute_units) {
@@ -1460,7 +1474,7 @@ hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int comp
     if (a.tile_counters && compute_units > 0) {
         if (a.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.tile_counters);
-#define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_grid<W>(compute_units), 256, 0, stream>>>(a)
+#define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_grid<W>(compute_units, a.total_tiles), 256, 0, stream>>>(a)
         NGX_BY_WALK(a.walk, NGX_LAUNCH)
 #undef NGX_LAUNCH
         return hipGetLastError();
@@ -1484,13 +1498,13 @@ hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream
 #endif
 
 template <int WALK>
-static int rgb_persistent_grid(const int compute_units) {
+static int rgb_persistent_grid(const int compute_units, const long long tiles) {
     static const int per_cu = [] {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rgb_persistent_kernel<WALK>, 256, 0) != hipSuccess || n < 1) n = 1;
         return n;
     }();
-    return (per_cu * compute_units + 7) / 8 * 8;
+    return capped_grid((per_cu * compute_units + 7) / 8 * 8, tiles);
 }
 
 hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_units) {
@@ -1499,7 +1513,7 @@ hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_un
     if (a.r.tile_counters && compute_units > 0) {
         if (a.r.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
-#define NGX_LAUNCH(W) rgb_persistent_kernel<W><<<rgb_persistent_grid<W>(compute_units), 256, 0, stream>>>(a)
+#define NGX_LAUNCH(W) rgb_persistent_kernel<W><<<rgb_persistent_grid<W>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
         NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
 #undef NGX_LAUNCH
         return hipGetLastError();
