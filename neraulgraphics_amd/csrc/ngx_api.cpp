@@ -241,6 +241,8 @@ struct ngx_ctx {
         a.matrices_out = d_matrices;
         a.tmpl_mesh = d_tmpl_mesh;
         a.mesh_bounds = d_mesh_bounds;
+        a.mesh_tris = d_mesh_tris;
+        a.mesh_skip_root = d_mesh_skip_root;
         a.inst = d_inst;
         a.nodes = d_nodes;
         a.wide = d_wide;

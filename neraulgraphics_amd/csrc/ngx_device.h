@@ -52,8 +52,9 @@ static_assert(sizeof(WideNode) == 64, "64-byte wide node");
 struct alignas(16) InstanceRec {
     float4 inv[3];
     int32_t mesh;
-    int32_t tmpl;     // index in the world's instance list: the object ID is OBJECT_INSTANCE + tmpl
-    int32_t pad0, pad1;
+    int32_t tmpl;       // index in the world's instance list: the object ID is OBJECT_INSTANCE + tmpl
+    int32_t mesh_tris;  // the mesh's triangle count and the root of its skip layout, copied here so the walk
+    int32_t skip_root;  // reads them with the instance's rows instead of with loads of their own
 };
 static_assert(sizeof(InstanceRec) == 64, "64-byte instance record");
 

@@ -45,6 +45,8 @@ struct TlasBuildArgs {
 
     const int *tmpl_mesh;      // [P]
     const float *mesh_bounds;  // [meshes][6]
+    const int *mesh_tris;      // [meshes]: triangle counts
+    const int *mesh_skip_root; // [meshes]: root of each mesh's skip layout
 
     InstanceRec *inst;         // [N]
     TlasNode *nodes;           // [2N-1], the reference layout

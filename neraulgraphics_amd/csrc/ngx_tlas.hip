@@ -158,7 +158,8 @@ __device__ void place_instance(const TlasBuildArgs &A, const int w, const int i,
     rec.inv[2] = make_float4(inv[8], inv[9], inv[10], inv[11]);
     rec.mesh = mesh;
     rec.tmpl = i;
-    rec.pad0 = rec.pad1 = 0;
+    rec.mesh_tris = A.mesh_tris[mesh];
+    rec.skip_root = A.mesh_skip_root[mesh];
     A.inst[g] = rec;
 
     const float *lb = A.mesh_bounds + 6 * mesh;

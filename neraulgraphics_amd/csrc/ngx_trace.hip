@@ -381,8 +381,8 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     int shared = 0;  // subtrees given to the shared per-lane walk
-    if (LANES && S.mesh_tris[meta.x] >= S.lane_min_tris) {
-        share_subtree(L, shared, root_mask, S.mesh_skip_root[meta.x]);
+    if (LANES && meta.z >= S.lane_min_tris) {
+        share_subtree(L, shared, root_mask, meta.w);
         walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
         return;
     }
