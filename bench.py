@@ -206,6 +206,8 @@ def bench_falling_bodies(args, torch):
     from neraulgraphics_amd.simulators import MJWarpBatchRenderer
     out = {}
     for nbody in (8, 32):
+        if args.scene_case and not args.scene_case.startswith(f"{nbody}_"):
+            continue
         fb = workloads.falling_bodies(nbody, 4096)
         xpos = torch.from_numpy(fb["geom_xpos"]).cuda()
         xmat = torch.from_numpy(fb["geom_xmat"]).cuda()
@@ -213,6 +215,9 @@ def bench_falling_bodies(args, torch):
         cams = torch.from_numpy(np.tile(fb["camera"].reshape(1, 12), (4096, 1))).cuda()
         light = Light(fb["light"], intensity=1.0, radius=0.0)
         for res, aov in ((64, False), (128, False)) + (((128, True),) if nbody == 8 else ()):
+            key = f"{nbody}_bodies_{res}" + ("_rgb_depth_id" if aov else "")
+            if args.scene_case and key != args.scene_case:
+                continue
             view = MJWarpBatchRenderer(fb["model"], 4096, res, res, lights=[light], fov=fb["fov"])
             k = [0]
 
@@ -225,7 +230,6 @@ def bench_falling_bodies(args, torch):
             reps = sorted(timed_steps(frame, block, 0, torch) for _ in range(args.scene_reps))
             ms = reps[len(reps) // 2] * 1e3
             ref = REFERENCE_README_RGB_DEPTH_ID_MS if aov else REFERENCE_SCENE_MS[(nbody, res)]
-            key = f"{nbody}_bodies_{res}" + ("_rgb_depth_id" if aov else "")
             out[key] = dict(ms=ms, images_per_s=4096 / ms * 1e3, reference_rtx5090_ms=ref, speedup_vs_reference=ref / ms)
             print(f"# falling_bodies {key}: {ms:.4f} ms/frame (reference on an RTX 5090: {ref} ms)", file=sys.stderr, flush=True)
             del view
@@ -310,6 +314,7 @@ def main():
     ap.add_argument("--no-rgb", action="store_true")
     ap.add_argument("--no-scene", action="store_true", help="skip the reference benchmark's falling-bodies scene")
     ap.add_argument("--scene-reps", type=int, default=15)
+    ap.add_argument("--scene-case", default="", help="only this case of the falling-bodies scene, e.g. 8_bodies_128")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 512 MiB three times with a plain copy kernel, to scale HBM counters")
