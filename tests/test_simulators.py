@@ -301,3 +301,25 @@ def test_a_newton_batch_renders_as_the_oracle(gpu, oracle):
     ref = forward(cfg, oracle)
     rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "newton batch")
     assert rep["hits"] > 0.1 * rep["pixels"]
+
+
+def test_the_falling_bodies_scene_is_the_reference_benchmarks():
+    """bench/scene.py's layout: rows of 8 bodies 0.3 apart, even ones 320-triangle balls of radius 0.12 and odd ones
+    boxes of half-size 0.1, falling from 0.3 + 0.02 i (+ up to 0.08) to rest on the ground; rotations stay rotations."""
+    from neraulgraphics_amd import workloads
+    fb = workloads.falling_bodies(12, worlds=5, frames=24)
+    shp, geoms = mujoco_shapes(fb["model"])
+    assert geoms.tolist() == list(range(12))
+    sc = build_scene(shp)
+    assert [m.triangle_count for m in sc.meshes] == [320, 12]
+    assert np.isclose(np.linalg.norm(sc.meshes[0].vertices, axis=1).max(), 0.12, atol=1e-6)
+    assert np.isclose(np.abs(sc.meshes[1].vertices).max(), 0.1)
+    x, z = fb["geom_xpos"][..., 0], fb["geom_xpos"][..., 2]
+    assert np.allclose(x[0, :, :8].mean(axis=0), (np.arange(8) - 3.5) * 0.3, atol=0.04)
+    rest = np.where(np.arange(12) % 2 == 0, 0.12, 0.1)
+    assert np.all(z >= rest - 1e-6) and np.allclose(z[-1], rest)              # at rest by the last frame
+    assert np.all(z[0] <= 0.3 + 0.02 * np.arange(12) + 0.08 + 1e-6)
+    r = fb["geom_xmat"].reshape(-1, 3, 3).astype(np.float64)
+    assert np.allclose(r @ np.swapaxes(r, 1, 2), np.eye(3), atol=1e-6) and np.allclose(np.linalg.det(r), 1.0, atol=1e-6)
+    cam = fb["camera"]
+    assert np.allclose(cam[:, 3], (0.0, -1.9, 1.1)) and np.isclose(fb["fov"], np.radians(45.0))
