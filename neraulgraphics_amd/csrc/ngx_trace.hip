@@ -1338,8 +1338,13 @@ __device__ __forceinline__ void wave_accumulate(const bool has, const int body, 
 constexpr int BWD_WAVES = 4;
 constexpr int MAX_BODIES = 256;
 
+// 5 workgroups of 4 wavefronts per compute unit (at most 100 VGPRs; posed_bodies forward + backward 746k -> 794k
+// images/s; 6 gives 781k)
+#ifndef NGX_BWD_MIN_BLOCKS
+#define NGX_BWD_MIN_BLOCKS 5
+#endif
 template <int WALK>
-__global__ __launch_bounds__(64 * BWD_WAVES) void depth_backward_kernel(const DepthBackwardArgs A) {
+__global__ __launch_bounds__(64 * BWD_WAVES, NGX_BWD_MIN_BLOCKS) void depth_backward_kernel(const DepthBackwardArgs A) {
     extern __shared__ float acc_all[];  // [BWD_WAVES][bodies][7]
     NGX_PAIR_LDS(L);
     const int view = blockIdx.x / A.parts, part = blockIdx.x % A.parts;
