@@ -323,3 +323,49 @@ def test_the_falling_bodies_scene_is_the_reference_benchmarks():
     assert np.allclose(r @ np.swapaxes(r, 1, 2), np.eye(3), atol=1e-6) and np.allclose(np.linalg.det(r), 1.0, atol=1e-6)
     cam = fb["camera"]
     assert np.allclose(cam[:, 3], (0.0, -1.9, 1.1)) and np.isclose(fb["fov"], np.radians(45.0))
+
+
+@pytest.mark.gpu
+def test_geom_frame_steps_replay_from_graphs(gpu):
+    """The MuJoCo Warp step (geom frames into the top-level build, then the RGB picture and depth from persistent
+    launches) captured in HIP graphs, one per frame, replayed in turn with direct steps between the replays: every
+    replay gives the images the same step gives launched directly."""
+    import torch
+    from neraulgraphics_amd import workloads
+    fb = workloads.falling_bodies(8, worlds=16, frames=3)
+    sim = MJWarpBatchRenderer(fb["model"], 16, 32, 32, lights=[ng.Light(fb["light"], radius=0.0)], fov=fb["fov"])
+    cams = torch.from_numpy(np.tile(fb["camera"].reshape(1, 12), (16, 1))).to(gpu)
+    frames = [SimpleNamespace(geom_xpos=torch.from_numpy(fb["geom_xpos"][f]).to(gpu),
+                              geom_xmat=torch.from_numpy(fb["geom_xmat"][f]).to(gpu)) for f in range(3)]
+
+    def step(k):
+        return sim.render(frames[k], cams, rgb=True, depth=True, zero_copy=True)
+
+    want = []
+    for k in range(3):
+        out = step(k)
+        torch.cuda.synchronize()
+        want.append({n: out[n].clone() for n in ("rgb", "depth")})
+    assert not torch.equal(want[0]["depth"], want[2]["depth"])
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step(0)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graphs = []
+    for k in range(3):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = step(k)
+        graphs.append((g, out))
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for k in (2, 0, 1):
+            g, out = graphs[k]
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out["depth"], want[k]["depth"]), (rep, k)
+            assert torch.equal(out["rgb"], want[k]["rgb"]), (rep, k)
+            step((k + 1) % 3)
+            torch.cuda.synchronize()
