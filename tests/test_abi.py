@@ -155,3 +155,21 @@ def test_skip_layout_is_an_ordered_walk_of_the_reference_tree(mesh):
             after = pos[i] + size[i]
             expect = want[after] if after < len(want) else -1
             assert int(skip["escape"][i][o]) == expect, f"octant {o}: node {i}'s escape"
+
+
+def test_a_missing_library_fails_loudly():
+    """No CPU stand-in: without the HIP library every entry point raises, here BatchRenderer's constructor."""
+    import os
+    import subprocess
+    import sys
+    code = ("import neraulgraphics_amd as ng\n"
+            "try:\n"
+            "    ng.BatchRenderer(8, 8, 1, meshes=[ng.make_box()], instances=[ng.Instance(0)])\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    print('no error')\n")
+    env = dict(os.environ, NGX_LIBRARY="/nonexistent/libngx.so")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120,
+                         cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert "raised:" in out.stdout and "is not built" in out.stdout, out.stdout + out.stderr
