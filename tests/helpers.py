@@ -2,7 +2,7 @@
 
 Object IDs and hit/miss are compared exactly, except at rays where two surfaces are hit at the same
 distance (to 1e-5 relative): there the reference keeps whichever it met first, which depends on the order
-a walk meets them. Depth, normal and range are compared to 1e-5 relative (the north-star tolerance);
+a walk meets them. Depth, normal and range are compared to 1e-5 relative (the path's stated tolerance);
 in practice the backend and the oracle compute the same IEEE singles, and the tests report how many
 values are bit-identical."""
 import numpy as np

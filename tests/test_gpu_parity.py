@@ -510,7 +510,49 @@ def test_pose_gradient_matches_fp64_closed_form_and_differences(gpu, oracle):
     rel_cf = np.linalg.norm(g_gpu - g_cf) / np.linalg.norm(g_cf)
     _say(f"|closed form - FD(1e-6)| / |FD| = {pin:.2e}; |GPU - FD(1e-4)| / |FD| = {rel:.2e}; |GPU - closed form| = {rel_cf:.2e}")
     assert pin < 1e-7  # the FP64 closed form is the derivative (a wrong sign or term would be O(1))
-    assert rel < 1e-3  # the north-star check: FP32 GPU gradient against FP64 central differences, step 1e-4
+    assert rel < 1e-3  # the bar: FP32 GPU gradient within 1e-3 of FP64 central differences, step 1e-4
+    assert rel_cf < 1e-4
+
+
+def test_range_gradient_matches_fp64_closed_form_and_differences(gpu, oracle):
+    """cast_range's backward (range_backward_kernel): dL/d(body poses) of L = 0.5 sum (range - target)^2 over rays
+    that hit in both, against the oracle's FP64 closed form and central differences through the same hit triangles."""
+    import torch
+    from neraulgraphics_amd.autograd import cast_range
+    _say("range backward: posed_bodies, 8 worlds x 2048 rays")
+    cfg = workloads.make("posed_bodies", worlds=8)
+    r = workloads.make_renderer(cfg)
+    W, R = cfg["worlds"], 2048
+    rng = np.random.default_rng(21)
+    eye = cfg["cameras"].reshape(W, 3, 4)[:, :, 3]
+    aim = cfg["body_poses"][np.arange(W)[:, None], rng.integers(0, cfg["bodies"], (W, R)), :3]
+    origins = np.repeat(eye[:, None, :], R, axis=1).astype(np.float32)
+    dirs = (aim + rng.normal(0, 0.3, aim.shape) - origins).astype(np.float32)
+    r.set_body_poses(cfg["target_poses"])
+    target = r.cast_rays(origins, dirs, 100.0)["distance"].copy()
+    poses = torch.from_numpy(cfg["body_poses"]).cuda().requires_grad_(True)
+    dist = cast_range(r, poses, torch.from_numpy(origins).cuda(), torch.from_numpy(dirs).cuda(), 100.0)
+    tgt = torch.from_numpy(target).cuda()
+    both = torch.isfinite(dist) & torch.isfinite(tgt)
+    diff = torch.where(both, dist - tgt, torch.zeros_like(dist))
+    (0.5 * (diff * diff).sum()).backward()
+    g_gpu = poses.grad.cpu().numpy().astype(np.float64)
+    assert int(both.sum()) > W * R // 4
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], origins, dirs, 100.0)
+    d32 = dirs.reshape(-1, 3)
+    unit = (d32 * (np.float32(1.0) / np.sqrt((d32 * d32).sum(axis=1, dtype=np.float32)))[:, None]).astype(np.float64)
+    args = (origins.reshape(-1, 3).astype(np.float64), unit, diff.detach().cpu().numpy().reshape(-1).astype(np.float64),
+            np.ones(W * R), ref["hit_inst"].reshape(-1), ref["hit_tri"].reshape(-1), np.repeat(np.arange(W), R).astype(np.int32),
+            st["world_off"], cfg["bodies"], cfg["body_of_instance"], cfg["local_pose"], cfg["scale"], cfg["body_poses"],
+            st["meshes"]["tris"])
+    g_cf = oracle.pose_grad(0, 0.0, *args)
+    g_fd = oracle.pose_grad(1, 1e-4, *args)
+    rel = np.linalg.norm(g_gpu - g_fd) / np.linalg.norm(g_fd)
+    rel_cf = np.linalg.norm(g_gpu - g_cf) / np.linalg.norm(g_cf)
+    _say(f"|GPU - FD(1e-4)| / |FD| = {rel:.2e}; |GPU - closed form| / |closed form| = {rel_cf:.2e}")
+    assert rel < 1e-3
     assert rel_cf < 1e-4
 
 
