@@ -653,9 +653,30 @@ __global__ __launch_bounds__(256) void big_write_kernel(const TlasBuildArgs A, c
 // A query's rays come in the caller's order; a LiDAR's consecutive rays fan out over many degrees, so a
 // wavefront of 64 of them shares little of either tree. Each world's rays are put in a stable order of a
 // 12-bit direction cell (octahedral map of the direction, 64 x 64 cells, Morton order) by one workgroup per
-// world: three passes of the same 4-bit stable radix sort, over chunks of 1024 rays. The tracer then gives
-// a wavefront 64 consecutive rays of that order; results still go to each ray's own place.
+// world: a stable LSD radix sort over chunks of 1024 rays, in two passes of 6-bit digits for up to 65,536 rays a
+// world (the counts of every chunk and digit fit the workgroup's LDS) and three of 4-bit digits above that; both
+// give the same stable order. The tracer then gives a wavefront 64 consecutive rays of that order; results still
+// go to each ray's own place.
 constexpr int RAY_ORDER_MAX_CHUNKS = RAY_ORDER_MAX_RAYS / 1024;
+constexpr int RAY_ORDER_WIDE_MAX_RAYS = 65536;
+
+// tile_digit for a BITS-bit digit: the digit of each key of a 1024-key tile, its rank among the equal digits of
+// its wave, and how many of its wave share the digit (valid for the rank-0 lane).
+template <int BITS>
+__device__ __forceinline__ void tile_digit_bits(const bool has, const uint32_t k, const int shift, uint32_t &digit, int &rank,
+                                                int &count) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    digit = (k >> shift) & ((1u << BITS) - 1u);
+    uint64_t peers = ballot(has);
+    for (int bit = 0; bit < BITS; ++bit) {
+        const bool set = (digit >> bit) & 1u;
+        const uint64_t vote = ballot(has && set);
+        peers &= set ? vote : ~vote;
+    }
+    rank = popc64(peers & below);
+    count = popc64(peers);
+}
 
 __device__ __forceinline__ uint32_t spread6(uint32_t v) {
     uint32_t r = 0;
@@ -678,10 +699,12 @@ __device__ __forceinline__ uint32_t direction_cell(const float x, const float y,
     return (spread6(qu) << 1) | spread6(qv);
 }
 
+template <int BITS, int MAX_CHUNKS>
 __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, const int R, uint32_t *ka, uint32_t *kb, int *va, int *vb) {
-    constexpr int NW = 16;
-    __shared__ int counts[16 * RAY_ORDER_MAX_CHUNKS];
-    __shared__ int hist[16 * NW];
+    constexpr int NW = 16, NB = 1 << BITS, PASSES = 12 / BITS;
+    static_assert(PASSES * BITS == 12 && NB * NW <= 1024, "12-bit keys in whole digits, one histogram entry a thread");
+    __shared__ int counts[NB * MAX_CHUNKS];
+    __shared__ int hist[NB * NW];
     __shared__ int wave_tot[NW];
     __shared__ int carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -695,19 +718,19 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
     __syncthreads();
     uint32_t *ks = ka, *kd = kb;
     int *vs = va, *vd = vb;
-    for (int pass = 0; pass < 3; ++pass) {
-        const int shift = 4 * pass;
+    for (int pass = 0; pass < PASSES; ++pass) {
+        const int shift = BITS * pass;
         for (int c = 0; c < chunks; ++c) {
             const int i = c * 1024 + tid;
             const bool has = i < R;
-            if (tid < 16 * NW) hist[tid] = 0;
+            if (tid < NB * NW) hist[tid] = 0;
             __syncthreads();
             uint32_t digit;
             int rank, cnt;
-            tile_digit(has, has ? ks[base + i] : 0u, shift, digit, rank, cnt);
+            tile_digit_bits<BITS>(has, has ? ks[base + i] : 0u, shift, digit, rank, cnt);
             if (has && rank == 0) hist[digit * NW + wave] = cnt;
             __syncthreads();
-            if (tid < 16) {
+            if (tid < NB) {
                 int sum = 0;
                 for (int wv = 0; wv < NW; ++wv) sum += hist[tid * NW + wv];
                 counts[tid * chunks + c] = sum;
@@ -715,7 +738,7 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
             __syncthreads();
         }
         // exclusive scan of the counts in digit-major order
-        const int total = 16 * chunks;
+        const int total = NB * chunks;
         if (tid == 0) carry = 0;
         __syncthreads();
         for (int b0 = 0; b0 < total; b0 += 1024) {
@@ -740,14 +763,14 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
             const bool has = i < R;
             const uint32_t k = has ? ks[base + i] : 0u;
             const int v = has ? vs[base + i] : 0;
-            if (tid < 16 * NW) hist[tid] = 0;
+            if (tid < NB * NW) hist[tid] = 0;
             __syncthreads();
             uint32_t digit;
             int rank, cnt;
-            tile_digit(has, k, shift, digit, rank, cnt);
+            tile_digit_bits<BITS>(has, k, shift, digit, rank, cnt);
             if (has && rank == 0) hist[digit * NW + wave] = cnt;
             __syncthreads();
-            block_exclusive_scan(hist, 16 * NW, wave_tot);
+            block_exclusive_scan(hist, NB * NW, wave_tot);
             if (has) {
                 const int dest = counts[digit * chunks + c] + hist[digit * NW + wave] - hist[digit * NW] + rank;
                 kd[base + dest] = k;
@@ -796,8 +819,14 @@ hipError_t launch_ray_order(const float *dirs, const int rays_per_world, const i
                             hipStream_t stream) {
     if (rays_per_world <= 0 || rays_per_world > RAY_ORDER_MAX_RAYS || worlds <= 0) return hipErrorInvalidValue;
     const size_t n = (size_t)rays_per_world * worlds;
-    ray_order_kernel<<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals, vals + n);
-    *order = vals + n;  // three passes end in the second buffers
+    if (rays_per_world <= RAY_ORDER_WIDE_MAX_RAYS) {
+        ray_order_kernel<6, RAY_ORDER_WIDE_MAX_RAYS / 1024><<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals,
+                                                                                       vals + n);
+        *order = vals;  // two passes end in the first buffers
+    } else {
+        ray_order_kernel<4, RAY_ORDER_MAX_CHUNKS><<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals, vals + n);
+        *order = vals + n;  // three passes end in the second buffers
+    }
     return hipGetLastError();
 }
 

@@ -172,6 +172,22 @@ def test_ray_queries_match_the_oracle(gpu, oracle, walk, sort_rays):
     assert rep["hits"] > 0
 
 
+def test_ray_order_of_many_rays_a_world(gpu, oracle):
+    """Above 65,536 rays a world the direction order sorts with 4-bit digits in three passes (below, 6-bit digits in
+    two): 2 worlds of 70,000 rays in direction order against the oracle."""
+    _say("ray parity: 2 worlds x 70,000 rays, hybrid walk, direction order")
+    cfg = workloads.make("lidar_sweeps", worlds=2, rays_per_world=70016)
+    r = workloads.make_renderer(cfg)
+    r.set_trace_config(**WALKS["hybrid"], sort_rays=True)
+    o, d = cfg["origins"][:, :70000], cfg["directions"][:, :70000]
+    out = r.cast_rays(o, d, cfg["max_distance"])
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], o, d, cfg["max_distance"])
+    rep = compare_rays(out, ref, "70,000 rays a world")
+    assert rep["hits"] > 0
+
+
 @pytest.mark.parametrize("walk", list(WALKS))
 def test_rays_of_uneven_worlds_per_wave(gpu, oracle, walk):
     _say(f"ray parity: 37 rays per world, so waves straddle worlds, {walk} walk")
