@@ -858,9 +858,10 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
     if (A.object_id) A.object_id[idx] = object;
 }
 
-// Workgroups per compute unit the hybrid render kernels are built for (their registers follow: 6 gives 80 VGPRs).
+// Workgroups per compute unit the hybrid render kernels are built for (their registers follow: 7 gives at most 72
+// VGPRs, with more spilled; body_worlds 10.49 ms at 6, 10.20 ms at 7).
 #ifndef NGX_HYBRID_MIN_BLOCKS
-#define NGX_HYBRID_MIN_BLOCKS 6
+#define NGX_HYBRID_MIN_BLOCKS 7
 #endif
 
 // One tile per wavefront, tiles dealt by the dispatcher.
