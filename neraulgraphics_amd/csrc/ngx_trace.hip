@@ -1076,14 +1076,14 @@ __device__ __forceinline__ float srgb_channel(const float v) {
 }
 
 // Workgroups per compute unit the RGB kernels are compiled for: 4 (16 wavefronts, at most 128 VGPRs) for the
-// packet and per-lane walks, 5 (at most 102 VGPRs, a few more spills) for the hybrid walk, whose scenes have
-// more to hide (body_worlds RGB 27.9 -> 26.2 ms; the 8-body falling-bodies scene, packet walk, is 3.45 ms at 4
-// and 3.87 ms at 5).
+// packet and per-lane walks, 7 (at most 72 VGPRs, many more spills) for the hybrid walk, whose scenes have more
+// latency to hide (body_worlds RGB: 25.7 ms at 5, 24.3 at 6, 23.5 at 7, 33.0 at 8; the 8-body falling-bodies scene,
+// packet walk, is 3.1 ms at 4 and 4.2 at 6).
 #ifndef NGX_RGB_MIN_BLOCKS
 #define NGX_RGB_MIN_BLOCKS 4
 #endif
 #ifndef NGX_RGB_MIN_BLOCKS_HYBRID
-#define NGX_RGB_MIN_BLOCKS_HYBRID 5
+#define NGX_RGB_MIN_BLOCKS_HYBRID 7
 #endif
 #define NGX_RGB_BLOCKS(W) ((W) == WALK_HYBRID ? NGX_RGB_MIN_BLOCKS_HYBRID : NGX_RGB_MIN_BLOCKS)
 template <int WALK>
