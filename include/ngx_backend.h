@@ -9,7 +9,7 @@
  * Conventions (the reference's, with the differences named):
  *  - Errors: every call returns an ngx_status; NGX_INVALID_ARGUMENT is the reference's
  *    std::invalid_argument (bad input, checked before anything changes, so a rejected call changes
- *    nothing), NGX_DEVICE_ERROR its CudaError (src/include/neraulgraphics/renderer.h:32-39).
+ *    nothing), NGX_DEVICE_ERROR its CudaError (include/neraulgraphics/renderer.h:30-39).
  *    ngx_last_error() gives the message of the calling thread's last failure.
  *  - Memory: NGX_MEM_HOST arrays are validated as the reference validates host arrays
  *    (src/batch.cpp:91-192); NGX_MEM_DEVICE arrays (HIP device pointers) are trusted, as there.
@@ -201,11 +201,14 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
 /* No counterpart in the reference: dL/d(body poses) of a depth render, through each pixel's fixed hit
  * triangle. Device arrays: body_poses (worlds x B x 7, the poses to differentiate at; the top-level trees
  * are rebuilt from them first), cams (views x 12), depth (views x H x W, the forward's output: it bounds
- * the re-trace and marks misses), grad_depth (dL/d depth), grad_out (worlds x B x 7, overwritten). */
+ * the re-trace and marks misses), grad_depth (dL/d depth), grad_out (worlds x B x 7, overwritten).
+ * The gradient is summed per body in LDS: a binding of more than NGX_BACKWARD_MAX_BODIES bodies per world is
+ * refused with NGX_UNSUPPORTED, before anything changes. */
+#define NGX_BACKWARD_MAX_BODIES 256
 ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov,
                                    const float *depth, const float *grad_depth, float *grad_out, void *stream);
 
-/* No counterpart in the reference: the same for a ray query's distances. Device arrays. */
+/* No counterpart in the reference: the same for a ray query's distances. Device arrays; the same body limit. */
 ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const float *origins, const float *directions,
                                    int32_t rays_per_world, float max_distance, const float *distance,
                                    const float *grad_distance, float *grad_out, void *stream);
@@ -213,7 +216,9 @@ ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const 
 /* Backend::copy_to_host (src/backend.h:54). Synchronous. */
 ngx_status ngx_copy_to_host(ngx_ctx *ctx, const void *src, void *dst, size_t bytes);
 
-/* Waits for everything queued on the context's last stream. */
+/* Waits for everything queued on the context's last stream. Returns NGX_DEVICE_ERROR when a tree walk since the
+ * last call ran out of stack (a tree deeper than the walks hold: 63 levels below a world's or a mesh's root; the
+ * builds keep within that, so this marks a corrupted tree), in which case those results may be wrong. */
 ngx_status ngx_synchronize(ngx_ctx *ctx);
 
 /* Backend::last_kernel_ms (src/backend.h:83): device time of the last render or ray query's kernel, in ms,

@@ -59,6 +59,21 @@ def _as_input(a, what, count=None, dtype=np.float32):
     return arr.ctypes.data_as(ctypes.c_void_p), _lib.NGX_MEM_HOST, arr
 
 
+def _device_operand(a, what, count, device_id):
+    """`a` as a contiguous float32 torch tensor of `count` values on GPU `device_id`, for the entry points that take
+    device pointers only (the pose backward); anything else is refused with ValueError before any device work."""
+    t = _torch()
+    if not isinstance(a, t.Tensor):
+        raise ValueError(f"{what} must be a torch tensor on the GPU, got {type(a).__name__}")
+    if not a.is_cuda or a.device.index != device_id:
+        raise ValueError(f"{what} must be on the renderer's GPU (cuda:{device_id}), got {a.device}")
+    if a.dtype != t.float32:
+        raise ValueError(f"{what} must be float32, got {a.dtype}")
+    if a.numel() != count:
+        raise ValueError(f"{what} has {a.numel()} values, but {count} are needed")
+    return a.contiguous()
+
+
 def _serialized(method):
     """Runs the method under the renderer's lock: the backend context is not thread-safe."""
     @functools.wraps(method)
@@ -203,6 +218,7 @@ class BatchRenderer:
     view_count = property(lambda self: self._worlds * self._cpw)
     instances_per_world = property(lambda self: self._per_world)
     device = property(lambda self: "cuda")
+    device_id = property(lambda self: self._device_id)
     bodies_per_world = property(lambda self: self._bodies)
 
     @property
@@ -409,27 +425,45 @@ class BatchRenderer:
     # ---- gradients (no counterpart in the reference)
     @_serialized
     def pose_backward_depth(self, body_poses, cameras, fov, depth, grad_depth):
-        """dL/d(body_poses) given dL/d(depth) of a render from these poses and cameras (all torch device tensors)."""
+        """dL/d(body_poses) given dL/d(depth) of a render from these poses and cameras. Every argument is a float32
+        torch tensor on the renderer's GPU: body_poses (world_count, bodies_per_world, 7), cameras (views, 3, 4),
+        depth and grad_depth (views, H, W); anything else raises ValueError."""
         t = _torch()
         if self._bodies is None:
             raise ValueError("the pose backward needs bind_bodies first")
-        grad = t.empty((self._worlds, self._bodies, 7), dtype=t.float32, device=body_poses.device)
-        args = [x.contiguous() for x in (body_poses, cameras, depth, grad_depth)]
-        _lib.check(self._lib.ngx_pose_backward_depth(self._h, args[0].data_ptr(), args[1].data_ptr(), float(fov),
-                                                     args[2].data_ptr(), args[3].data_ptr(), grad.data_ptr(), _stream()))
+        views, dev = self.view_count, self._device_id
+        pixels = views * self._height * self._width
+        poses = _device_operand(body_poses, "body_poses", self._worlds * self._bodies * 7, dev)
+        cams = _device_operand(cameras, "cameras", views * 12, dev)
+        d = _device_operand(depth, "depth", pixels, dev)
+        gd = _device_operand(grad_depth, "grad_depth", pixels, dev)
+        grad = t.empty((self._worlds, self._bodies, 7), dtype=t.float32, device=poses.device)
+        _lib.check(self._lib.ngx_pose_backward_depth(self._h, poses.data_ptr(), cams.data_ptr(), float(fov), d.data_ptr(),
+                                                     gd.data_ptr(), grad.data_ptr(), _stream()))
         return grad
 
     @_serialized
     def pose_backward_range(self, body_poses, origins, directions, max_distance, distance, grad_distance):
+        """dL/d(body_poses) given dL/d(distance) of a ray query from these poses. Every argument is a float32 torch
+        tensor on the renderer's GPU: body_poses (world_count, bodies_per_world, 7), origins and directions
+        (world_count, rays_per_world, 3), distance and grad_distance (world_count, rays_per_world)."""
         t = _torch()
         if self._bodies is None:
             raise ValueError("the pose backward needs bind_bodies first")
-        grad = t.empty((self._worlds, self._bodies, 7), dtype=t.float32, device=body_poses.device)
-        args = [x.contiguous() for x in (body_poses, origins, directions, distance, grad_distance)]
-        r = args[1].numel() // (3 * self._worlds)
-        _lib.check(self._lib.ngx_pose_backward_range(self._h, args[0].data_ptr(), args[1].data_ptr(), args[2].data_ptr(), int(r),
-                                                     float(max_distance), args[3].data_ptr(), args[4].data_ptr(),
-                                                     grad.data_ptr(), _stream()))
+        w, dev = self._worlds, self._device_id
+        n = origins.numel() if isinstance(origins, t.Tensor) else -1
+        if n < 0 or n % (3 * w) != 0:
+            raise ValueError(f"origins must hold 3 floats for each ray of each of the {w} worlds: (world_count, rays_per_world, 3)")
+        r = n // (3 * w)
+        poses = _device_operand(body_poses, "body_poses", w * self._bodies * 7, dev)
+        o = _device_operand(origins, "origins", 3 * w * r, dev)
+        dr = _device_operand(directions, "directions", 3 * w * r, dev)
+        dist = _device_operand(distance, "distance", w * r, dev)
+        gd = _device_operand(grad_distance, "grad_distance", w * r, dev)
+        grad = t.empty((w, self._bodies, 7), dtype=t.float32, device=poses.device)
+        _lib.check(self._lib.ngx_pose_backward_range(self._h, poses.data_ptr(), o.data_ptr(), dr.data_ptr(), int(r),
+                                                     float(max_distance), dist.data_ptr(), gd.data_ptr(), grad.data_ptr(),
+                                                     _stream()))
         return grad
 
     # ---- build read-back, for checking the top-level build

@@ -13,6 +13,8 @@
 #include "ngx_host.h"
 #include "ngx_kernels.h"
 
+static_assert(ngx::BACKWARD_MAX_BODIES == NGX_BACKWARD_MAX_BODIES, "the header states the backward's body limit");
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -178,11 +180,12 @@ struct ngx_ctx {
     int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000, sort_rays = 1;
     int compute_units = 0;              // sizes the persistent launch
     int *d_tile_counters = nullptr;
+    int *d_overflow = nullptr;          // set by a walk whose stack would pass its capacity; read by ngx_synchronize
 
     ~ngx_ctx() {
         int *ints[] = {d_world_off, d_tmpl_mesh, d_roots, d_body_of_instance, d_dbg_sorted_ids, d_blas_parent, d_blas_wide_node, d_wide_parent,
                        d_tri_verts, d_node_parent, d_node_wide, d_node_arrivals,
-                       d_geom_of_instance, d_tile_counters, d_mesh_skip_root, d_mesh_tris};
+                       d_geom_of_instance, d_tile_counters, d_mesh_skip_root, d_mesh_tris, d_overflow};
         for (int *p : ints) if (p) (void)hipFree(p);
         float *floats[] = {d_vertices, d_mesh_bounds, d_matrices, d_matrix_stage, d_local_pose, d_scale, d_local_rot_scale, d_body_stage,
                            d_geom_scale, d_geom_stage, d_depth, d_normal, d_cam_stage, d_ray_o, d_ray_d, d_hit_dist, d_hit_normal, d_partial, d_dbg_aabb,
@@ -221,6 +224,7 @@ struct ngx_ctx {
         s.mesh_skip_root = d_mesh_skip_root;
         s.mesh_tris = d_mesh_tris;
         s.lane_min_tris = lane_min_tris;
+        s.overflow = d_overflow;
         return s;
     }
 
@@ -419,6 +423,8 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             c.compute_units = std::max(1, prop.multiProcessorCount);
         }
         device_alloc(c.d_tile_counters, (size_t)ngx::TILE_COUNTER_INTS);
+        device_alloc(c.d_overflow, 1);
+        HIP_CHECK(hipMemset(c.d_overflow, 0, sizeof(int)));
         HIP_CHECK(hipEventCreate(&c.ev0));
         HIP_CHECK(hipEventCreate(&c.ev1));
 
@@ -427,6 +433,18 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             ngx::host::MeshInput mi{s.mesh_count, s.vertices, s.vertex_offsets, s.indices, s.index_offsets, s.normals, s.mesh_has_normals};
             ngx::host::BuiltMeshes built;
             ngx::host::build_meshes(mi, built);
+            {
+                // the packet walk keeps one stack entry per level of a bottom-level tree: the build stops at 60
+                int deepest = 0;
+                for (size_t k = 0; k < built.node_parent.size(); ++k) {
+                    int d = 0;
+                    for (int p = built.node_parent[k]; p >= 0 && d <= ngx::WALK_STACK; p = built.node_parent[p]) ++d;
+                    deepest = std::max(deepest, d);
+                }
+                if (deepest >= ngx::WALK_STACK)
+                    fail(NGX_DEVICE_ERROR, "a bottom-level tree is " + std::to_string(deepest) + " levels deep; the walks hold " +
+                                               std::to_string(ngx::WALK_STACK - 1));
+            }
             upload(c.d_blas_wide, built.wide.data(), built.wide.size() ? built.wide.size() : 0);
             if (!c.d_blas_wide) device_alloc(c.d_blas_wide, 1);
             upload(c.d_blas_parent, built.wide_parent.data(), built.wide_parent.size());
@@ -961,6 +979,9 @@ ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const 
     return guarded([&] {
         require(ctx != nullptr, "context is null");
         require(ctx->bound && ctx->bodies > 0, "the pose backward needs bodies: call bind_bodies first");
+        if (ctx->bodies > ngx::BACKWARD_MAX_BODIES)
+            fail(NGX_UNSUPPORTED, "the pose backward takes at most " + std::to_string(ngx::BACKWARD_MAX_BODIES) +
+                                      " bodies per world; the binding has " + std::to_string(ctx->bodies));
         require(body_poses && cams12 && depth && grad_depth && grad_out, "device arrays must not be null");
         require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
         ctx->set_device();
@@ -1006,8 +1027,12 @@ ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const 
     return guarded([&] {
         require(ctx != nullptr, "context is null");
         require(ctx->bound && ctx->bodies > 0, "the pose backward needs bodies: call bind_bodies first");
+        if (ctx->bodies > ngx::BACKWARD_MAX_BODIES)
+            fail(NGX_UNSUPPORTED, "the pose backward takes at most " + std::to_string(ngx::BACKWARD_MAX_BODIES) +
+                                      " bodies per world; the binding has " + std::to_string(ctx->bodies));
         require(body_poses && origins && directions && distance && grad_distance && grad_out, "device arrays must not be null");
         require(rays_per_world >= 0, "rays_per_world must be 0 or more");
+        require(max_distance > 0.0f, "max_distance must be positive, got " + std::to_string(max_distance));
         ctx->set_device();
         const hipStream_t s = static_cast<hipStream_t>(stream);
         const int chunks = std::max(1, std::min(64, rays_per_world / 4096));
@@ -1054,6 +1079,13 @@ ngx_status ngx_synchronize(ngx_ctx *ctx) {
         require(ctx != nullptr, "context is null");
         ctx->set_device();
         HIP_CHECK(hipDeviceSynchronize());
+        int overflow = 0;
+        HIP_CHECK(hipMemcpy(&overflow, ctx->d_overflow, sizeof(int), hipMemcpyDeviceToHost));
+        if (overflow) {
+            HIP_CHECK(hipMemset(ctx->d_overflow, 0, sizeof(int)));
+            fail(NGX_DEVICE_ERROR, "a tree walk ran out of stack (a tree deeper than " + std::to_string(ngx::WALK_STACK - 1) +
+                                       " levels below a root): the images or ray hits since the last synchronize may be wrong");
+        }
     });
 }
 
@@ -1087,12 +1119,12 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
         const ngx_trace_config &c = *config;
         require(c.tile_w == 8 || c.tile_w == 16 || c.tile_w == 32 || c.tile_w == 64, "tile_w must be 8, 16, 32 or 64");
         require(c.pair_limit >= 1 && c.pair_limit <= 64, "pair_limit must be 1 to 64");
+        require(c.lane_min_triangles >= 0, "lane_min_triangles must not be negative");
         ctx->tile_w = c.tile_w;
         ctx->xcd_remap = c.xcd_remap ? 1 : 0;
         ctx->packet = c.packet ? 1 : 0;
         ctx->pair_limit = c.pair_limit;
         ctx->persistent = c.persistent ? 1 : 0;
-        require(c.lane_min_triangles >= 0, "lane_min_triangles must not be negative");
         ctx->lane_min_tris = c.lane_min_triangles;
         ctx->sort_rays = c.sort_rays ? 1 : 0;
     });

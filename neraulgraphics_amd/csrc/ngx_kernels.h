@@ -82,7 +82,13 @@ struct TraceScene {
     const int *mesh_skip_root; // per mesh: its root in skip
     const int *mesh_tris;      // per mesh: triangle count
     int lane_min_tris;         // packet walk: meshes of at least this many triangles are walked lane by lane
+    int *overflow;             // set to 1 by a walk whose stack or trail would pass its capacity (a tree deeper than
+                               // the walks allow); the walk then drops what does not fit, so its images are wrong
 };
+
+// The walks' depth limits: a wave-uniform stack of WALK_STACK entries per level for the packet walk, a 64-bit trail per
+// level for the per-lane walk.
+constexpr int WALK_STACK = 64;
 
 struct RenderArgs {
     TraceScene s;
@@ -178,6 +184,9 @@ struct PoseGradFinishArgs {
     const float *body_poses;   // [worlds][bodies][7]
     float *grad;               // [worlds][bodies][7]
 };
+
+// The pose backward sums gradients per body in LDS: at most this many bodies per world.
+constexpr int BACKWARD_MAX_BODIES = 256;
 
 // Builds every world of at most WORKGROUP_BUILD_MAX instances (max_count: the largest such world), one
 // workgroup per world; larger worlds are skipped.

@@ -91,8 +91,8 @@ struct PairLds {
     int share[128];
     int leaf_first[128], leaf_ray_count[128];
     // the packet walk's two wave-uniform stacks (top level, bottom level): node reference and lane mask
-    int st_ref[2][64];
-    uint32_t st_lo[2][64], st_hi[2][64];
+    int st_ref[2][WALK_STACK];
+    uint32_t st_lo[2][WALK_STACK], st_hi[2][WALK_STACK];
 };
 
 // Orders a wavefront's LDS accesses: a wavefront's LDS operations are performed in order, so a
@@ -112,12 +112,18 @@ __device__ __forceinline__ int rank_in(const uint64_t mask) {
 struct LdsStack {
     int *ref;
     uint32_t *lo, *hi;
+    int *overflow;
     int sp;
-    __device__ __forceinline__ LdsStack(PairLds *L, const int level) : ref(L->st_ref[level]), lo(L->st_lo[level]), hi(L->st_hi[level]), sp(0) {}
-    // Depth bounds keep sp below 64 (a top-level tree has fewer than 64 levels below a world's root, a bottom-level
-    // tree at most 60); the guard only keeps a malformed tree from writing past the stack.
+    __device__ __forceinline__ LdsStack(PairLds *L, const int level, int *flag)
+        : ref(L->st_ref[level]), lo(L->st_lo[level]), hi(L->st_hi[level]), overflow(flag), sp(0) {}
+    // Depth bounds keep sp below WALK_STACK (a top-level tree has fewer than 64 levels below a world's root, and the
+    // host refuses bottom-level trees deeper than 60). A tree that breaks them anyway loses the entry that does not
+    // fit, and the walk reports it through the scene's overflow flag, which the host turns into an error.
     __device__ __forceinline__ void push(const int r, const uint64_t m) {
-        if (sp >= 64) return;
+        if (sp >= WALK_STACK) {
+            if (lane_id() == 0) *overflow = 1;
+            return;
+        }
         if (lane_id() == 0) {
             ref[sp] = r;
             lo[sp] = (uint32_t)m;
@@ -393,7 +399,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
         return;
     }
-    LdsStack st(L, 1);
+    LdsStack st(L, 1, S.overflow);
     int cur = root_ref;
     uint64_t cm = root_mask;
     while (true) {
@@ -479,7 +485,7 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
     // One entry is pushed per level at most. Below a world's root every level fixes at least one more bit
     // of a Morton code (30 bits) or, among equal codes, of the leaves' positions (at most ceil(log2 n) + 1
     // bits over n consecutive positions): fewer than 64 levels for any n < 2^31, so 64 entries do.
-    LdsStack st(L, 0);
+    LdsStack st(L, 0, S.overflow);
     int cur = root;
     uint64_t cm = live;
     while (true) {
@@ -537,6 +543,7 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
     int state = DONE, level = 0;
     int cur_t = -1, cur_b = -1;
     uint64_t bits_t = 0, side_t = 0, bits_b = 0, side_b = 0;
+    int depth_t = 0, depth_b = 0;  // levels below the level's root: the trails hold 64 levels
     int leaf_ref = 0, leaf_count = 0, g = 0;
     float3 lo = o, ld = d, inv_ld = inv_d;
     if (live) {
@@ -562,6 +569,7 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
                 int cur = bottom ? cur_b : cur_t;
                 uint64_t bits = bottom ? bits_b : bits_t;
                 uint64_t side = bottom ? side_b : side_t;
+                int depth = bottom ? depth_b : depth_t;
                 const bool pending = on_back && (bits & 1ull);
                 if (on_node || pending) {
                     const float4 *np = reinterpret_cast<const float4 *>((bottom ? S.blas_wide : S.tlas_wide) + cur);
@@ -602,6 +610,8 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
                             cur = child;
                             bits <<= 1;
                             side <<= 1;
+                            // a 65th level would push the root's bit out of the trail
+                            if (++depth >= 64) *S.overflow = 1;
                             state = NODE;
                         } else {
                             state = LEAF;
@@ -619,16 +629,19 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
                         cur = parent;
                         bits >>= 1;
                         side >>= 1;
+                        --depth;
                     }
                 }
                 if (bottom) {
                     cur_b = cur;
                     bits_b = bits;
                     side_b = side;
+                    depth_b = depth;
                 } else {
                     cur_t = cur;
                     bits_t = bits;
                     side_t = side;
+                    depth_t = depth;
                 }
             }
         }
@@ -650,6 +663,7 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
                 if (box_entry(m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, lo, inv_ld, tmax) < NO_HIT) {
                     level = 1;
                     bits_b = side_b = 0;
+                    depth_b = 0;
                     const int root_ref = __float_as_int(m0.w);
                     if (root_ref >= 0) {
                         cur_b = root_ref;
@@ -1341,7 +1355,6 @@ __device__ __forceinline__ void wave_accumulate(const bool has, const int body, 
 }
 
 constexpr int BWD_WAVES = 4;
-constexpr int MAX_BODIES = 256;
 
 // 5 workgroups of 4 wavefronts per compute unit (at most 100 VGPRs; posed_bodies forward + backward 746k -> 794k
 // images/s; 6 gives 781k)
@@ -1552,7 +1565,7 @@ hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
 
 hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream) {
     if (a.views <= 0) return hipSuccess;
-    if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
+    if (a.bodies_per_world <= 0 || a.bodies_per_world > BACKWARD_MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
 #define NGX_LAUNCH(W) depth_backward_kernel<W><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a)
     NGX_BY_WALK(a.walk, NGX_LAUNCH)
@@ -1562,7 +1575,7 @@ hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream)
 
 hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream) {
     if (a.worlds <= 0 || a.chunks <= 0) return hipSuccess;
-    if (a.bodies_per_world <= 0 || a.bodies_per_world > MAX_BODIES) return hipErrorInvalidValue;
+    if (a.bodies_per_world <= 0 || a.bodies_per_world > BACKWARD_MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
 #define NGX_LAUNCH(W) range_backward_kernel<W><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a)
     NGX_BY_WALK(a.walk, NGX_LAUNCH)

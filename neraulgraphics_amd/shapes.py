@@ -18,7 +18,7 @@ def _revolve(profile, segments: int) -> Mesh:
     vertex on the axis (a pole or the centre of a cap); any other row is a ring of `segments` vertices. Neighbouring
     rings are joined by quads of two triangles, a ring and a point by a fan."""
     if segments < 3:
-        raise ValueError(f"segments must be at least 3, got {segments}")
+        raise ValueError(f"a surface of revolution needs 3 or more segments around its axis, not {segments}")
     phi = 2.0 * np.pi * np.arange(segments) / segments
     ring_xy = np.stack([np.cos(phi), np.sin(phi)], axis=1)
     verts, first = [], []
@@ -85,17 +85,26 @@ def plane(width: float, length: float, tile: float | None = None) -> Mesh:
 
 def look_at(eye, target, up=(0.0, 0.0, 1.0)) -> np.ndarray:
     """(N, 3, 4) camera-to-world matrices (OpenGL: x right, y up, looking along -z) of cameras at `eye` looking at
-    `target`; both (3,) or (N, 3), broadcast together. `up` is the world's up axis (Z for the simulators)."""
+    `target`; both (3,) or (N, 3), broadcast together. `up` is the world's up axis (Z for the simulators).
+
+    The frame is built from the camera's back axis b (from the target to the eye): right = up x b and the camera's
+    up = b x right, both unit length; the columns are right, up, b and the eye."""
     eye, target = np.broadcast_arrays(np.atleast_2d(np.asarray(eye, np.float64)), np.atleast_2d(np.asarray(target, np.float64)))
-    fwd = target - eye
-    fwd = fwd / np.linalg.norm(fwd, axis=1, keepdims=True)
-    right = np.cross(fwd, np.asarray(up, np.float64))
-    n = np.linalg.norm(right, axis=1, keepdims=True)
-    if np.any(n < 1e-9):
-        raise ValueError("a camera looks along the up axis, so its right direction is undefined; move the eye or change up")
-    right = right / n
-    cam_up = np.cross(right, fwd)
-    return np.stack([right, cam_up, -fwd, eye], axis=2).astype(np.float32)
+    back = eye - target
+    back = back / np.linalg.norm(back, axis=1, keepdims=True)
+    side = np.cross(np.broadcast_to(np.asarray(up, np.float64), back.shape), back)
+    length = np.linalg.norm(side, axis=1)
+    bad = np.flatnonzero(length < 1e-9)
+    if bad.size:
+        raise ValueError(f"look_at: camera {int(bad[0])} views along the up axis, which leaves its horizontal unset; "
+                         "pick another eye, target or up")
+    side = side / length[:, None]
+    frame = np.empty((len(back), 3, 4))
+    frame[:, :, 0] = side
+    frame[:, :, 1] = np.cross(back, side)
+    frame[:, :, 2] = back
+    frame[:, :, 3] = eye
+    return frame.astype(np.float32)
 
 
 def quat_to_matrix(q) -> np.ndarray:
