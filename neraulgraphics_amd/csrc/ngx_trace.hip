@@ -547,11 +547,14 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
     int leaf_ref = 0, leaf_count = 0, g = 0;
     float3 lo = o, ld = d, inv_ld = inv_d;
     if (live) {
+        // an empty world's root is -1; test that first, since with no instances at all total - 1 is -1 too
         const int root = S.roots[world];
-        if (root >= S.total - 1) {
+        if (root < 0) {
+            state = DONE;
+        } else if (root >= S.total - 1) {
             state = LEAF;
             leaf_ref = ~(root - (S.total - 1));
-        } else if (root >= 0) {
+        } else {
             state = NODE;
             cur_t = root;
         }
