@@ -208,6 +208,19 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
 ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov,
                                    const float *depth, const float *grad_depth, float *grad_out, void *stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): ngx_pose_backward_depth, same arguments and results, with
+ * its per-workgroup partial sums in a fixed device array of NGX_NOALLOC_SCRATCH_FLOATS floats (32 MiB) that each GPU
+ * holds from ngx_create until the process ends, instead of memory the context allocates; it calls no allocator
+ * and does not wait on the stream. It needs views x parts x bodies_per_world x 7 floats, parts = min(16,
+ * max(1, tiles / 8)) for a frame of `tiles` 8x8 tiles; a larger call is refused with NGX_INVALID_ARGUMENT
+ * before anything changes. Every call on one GPU shares that GPU's array (it keeps the last call's contents;
+ * each call overwrites what it reads): calls that overlap on one GPU, from other streams, threads or
+ * contexts, race and give wrong gradients. */
+#define NGX_NOALLOC_SCRATCH_FLOATS (8 * 1024 * 1024)
+ngx_status ngx_pose_backward_depth_noalloc(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov,
+                                           const float *depth, const float *grad_depth, float *grad_out, void *stream);
+
 /* No counterpart in the reference: the same for a ray query's distances. Device arrays; the same body limit. */
 ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const float *origins, const float *directions,
                                    int32_t rays_per_world, float max_distance, const float *distance,

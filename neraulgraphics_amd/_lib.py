@@ -91,6 +91,7 @@ _SIGNATURES = {
     "ngx_cast_rays_batch": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _VP,
                                              ctypes.POINTER(RayHits)]),
     "ngx_pose_backward_depth": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_float, _VP, _VP, _VP, _VP]),
+    "ngx_pose_backward_depth_noalloc": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_float, _VP, _VP, _VP, _VP]),
     "ngx_pose_backward_range": (ctypes.c_int32, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.c_float, _VP, _VP, _VP, _VP]),
     "ngx_copy_to_host": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_size_t]),
     "ngx_synchronize": (ctypes.c_int32, [_VP]),

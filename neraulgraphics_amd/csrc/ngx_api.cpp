@@ -14,6 +14,7 @@
 #include "ngx_kernels.h"
 
 static_assert(ngx::BACKWARD_MAX_BODIES == NGX_BACKWARD_MAX_BODIES, "the header states the backward's body limit");
+static_assert(ngx::NOALLOC_SCRATCH_FLOATS == (size_t)NGX_NOALLOC_SCRATCH_FLOATS, "the header states the scratch array's size");
 
 namespace {
 
@@ -166,6 +167,7 @@ struct ngx_ctx {
     long long ray_cap = 0;
     float *d_partial = nullptr;
     size_t partial_cap = 0;
+    float *d_noalloc_scratch = nullptr;  // this device's fixed scratch array (not owned: it lives as long as the process)
 
     // build intermediates
     bool keep_build = false;
@@ -422,6 +424,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
             c.compute_units = std::max(1, prop.multiProcessorCount);
         }
+        HIP_CHECK(ngx::noalloc_scratch(&c.d_noalloc_scratch));
         device_alloc(c.d_tile_counters, (size_t)ngx::TILE_COUNTER_INTS);
         device_alloc(c.d_overflow, 1);
         HIP_CHECK(hipMemset(c.d_overflow, 0, sizeof(int)));
@@ -974,51 +977,75 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
     });
 }
 
-ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov, const float *depth,
-                                   const float *grad_depth, float *grad_out, void *stream) {
-    return guarded([&] {
-        require(ctx != nullptr, "context is null");
-        require(ctx->bound && ctx->bodies > 0, "the pose backward needs bodies: call bind_bodies first");
-        if (ctx->bodies > ngx::BACKWARD_MAX_BODIES)
-            fail(NGX_UNSUPPORTED, "the pose backward takes at most " + std::to_string(ngx::BACKWARD_MAX_BODIES) +
-                                      " bodies per world; the binding has " + std::to_string(ctx->bodies));
-        require(body_poses && cams12 && depth && grad_depth && grad_out, "device arrays must not be null");
-        require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
-        ctx->set_device();
-        const hipStream_t s = static_cast<hipStream_t>(stream);
-        // enough workgroups to fill the GPU: each view's tiles are split over `parts` workgroups of 4 waves
-        const int tiles = ((ctx->width + 7) / 8) * ((ctx->height + 7) / 8);
-        const int parts = std::max(1, std::min(16, tiles / 8));
-        const size_t need = (size_t)ctx->views * parts * ctx->bodies * 7;
+namespace {
+
+// The depth backward of ngx_pose_backward_depth and ngx_pose_backward_depth_noalloc. With no_alloc, the
+// per-workgroup partial sums go to the device's fixed scratch array (looked up when the context was created)
+// and nothing is allocated or waited for; otherwise to the context's buffer, grown as needed.
+void pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, const float fov, const float *depth,
+                         const float *grad_depth, float *grad_out, void *stream, const bool no_alloc) {
+    require(ctx != nullptr, "context is null");
+    require(ctx->bound && ctx->bodies > 0, "the pose backward needs bodies: call bind_bodies first");
+    if (ctx->bodies > ngx::BACKWARD_MAX_BODIES)
+        fail(NGX_UNSUPPORTED, "the pose backward takes at most " + std::to_string(ngx::BACKWARD_MAX_BODIES) +
+                                  " bodies per world; the binding has " + std::to_string(ctx->bodies));
+    require(body_poses && cams12 && depth && grad_depth && grad_out, "device arrays must not be null");
+    require(fov > 0.0f && fov < 3.14159265f, "fov must be between 0 and pi radians, got " + std::to_string(fov));
+    // enough workgroups to fill the GPU: each view's tiles are split over `parts` workgroups of 4 waves
+    const int tiles = ((ctx->width + 7) / 8) * ((ctx->height + 7) / 8);
+    const int parts = std::max(1, std::min(16, tiles / 8));
+    const size_t need = (size_t)ctx->views * parts * ctx->bodies * 7;
+    if (no_alloc)
+        require(need <= ngx::NOALLOC_SCRATCH_FLOATS,
+                "the allocation-free backward holds " + std::to_string(ngx::NOALLOC_SCRATCH_FLOATS) + " floats of partial sums; " +
+                    std::to_string(ctx->views) + " views x " + std::to_string(parts) + " parts x " + std::to_string(ctx->bodies) +
+                    " bodies x 7 need " + std::to_string(need) + ": use ngx_pose_backward_depth");
+    ctx->set_device();
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    float *partial = ctx->d_noalloc_scratch;
+    if (!no_alloc) {
         if (need > ctx->partial_cap) {
             HIP_CHECK(hipStreamSynchronize(s));
             device_alloc(ctx->d_partial, need);
             ctx->partial_cap = need;
         }
-        ctx->build_from_bodies(body_poses, s);
-        ngx::DepthBackwardArgs a{};
-        a.s = ctx->trace_scene();
-        a.cams = cams12;
-        a.fov_scale = std::tan(fov * 0.5f);
-        a.aspect = (float)ctx->width / (float)ctx->height;
-        a.width = ctx->width;
-        a.height = ctx->height;
-        a.cameras_per_world = ctx->cpw;
-        a.views = ctx->views;
-        a.depth = depth;
-        a.grad_depth = grad_depth;
-        a.bodies_per_world = ctx->bodies;
-        a.body_of_instance = ctx->d_body_of_instance;
-        a.local_rot_scale = ctx->d_local_rot_scale;
-        a.body_poses = body_poses;
-        a.parts = parts;
-        a.partial = ctx->d_partial;
-        a.walk = ctx->walk();
-        HIP_CHECK(ngx::launch_depth_backward(a, s));
-        ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw * parts, ctx->d_partial, body_poses, grad_out};
-        HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
-        ctx->last_stream = s;
-    });
+        partial = ctx->d_partial;
+    }
+    ctx->build_from_bodies(body_poses, s);
+    ngx::DepthBackwardArgs a{};
+    a.s = ctx->trace_scene();
+    a.cams = cams12;
+    a.fov_scale = std::tan(fov * 0.5f);
+    a.aspect = (float)ctx->width / (float)ctx->height;
+    a.width = ctx->width;
+    a.height = ctx->height;
+    a.cameras_per_world = ctx->cpw;
+    a.views = ctx->views;
+    a.depth = depth;
+    a.grad_depth = grad_depth;
+    a.bodies_per_world = ctx->bodies;
+    a.body_of_instance = ctx->d_body_of_instance;
+    a.local_rot_scale = ctx->d_local_rot_scale;
+    a.body_poses = body_poses;
+    a.parts = parts;
+    a.partial = partial;
+    a.walk = ctx->walk();
+    HIP_CHECK(ngx::launch_depth_backward(a, s));
+    ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw * parts, partial, body_poses, grad_out};
+    HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
+    ctx->last_stream = s;
+}
+
+}  // namespace
+
+ngx_status ngx_pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov, const float *depth,
+                                   const float *grad_depth, float *grad_out, void *stream) {
+    return guarded([&] { pose_backward_depth(ctx, body_poses, cams12, fov, depth, grad_depth, grad_out, stream, false); });
+}
+
+ngx_status ngx_pose_backward_depth_noalloc(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov, const float *depth,
+                                           const float *grad_depth, float *grad_out, void *stream) {
+    return guarded([&] { pose_backward_depth(ctx, body_poses, cams12, fov, depth, grad_depth, grad_out, stream, true); });
 }
 
 ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const float *origins, const float *directions,

@@ -188,6 +188,12 @@ struct PoseGradFinishArgs {
 // The pose backward sums gradients per body in LDS: at most this many bodies per world.
 constexpr int BACKWARD_MAX_BODIES = 256;
 
+// The fixed device array of the allocation-free depth backward (ngx_pose_backward_depth_noalloc): its partial
+// sums, views x parts x bodies x 7 floats, must fit. noalloc_scratch gives the current device's address of it,
+// looked up once per device; every call on one device shares it.
+constexpr size_t NOALLOC_SCRATCH_FLOATS = (size_t)8 << 20;
+hipError_t noalloc_scratch(float **out);
+
 // Builds every world of at most WORKGROUP_BUILD_MAX instances (max_count: the largest such world), one
 // workgroup per world; larger worlds are skipped.
 hipError_t launch_tlas_build(const TlasBuildArgs &a, int max_count, hipStream_t stream);

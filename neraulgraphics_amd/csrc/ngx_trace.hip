@@ -12,7 +12,12 @@
 // ray enters before its current closest distance, and keeps a hit only when strictly closer
 // (mesh_dev.h:98), so the result equals the per-ray walk's except where two surfaces meet the ray at the
 // same distance.
+#include <mutex>
 #include "ngx_kernels.h"
+
+// Partial sums of ngx_pose_backward_depth_noalloc, which allocates nothing: NOALLOC_SCRATCH_FLOATS floats
+// (32 MiB) that every GPU the library runs on holds until the process ends.
+static __device__ __align__(256) float scratch[ngx::NOALLOC_SCRATCH_FLOATS];
 
 namespace ngx {
 namespace {
@@ -1591,6 +1596,31 @@ hipError_t launch_pose_grad_finish(const PoseGradFinishArgs &a, hipStream_t stre
     if (n <= 0) return hipSuccess;
     pose_grad_finish_kernel<<<(n + 255) / 256, 256, 0, stream>>>(a);
     return hipGetLastError();
+}
+
+// The allocation-free depth backward's partial sums (ngx_pose_backward_depth_noalloc). The name `scratch` is
+// a host-side stand-in: kernels and copies take the device address that noalloc_scratch() looks up, one per
+// device, since each GPU holds its own copy of the array.
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static float *g_scratch_ptr[MAX_DEVICES] = {};
+
+hipError_t noalloc_scratch(float **out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess) return e;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace ngx
