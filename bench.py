@@ -57,6 +57,33 @@ def timed_steps(step, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
+def reference_blocks(step, torch, warm_s=1.5, blocks=30, block_s=0.05):
+    """The reference's timing protocol (bench/timing.py:43-72 of the reference): steps for at least warm_s seconds
+    untimed, then `blocks` blocks of about block_s seconds each, every block between two device syncs; per-step
+    times of the blocks' median, spread and extremes."""
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < warm_s or n < 3:
+        step()
+        n += 1
+        if n % 8 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / n
+    k = int(min(2000, max(1, round(block_s / per))))
+    ms = sorted(timed_steps(step, k, 0, torch) * 1e3 for _ in range(blocks))
+    q = np.percentile(ms, [10, 50, 90])
+    return dict(median_ms=float(q[1]), p10_ms=float(q[0]), p90_ms=float(q[2]), min_ms=ms[0], max_ms=ms[-1], blocks=blocks,
+                steps_per_block=k, warmup_s=warm_s)
+
+
+def backward_bytes(views, width, height, instances, worlds, bodies, parts):
+    """Algorithmic bytes of one depth-backward launch: the forward's depth and dL/d depth read (4 + 4 B per pixel),
+    the cameras (48 B per view), each world's top-level tree and instance records read once (64 + 64 B per instance),
+    the body poses (28 B per body) and the per-workgroup partial sums written (28 B per body and part)."""
+    return 8 * views * width * height + 48 * views + 128 * instances + 28 * worlds * bodies + 28 * views * parts * bodies
+
+
 def bench_body_worlds(args, torch):
     from neraulgraphics_amd import workloads
     cfg = workloads.make("body_worlds")
@@ -128,6 +155,8 @@ def bench_body_worlds(args, torch):
     else:
         step = step_direct
     sec = timed_steps(step, args.steps, args.warmup, torch)
+    # then the reference's protocol on the same step: 1.5 s of warm-up, 30 blocks of about 50 ms, the median block
+    blocks = reference_blocks(step, torch)
     views = cfg["worlds"] * cfg["cameras_per_world"]
     inst = int(np.sum(cfg["counts"]))
     # correctness guard on the timed buffers: the last step's images are the ones of its poses
@@ -135,7 +164,7 @@ def bench_body_worlds(args, torch):
     torch.cuda.synchronize()
     hit_frac = float(torch.isfinite(out["depth"]).float().mean())
     return dict(cfg=cfg, sec=sec, build_ms=build_ms, render_ms=render_ms, views=views, instances=inst, graph=bool(graphs),
-                hit_frac=hit_frac, renderer=r, trace_config=r.trace_config)
+                hit_frac=hit_frac, renderer=r, trace_config=r.trace_config, blocks=blocks)
 
 
 def bench_instanced_worlds(args, torch):
@@ -311,8 +340,28 @@ def bench_posed_bodies(args, torch):
     steps = max(5, args.steps // 2)
     t_f = timed_steps(fwd, steps, max(2, args.warmup), torch)
     t_fb = timed_steps(fwd_bwd, steps, max(2, args.warmup), torch)
+    # the backward kernel alone, timed by the library's events on the stream it runs on
+    kernel_ms = []
+    for _ in range(steps):
+        fwd_bwd()
+        torch.cuda.synchronize()
+        kernel_ms.append(r.last_kernel_ms)
+    parts = max(1, min(16, ((cfg["width"] + 7) // 8) * ((cfg["height"] + 7) // 8) // 8))
+    nbytes = backward_bytes(cfg["worlds"], cfg["width"], cfg["height"], cfg["worlds"] * len(cfg["instances"]), cfg["worlds"],
+                            cfg["bodies"], parts)
     return dict(fwd_images_per_s=cfg["worlds"] / t_f, fwd_bwd_images_per_s=cfg["worlds"] / t_fb, ratio=t_f / t_fb,
-                worlds=cfg["worlds"])
+                worlds=cfg["worlds"], backward_kernel_ms=float(np.mean(kernel_ms)), backward_bytes=nbytes)
+
+
+def cpu_threads():
+    """The reference CPU backend runs on std::thread::hardware_concurrency() threads (src/cpu_backend.cpp:94-101 of the
+    reference). Here: every CPU this process may run on (its affinity mask), at most 16, the host cores one GPU's
+    share of a node gets."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def cpu_baseline(seconds=12.0):
@@ -320,7 +369,7 @@ def cpu_baseline(seconds=12.0):
     from neraulgraphics_amd import workloads
     from oracle.oracle import Oracle, SahTlas
     o = Oracle()
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     sample = 64
     cfg = workloads.make("body_worlds", worlds=sample)
     counts = cfg["counts"]
@@ -344,9 +393,49 @@ def cpu_baseline(seconds=12.0):
         el = time.perf_counter() - t0
         if el > seconds:
             break
-    return dict(value=images / el, unit="images/s", cores=threads, kind="port",
-                sample=f"{images} images of body_worlds ({sample} worlds a step, 128x128 depth+ID, body poses, "
-                       f"per-world SAH top level with refit) in {el:.1f} s on {threads} threads")
+    out = dict(value=images / el, unit="images/s", cores=threads, kind="port",
+               sample=f"{images} images of body_worlds ({sample} worlds a step, 128x128 depth+ID, body poses, "
+                      f"per-world SAH top level with refit) in {el:.1f} s on {threads} threads "
+                      f"(the host shows {os.cpu_count()} CPUs)")
+    out.update(reference_core_calibration(o, cfg, nodes, roots, inv, blas_root, meshes))
+    return out
+
+
+def reference_core_calibration(o, cfg, nodes, roots, inv, blas_root, meshes, views=4):
+    """The port's traversal against the reference's own compiled traversal core (src/mesh_dev.h of the reference, built
+    into oracle/_ref where the reference's sources are present) on the same camera rays, one thread each: the ratio
+    relates the CPU baseline (the port) to the reference CPU backend's own speed. {} where oracle/_ref is absent."""
+    from oracle.oracle import Ref, ref_available
+    if not ref_available():
+        return {}
+    ref = Ref()
+    w, h = cfg["width"], cfg["height"]
+    orig, dirs, _ = ref.camera_rays(cfg["cameras"][:views], cfg["fov"], w, h)
+    orig, dirs = orig.reshape(-1, 3), dirs.reshape(-1, 3)
+    ray_world = np.repeat(np.arange(views, dtype=np.int32), w * h)
+    t0 = time.perf_counter()
+    r = ref.mesh_intersect(nodes, roots, inv, blas_root, meshes["nodes"], meshes["tris"], ray_world, orig, dirs)
+    t_ref = time.perf_counter() - t0
+    # the port over the same rays: the first `views` worlds' rays as a ray query
+    world_off = np.concatenate([[0], np.cumsum(cfg["counts"])]).astype(np.int32)[:views + 1]
+    t0 = time.perf_counter()
+    p = o.cast_rays(nodes, roots, inv, blas_root, meshes["nodes"], meshes["tris"], meshes["attrs"], world_off,
+                    orig.reshape(views, -1, 3), dirs.reshape(views, -1, 3), 1e30, threads=1)
+    t_port = time.perf_counter() - t0
+    same = int(np.sum((r["inst"] >= 0) == np.isfinite(p["distance"].reshape(-1))))
+    return dict(reference_core_ratio=t_port / t_ref,
+                reference_core_note=f"{len(ray_world)} camera rays of {views} worlds, one thread each: the reference's compiled "
+                                    f"traversal core {t_ref:.2f} s, the port {t_port:.2f} s ({same} of {len(ray_world)} rays "
+                                    "agree on hit or miss)")
+
+
+def _profile_summary(name):
+    """A traffic summary committed under profiles/ (HBM bytes per launch from rocprofv3 counters), or {}."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
 
 
 def main():
@@ -369,7 +458,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 512 MiB three times with a plain copy kernel, to scale HBM counters")
+    ap.add_argument("--only", default="", help="run only this leg (for profiling): body_worlds, instanced, lidar, bwd, rgb, "
+                                               "scene or cpu; the line then holds that leg's fields")
     args = ap.parse_args()
+    if args.only:
+        legs = ("body_worlds", "instanced", "lidar", "bwd", "rgb", "scene", "cpu")
+        if args.only not in legs:
+            ap.error(f"--only takes one of {', '.join(legs)}")
+        args.no_instanced = args.only != "instanced"
+        args.no_lidar = args.only != "lidar"
+        args.no_bwd = args.only != "bwd"
+        args.no_rgb = args.only != "rgb"
+        args.no_scene = args.only != "scene"
+        args.no_cpu = args.only != "cpu"
     if args.gpus != 1:
         print(json.dumps({"error": f"this benchmark runs on one GPU; --gpus {args.gpus} is not supported: the path shards "
                                    "by world with no exchange, so N GPUs run N independent batches"}), flush=True)
@@ -386,12 +487,14 @@ def main():
         torch.cuda.synchronize()
         print("# copy calibration: 3 copies of 536870912 bytes", file=sys.stderr, flush=True)
         del src, dst
-    print("# body_worlds: 4096 worlds 128x128 depth+ID", file=sys.stderr, flush=True)
-    c2 = bench_body_worlds(args, torch)
-    sec = c2["sec"]
-    rb = render_bytes(c2["views"], 128, 128, c2["instances"])
-    achieved = rb / (c2["render_ms"] * 1e-3) / 1e9
-    print(f"# step {sec * 1e3:.3f} ms (build {c2['build_ms']:.3f} ms, render {c2['render_ms']:.3f} ms)", file=sys.stderr, flush=True)
+    c2 = None
+    if not args.only or args.only == "body_worlds":
+        print("# body_worlds: 4096 worlds 128x128 depth+ID", file=sys.stderr, flush=True)
+        c2 = bench_body_worlds(args, torch)
+        b = c2["blocks"]
+        print(f"# step {c2['sec'] * 1e3:.3f} ms (build {c2['build_ms']:.3f} ms, render {c2['render_ms']:.3f} ms); "
+              f"reference protocol: median {b['median_ms']:.3f} ms, p10-p90 {b['p10_ms']:.3f}-{b['p90_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
     c1 = None
     if not args.no_instanced:
         print("# instanced_worlds: 4096 worlds 64x64 depth+ID", file=sys.stderr, flush=True)
@@ -417,41 +520,35 @@ def main():
     if not args.no_cpu:
         print("# CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_seconds)
-    kernel = "render_persistent_kernel" if c2["trace_config"].get("persistent") else "render_kernel"
-    prof = os.path.join(ROOT, "profiles", f"{kernel}_body_worlds_traffic.json")
-    traffic, valu_busy = None, None
-    if os.path.exists(prof):
-        with open(prof) as f:
-            summary = json.load(f)
-        traffic = summary.get("hbm_bytes_per_launch")
-        valu_busy = summary.get("kernels", {}).get("render_kernel_sq", {}).get("valu_busy")
-    line = {
-        "metric": METRIC,
-        "value": c2["views"] / sec,
-        "unit": "images/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "body_worlds: 4096 worlds, 128x128 depth+object-ID, U{8..256} of a 256-instance "
-                               "template per world, 64 body poses per world, new poses every step",
-                   "worlds": 4096, "width": 128, "height": 128, "instances": c2["instances"], "outputs": "depth+ID",
-                   "hip_graph": c2["graph"], "trace_config": c2["trace_config"]},
-        "rays_per_s": c2["views"] * 128 * 128 / sec,
-        "build_ms": c2["build_ms"],
-        "render_ms": c2["render_ms"],
-        "hit_fraction": c2["hit_frac"],
-        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": rb,
-                     # the bound that holds: the fraction of cycles every SIMD issues a vector instruction (rocprof, same kernel)
-                     "valu_busy": valu_busy},
-        "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
-    }
+    line = {"metric": METRIC, "value": None, "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic"}
+    if c2:
+        sec = c2["sec"]
+        rb = render_bytes(c2["views"], 128, 128, c2["instances"])
+        achieved = rb / (c2["render_ms"] * 1e-3) / 1e9
+        kernel = "render_persistent_kernel" if c2["trace_config"].get("persistent") else "render_kernel"
+        summary = _profile_summary(f"{kernel}_body_worlds_traffic.json")
+        line.update({
+            "value": c2["views"] / sec,
+            "ms_per_step": sec * 1e3,
+            "config": {"workload": "body_worlds: 4096 worlds, 128x128 depth+object-ID, U{8..256} of a 256-instance "
+                                   "template per world, 64 body poses per world, new poses every step",
+                       "worlds": 4096, "width": 128, "height": 128, "instances": c2["instances"], "outputs": "depth+ID",
+                       "hip_graph": c2["graph"], "trace_config": c2["trace_config"]},
+            "timing_blocks": c2["blocks"],
+            "rays_per_s": c2["views"] * 128 * 128 / sec,
+            "build_ms": c2["build_ms"],
+            "render_ms": c2["render_ms"],
+            "hit_fraction": c2["hit_frac"],
+            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": summary.get("hbm_bytes_per_launch"),
+                         "algorithmic_bytes_per_launch": rb,
+                         # the bound that holds: the fraction of cycles every SIMD issues a vector instruction
+                         # (rocprof, same kernel)
+                         "valu_busy": summary.get("kernels", {}).get("render_kernel_sq", {}).get("valu_busy")},
+            "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
+        })
     if c1:
         line.update(instanced_worlds_64x64_images_per_s=c1["images_per_s"], instanced_worlds_64x64_ms=c1["ms"],
                     instanced_worlds_reference_claim_images_per_s=c1["reference_claim_images_per_s"])
@@ -460,6 +557,11 @@ def main():
     if c4:
         line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])
+        bw = c4["backward_bytes"] / (c4["backward_kernel_ms"] * 1e-3) / 1e9
+        line["fwd_bwd_roofline"] = {
+            "bound": "hbm", "kernel": "depth_backward_kernel", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": bw / HBM_PEAK_GBS, "traffic": _profile_summary("depth_backward_kernel_posed_bodies_traffic.json").get("hbm_bytes_per_launch"),
+            "algorithmic_bytes_per_launch": c4["backward_bytes"], "kernel_ms": c4["backward_kernel_ms"]}
     if crgb:
         line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"])
     if cscene:

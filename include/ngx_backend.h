@@ -234,8 +234,9 @@ ngx_status ngx_copy_to_host(ngx_ctx *ctx, const void *src, void *dst, size_t byt
  * builds keep within that, so this marks a corrupted tree), in which case those results may be wrong. */
 ngx_status ngx_synchronize(ngx_ctx *ctx);
 
-/* Backend::last_kernel_ms (src/backend.h:83): device time of the last render or ray query's kernel, in ms,
- * measured with events; read after ngx_synchronize. */
+/* Backend::last_kernel_ms (src/backend.h:83): device time of the last render or ray query's kernel (or, after a
+ * pose backward, of its backward kernel), in ms, measured with events; read after ngx_synchronize. 0 after a call
+ * made while its stream was being captured into a graph. */
 float ngx_last_kernel_ms(ngx_ctx *ctx);
 
 /* BatchRenderer's accessors (include/neraulgraphics/batch.h:214-221). */

@@ -1030,7 +1030,12 @@ void pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cam
     a.parts = parts;
     a.partial = partial;
     a.walk = ctx->walk();
+    // the backward kernel alone is what ngx_last_kernel_ms reports after this call
+    const bool timed = !ctx->capturing(s);
+    if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
     HIP_CHECK(ngx::launch_depth_backward(a, s));
+    if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
+    ctx->timed = timed;
     ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, ctx->cpw * parts, partial, body_poses, grad_out};
     HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
     ctx->last_stream = s;
@@ -1086,7 +1091,11 @@ ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const 
         a.body_poses = body_poses;
         a.worlds = ctx->worlds;
         a.partial = ctx->d_partial;
+        const bool timed = !ctx->capturing(s);
+        if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
         HIP_CHECK(ngx::launch_range_backward(a, s));
+        if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
+        ctx->timed = timed;
         ngx::PoseGradFinishArgs f{ctx->worlds, ctx->bodies, chunks, ctx->d_partial, body_poses, grad_out};
         HIP_CHECK(ngx::launch_pose_grad_finish(f, s));
         ctx->last_stream = s;
