@@ -136,7 +136,8 @@ def make(name, worlds=None, rays_per_world=None, seed=None):
         origins = np.broadcast_to(origin[:, None, :], (W, beams * az_n, 3)).copy()
         insts = [Instance(0) for _ in range(P)]
         return dict(width=64, height=64, worlds=W, cameras_per_world=1, meshes=[mesh], instances=insts, counts=counts,
-                    mode="matrices", matrices=mats, origins=origins, directions=dirs, max_distance=100.0)
+                    mode="matrices", matrices=mats, origins=origins, directions=dirs, max_distance=100.0, positions=pos,
+                    quaternions=quat, scales=scale)
     raise AssertionError(name)
 
 

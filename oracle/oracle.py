@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(HERE, "libngx_oracle.so")
+# NGX_ORACLE_LIB may name a sanitizer build of the same source (oracle/Makefile's `sanitize` target)
+ORACLE_LIB = os.environ.get("NGX_ORACLE_LIB", os.path.join(HERE, "libngx_oracle.so"))
 REF_LIB = os.path.join(HERE, "_ref", "libngx_ref.so")
 
 TLAS_NODE = np.dtype([("lo", "<f4", (3,)), ("left", "<i4"), ("hi", "<f4", (3,)), ("right", "<i4")])

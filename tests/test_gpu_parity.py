@@ -572,6 +572,127 @@ def test_range_gradient_matches_fp64_closed_form_and_differences(gpu, oracle):
     assert rel_cf < 1e-4
 
 
+def _depth_gradient_refs(oracle, cfg, weight, sample):
+    """The oracle's FP64 closed-form gradient and central differences (step 1e-4) of sum(weight x depth) for the
+    worlds in `sample` (weight: (len(sample) x cameras_per_world, H, W) over those worlds' views), through the hit
+    triangles of the oracle's own render of those worlds."""
+    sub = world_subset(cfg, sample)
+    ref = forward(sub, oracle)
+    V, H, W = ref["depth"].shape
+    origins, dirs, cosv = np.zeros((V * H * W, 3)), np.zeros((V * H * W, 3)), np.zeros(V * H * W)
+    for v in range(V):
+        for y in range(H):
+            for x in range(W):
+                o, d, f = oracle.camera_ray(sub["cameras"][v], sub["fov"], W, H, x, y)
+                k = (v * H + y) * W + x
+                origins[k], dirs[k], cosv[k] = o, d, float(np.dot(d.astype(np.float64), f.astype(np.float64)))
+    ray_world = np.repeat(np.arange(V) // sub["cameras_per_world"], H * W).astype(np.int32)
+    args = (origins, dirs, np.asarray(weight, np.float64).reshape(-1), cosv, ref["hit_inst"].reshape(-1), ref["hit_tri"].reshape(-1),
+            ray_world, ref["world_off"], sub["bodies"], sub["body_of_instance"], sub["local_pose"], sub["scale"], sub["body_poses"],
+            ref["meshes"]["tris"])
+    return oracle.pose_grad(0, 0.0, *args), oracle.pose_grad(1, 1e-4, *args)
+
+
+def _depth_loss_gradient(r, cfg, cams):
+    """GPU gradient of 0.5 |depth - target|^2 (pixels that hit in both) and the per-pixel weight depth - target."""
+    import torch
+    from neraulgraphics_amd.autograd import render_depth
+    c = torch.from_numpy(np.ascontiguousarray(cams, np.float32)).cuda()
+    r.set_body_poses(cfg["target_poses"])
+    target = r.render(c, cfg["fov"], depth=True, zero_copy=True)["depth"].clone()
+    poses = torch.from_numpy(cfg["body_poses"]).cuda().requires_grad_(True)
+    depth = render_depth(r, poses, c, cfg["fov"])
+    both = torch.isfinite(depth) & torch.isfinite(target)
+    diff = torch.where(both, depth - target, torch.zeros_like(depth))
+    (0.5 * (diff * diff).sum()).backward()
+    return poses.grad.cpu().numpy().astype(np.float64), diff.detach().cpu().numpy()
+
+
+def test_pose_gradient_with_three_cameras_a_world(gpu, oracle):
+    """The depth backward with 3 cameras a world: each world's gradient sums its three views' partial sums (the
+    cameras_per_world x parts reduction of the finish kernel), against the oracle's FP64 closed form and central
+    differences over all 3 x 8 views."""
+    from neraulgraphics_amd.batch import BatchRenderer
+    _say("backward: posed_bodies, 8 worlds x 3 cameras")
+    cfg = workloads.make("posed_bodies", worlds=8)
+    cpw = 3
+    rng = np.random.default_rng(41)
+    cams = np.repeat(cfg["cameras"].reshape(8, 1, 12), cpw, axis=1).copy()
+    cams[:, 1:, [3, 7, 11]] += rng.normal(0.0, 0.4, (8, cpw - 1, 3)).astype(np.float32)
+    cfg = dict(cfg, cameras_per_world=cpw, cameras=cams.reshape(-1, 12))
+    r = BatchRenderer(64, 64, 8, cpw, meshes=cfg["meshes"], instances=cfg["instances"])
+    r.bind_bodies(cfg["body_of_instance"], cfg["local_pose"], cfg["scale"], cfg["bodies"])
+    g_gpu, weight = _depth_loss_gradient(r, cfg, cfg["cameras"])
+    g_cf, g_fd = _depth_gradient_refs(oracle, cfg, weight, np.arange(8))
+    rel = np.linalg.norm(g_gpu - g_fd) / np.linalg.norm(g_fd)
+    rel_cf = np.linalg.norm(g_gpu - g_cf) / np.linalg.norm(g_cf)
+    _say(f"|GPU - FD(1e-4)| / |FD| = {rel:.2e}; |GPU - closed form| / |closed form| = {rel_cf:.2e}")
+    assert rel < 1e-3 and rel_cf < 1e-4
+
+
+def test_pose_gradient_of_the_full_posed_bodies_batch_on_a_sample(gpu, oracle):
+    """The fwd+bwd configuration at full size (1024 worlds at 64x64, 32 bodies each): one forward and one backward
+    over the whole batch; 12 worlds spread over it (first and last included) against the oracle's FP64 closed form
+    and central differences on those worlds alone. A world's gradient depends on its own views only."""
+    _say("backward: posed_bodies, 1024 worlds, 12 sampled worlds")
+    cfg = workloads.make("posed_bodies")
+    r = workloads.make_renderer(cfg)
+    g_gpu, weight = _depth_loss_gradient(r, cfg, cfg["cameras"])
+    sample = np.unique(np.concatenate([[0, cfg["worlds"] - 1], np.random.default_rng(9).choice(cfg["worlds"], 10, replace=False)]))
+    g_cf, g_fd = _depth_gradient_refs(oracle, cfg, weight[sample], sample)
+    g = g_gpu[sample]
+    rel = np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd)
+    rel_cf = np.linalg.norm(g - g_cf) / np.linalg.norm(g_cf)
+    _say(f"|GPU - FD(1e-4)| / |FD| = {rel:.2e}; |GPU - closed form| / |closed form| = {rel_cf:.2e}")
+    assert rel < 1e-3 and rel_cf < 1e-4
+    assert np.all(np.isfinite(g_gpu))
+
+
+def test_range_gradient_on_a_lidar_sweeps_sample(gpu, oracle):
+    """The range backward at the LiDAR configuration's per-world size: 3 worlds of 97 instances of the 138,000-triangle
+    mesh, each on its own body, 65,536 rays a world (64 beams x 1024 azimuths); L = 0.5 sum (range - target)^2 against
+    a target from poses moved by 0.02; the oracle's FP64 closed form and central differences (step 1e-4)."""
+    import torch
+    from neraulgraphics_amd.autograd import cast_range
+    from neraulgraphics_amd.batch import BatchRenderer
+    _say("range backward: lidar_sweeps, 3 worlds x 65536 rays")
+    lc = workloads.make("lidar_sweeps", worlds=3)
+    W, P, R = 3, len(lc["instances"]), lc["origins"].shape[1]
+    poses = np.concatenate([lc["positions"], lc["quaternions"]], axis=-1).astype(np.float32)
+    local = np.tile(np.array([0, 0, 0, 0, 0, 0, 1], np.float32), (P, 1))
+    sc = np.repeat(lc["scales"][0][:, None], 3, axis=1).astype(np.float32)
+    cfg = dict(lc, mode="bodies", bodies=P, body_of_instance=np.arange(P, dtype=np.int32), local_pose=local, scale=sc,
+               body_poses=poses)
+    r = BatchRenderer(64, 64, W, 1, meshes=lc["meshes"], instances=lc["instances"], instances_in_world=lc["counts"])
+    r.bind_bodies(cfg["body_of_instance"], local, sc, P)
+    moved = poses.copy()
+    moved[..., :3] += np.random.default_rng(43).normal(0.0, 0.02, moved[..., :3].shape).astype(np.float32)
+    r.set_body_poses(moved)
+    target = r.cast_rays(lc["origins"], lc["directions"], lc["max_distance"])["distance"].copy()
+    p = torch.from_numpy(poses).cuda().requires_grad_(True)
+    dist = cast_range(r, p, torch.from_numpy(lc["origins"]).cuda(), torch.from_numpy(lc["directions"]).cuda(), lc["max_distance"])
+    tgt = torch.from_numpy(target).cuda()
+    both = torch.isfinite(dist) & torch.isfinite(tgt)
+    diff = torch.where(both, dist - tgt, torch.zeros_like(dist))
+    (0.5 * (diff * diff).sum()).backward()
+    g_gpu = p.grad.cpu().numpy().astype(np.float64)
+    assert int(both.sum()) > W * R // 10
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], lc["origins"], lc["directions"], lc["max_distance"])
+    d32 = lc["directions"].reshape(-1, 3)
+    unit = (d32 * (np.float32(1.0) / np.sqrt((d32 * d32).sum(axis=1, dtype=np.float32)))[:, None]).astype(np.float64)
+    args = (lc["origins"].reshape(-1, 3).astype(np.float64), unit, diff.detach().cpu().numpy().reshape(-1).astype(np.float64),
+            np.ones(W * R), ref["hit_inst"].reshape(-1), ref["hit_tri"].reshape(-1), np.repeat(np.arange(W), R).astype(np.int32),
+            st["world_off"], P, cfg["body_of_instance"], local, sc, poses, st["meshes"]["tris"])
+    g_cf = oracle.pose_grad(0, 0.0, *args)
+    g_fd = oracle.pose_grad(1, 1e-4, *args)
+    rel = np.linalg.norm(g_gpu - g_fd) / np.linalg.norm(g_fd)
+    rel_cf = np.linalg.norm(g_gpu - g_cf) / np.linalg.norm(g_cf)
+    _say(f"|GPU - FD(1e-4)| / |FD| = {rel:.2e}; |GPU - closed form| / |closed form| = {rel_cf:.2e}")
+    assert rel < 1e-3 and rel_cf < 1e-4
+
+
 def render_target(r, cfg):
     r.set_body_poses(cfg["target_poses"])
     d = r.render(cfg["cameras"], cfg["fov"], depth=True)["depth"]
@@ -622,6 +743,41 @@ def test_rgb_picture_matches_the_oracle(gpu, oracle):
          f"pixels differ by more than 2 levels; bit-identical {float(np.mean(np.all(diff == 0, axis=-1))):.4f}")
     assert mean < 0.05
     assert len(np.unique(want.reshape(-1, 3), axis=0)) > 1000  # a picture, not a flat colour
+
+
+def test_rgb_depth_id_of_the_full_body_worlds_batch_on_a_sample_of_worlds(gpu, oracle):
+    """The RGB + depth + object-ID configuration at full size: 4096 worlds at 128x128, U{8..256} instances a world from
+    64 body poses, a checker floor, one soft sphere light and ambient 0.1 (the bench's scene), one launch each; 16
+    worlds spread over the batch against the oracle run on those worlds alone, to tests/test_batch.cpp:177's bar (mean
+    difference below 0.05 of a level per channel). A pixel's random stream depends on its position and the frame only,
+    so the sample sees what the whole batch sees."""
+    import torch
+    import neraulgraphics_amd as ng
+    _say("full-size RGB parity: body_worlds, 4096 worlds, 16 sampled worlds")
+    cfg = workloads.make("body_worlds")
+    floor = ng.Plane(y=-6.0, half_extent_x=40.0, z_min=-60.0, z_max=20.0)
+    lights = [ng.Light((0.0, 12.0, -6.0), intensity=1.2, radius=0.5)]
+    r = workloads.make_renderer(cfg, floor=floor, lights=lights)
+    r.set_ambient(0.1)
+    out = r.render(torch.from_numpy(cfg["cameras"]).cuda(), cfg["fov"], 1, rgb=True, depth=True, object_ids=True, frame=3,
+                   zero_copy=True)
+    torch.cuda.synchronize()
+    sample = np.unique(np.concatenate([[0, cfg["worlds"] - 1], np.random.default_rng(6).choice(cfg["worlds"], 14, replace=False)]))
+    idx = torch.from_numpy(sample).cuda()
+    got = {k: out[k][idx].cpu().numpy() for k in ("rgb", "depth", "object_ids")}
+    sub = world_subset(cfg, sample)
+    sub["floor"] = np.array([floor.y, floor.half_extent_x, floor.z_min, floor.z_max], np.float32)
+    sub["spheres"] = None
+    sub["shading"] = ng.shading_arrays(cfg["instances"], [], floor, lights, 0.1, None)
+    sub["spp"], sub["frame"] = 1, 3
+    ref = forward(sub, oracle, outputs=("rgb", "depth", "id"))
+    rep = compare_images(dict(depth=got["depth"], object_id=got["object_ids"]), ref, "body_worlds RGB sample, depth and ID")
+    diff = np.abs(got["rgb"].astype(np.int32) - ref["rgb"].astype(np.int32))
+    mean = diff.reshape(-1, 3).mean(axis=0)
+    _say(f"{rep}; mean colour difference per channel {mean}; bit-identical pixels {float(np.mean(np.all(diff == 0, axis=-1))):.5f}")
+    assert np.all(mean < 0.05)
+    assert rep["hits"] > 0.3 * rep["pixels"]
+    assert len(np.unique(ref["rgb"].reshape(-1, 3), axis=0)) > 1000
 
 
 def test_rgb_known_answers_on_the_device(gpu, oracle):

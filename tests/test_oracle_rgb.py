@@ -1,7 +1,10 @@
 """The oracle's RGB picture (a restatement of render_core.h:146-283) against answers fixed by the mathematics:
 a sky of one colour is that colour through the sRGB curve; a floor lit straight from above by a point light
 is its diffuse colour times intensity times albedo (Lambert at normal incidence); a floor point hidden from
-the only light by a sphere gets the ambient term alone; a mirror floor shows the sky. The reference's own
+the only light by a sphere gets the ambient term alone; a mirror floor shows the sky; a glossy floor's Phong
+highlight, a coloured spotlight's cone and a light's distance falloff follow their formulas pixel by pixel, each
+pixel's ray jittered by the first two draws of its PCG stream (which pins the stream's seeding and draw order). The
+reference's own
 RGB code cannot be run here (its surface shading calls the CPU backend's texture sampler, which does not
 build), so these closed forms are what pins it."""
 import numpy as np
@@ -100,3 +103,130 @@ def test_an_area_light_is_sampled_per_pixel_and_frame(oracle):
     assert c[ring].std() < a[ring].std()
     shaded = (ids == ng.OBJECT_FLOOR) & (a[..., 0] == 0)
     assert shaded.sum() > 0 and c[shaded].mean() < c[ring].mean()
+
+
+# ---- direct light over the floor, in closed form, with the jittered camera rays worked out independently ----------
+
+M32 = np.uint64(0xFFFFFFFF)
+
+
+def pcg_hash(x):
+    """The PCG-family integer hash (Jarzynski and Olano, "Hash Functions for GPU Rendering", 2020), on uint32 arrays."""
+    x = np.asarray(x, np.uint64) & M32
+    state = (x * np.uint64(747796405) + np.uint64(2891336453)) & M32
+    word = (((state >> ((state >> np.uint64(28)) + np.uint64(4))) ^ state) * np.uint64(277803737)) & M32
+    return ((word >> np.uint64(22)) ^ word) & M32
+
+
+def jitter(width, height, frame):
+    """Per pixel, the first two uniforms of its stream: seeded by hash(pixel index ^ hash(frame + 1)), each draw
+    re-hashing the state and scaling it by 2^-32; x's offset is drawn first."""
+    idx = np.arange(width * height, dtype=np.uint64)
+    state = pcg_hash(idx ^ pcg_hash(np.uint64(frame + 1)))
+    s1 = pcg_hash(state)
+    s2 = pcg_hash(s1)
+    to_unit = np.float32(2.3283064365386963e-10)
+    jx = s1.astype(np.float32) * to_unit
+    jy = s2.astype(np.float32) * to_unit
+    return jx.reshape(height, width), jy.reshape(height, width)
+
+
+def camera_dirs(cam12, fov, width, height, jx, jy):
+    """Unit directions (float64) through (x + jx, y + jy) of an OpenGL camera (x right, y up, looking along -z)."""
+    m = np.asarray(cam12, np.float64).reshape(3, 4)
+    right, up, back = (m[:, k] / np.linalg.norm(m[:, k]) for k in range(3))
+    fs = np.tan(fov / 2)
+    ys, xs = np.mgrid[0:height, 0:width]
+    sx = (2.0 * (xs + jx) / width - 1.0) * fs * (width / height)
+    sy = -(2.0 * (ys + jy) / height - 1.0) * fs
+    d = sx[..., None] * right + sy[..., None] * up - back
+    return m[:, 3], d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def srgb_float(v):
+    v = np.clip(v, 0.0, 1.0)
+    s = np.where(v <= 0.0031308, 12.92 * v, 1.055 * np.power(v, 1 / 2.4) - 0.055)
+    return np.floor(np.clip(s, 0, 1) * 255 + 0.5).astype(np.int32)
+
+
+def floor_direct(eye, dirs, floor_y, light, mat, tint):
+    """Radiance of the floor (normal +y) seen along dirs from eye, lit by one point light with nothing in the way: the
+    diffuse colour times the Lambert term times the diffuse albedo, plus the Phong term max(0, reflect(l, n) . d)^e
+    times the specular albedo, the light scaled by 1 / (1 + (distance / falloff)^2) and by the spotlight's smoothstep
+    from the cone's outer edge to its inner one."""
+    t = (floor_y - eye[1]) / dirs[..., 1]
+    p = eye + dirs * t[..., None]
+    to_l = np.asarray(light.position, np.float64) - p
+    dist = np.linalg.norm(to_l, axis=-1)
+    l = to_l / dist[..., None]
+    f = np.ones_like(dist)
+    if light.falloff > 0:
+        f /= 1.0 + (dist / light.falloff) ** 2
+    if light.spot_angle > 0:
+        axis = np.asarray(light.direction, np.float64)
+        axis /= np.linalg.norm(axis)
+        c = -(l @ axis)
+        outer, inner = np.cos(light.spot_angle), np.cos(light.spot_angle * (1.0 - light.spot_softness))
+        s = np.clip((c - outer) / max(inner - outer, 1e-6), 0.0, 1.0)
+        f *= s * s * (3.0 - 2.0 * s)
+    strength = light.intensity * f
+    lam = np.maximum(0.0, l[..., 1])
+    r = l.copy()
+    r[..., 1] = -r[..., 1]  # reflect(l, n) with n = +y
+    phong = np.maximum(0.0, np.sum(r * dirs, axis=-1)) ** mat.specular_exponent
+    col = np.asarray(light.color, np.float64)
+    lit = strength[..., None] * col
+    return np.asarray(tint, np.float64) * lit * lam[..., None] * mat.diffuse_albedo + lit * phong[..., None] * mat.specular_albedo
+
+
+def _floor_scene(light, mat, size=48, frame=0, eye=(0.0, 0.0, 0.0)):
+    floor = ng.Plane(y=-1.0, half_extent_x=50.0, z_min=-50.0, z_max=50.0, checker_scale=0.0, material=mat)
+    cam = ng.look_at(eye, (0.3, -1.0, -0.2), up=(0, 0, -1))
+    cfg = scene(lights=[light], floor=floor, cam=cam, size=size)
+    cfg["frame"] = frame
+    return cfg, cam.reshape(12)
+
+
+TINT = np.float32(0.3) * np.array([1.0, 0.7, 0.3])  # checker cell (0, 0): the floor's tinted colour
+
+
+def test_phong_highlight_and_lambert_term_follow_the_closed_form(oracle):
+    """A glossy floor (diffuse albedo 0.5, specular albedo 0.7, exponent 40) under a point light, seen at an angle:
+    every pixel equals the closed form within one level, with the jitter of each pixel's ray taken from its PCG stream.
+    The highlight is steep enough that pixel-centre rays (no jitter) or the jitter's two draws swapped miss it."""
+    mat = ng.Material((1.0, 1.0, 1.0), 0.5, 0.7, 0.0, 40.0)
+    light = ng.Light((0.9, 1.5, -0.5), intensity=1.4)
+    cfg, cam = _floor_scene(light, mat, frame=2)
+    got = forward(cfg, oracle, outputs=("rgb",))["rgb"][0].astype(np.int32)
+    jx, jy = jitter(48, 48, 2)
+    eye, d = camera_dirs(cam, 1.0, 48, 48, jx, jy)
+    want = srgb_float(floor_direct(eye, d, -1.0, light, mat, TINT))
+    assert np.abs(got - want).max() <= 1, np.abs(got - want).max()
+    assert got.max() >= 200 and len(np.unique(got[..., 0])) > 40  # a highlight and a gradient, not a flat colour
+    # the check of the check: without the jitter, or with its draws swapped, the same closed form misses
+    for a, b in ((np.full_like(jx, 0.5), np.full_like(jy, 0.5)), (jy, jx)):
+        _, d2 = camera_dirs(cam, 1.0, 48, 48, a, b)
+        assert np.abs(got - srgb_float(floor_direct(eye, d2, -1.0, light, mat, TINT))).max() > 1
+
+
+def test_spotlight_cone_and_distance_falloff_follow_the_closed_form(oracle):
+    """A coloured spotlight (half-angle 0.45 rad, softness 0.5) with falloff 1.2 over a matte floor: full strength
+    inside the inner cone, nothing outside the outer one, a smoothstep between, all divided by 1 + (d / 1.2)^2;
+    every pixel within one level of the closed form."""
+    mat = ng.Material((1.0, 1.0, 1.0), 0.9, 0.0, 0.0, 1.0)
+    light = ng.Light((0.3, 1.0, -0.4), intensity=4.0, falloff=1.2, direction=(0.1, -1.0, 0.05), spot_angle=0.45,
+                     spot_softness=0.5, color=(1.0, 0.8, 0.6))
+    cfg, cam = _floor_scene(light, mat, frame=5, eye=(0.0, 1.5, 0.0))
+    got = forward(cfg, oracle, outputs=("rgb",))["rgb"][0].astype(np.int32)
+    jx, jy = jitter(48, 48, 5)
+    eye, d = camera_dirs(cam, 1.0, 48, 48, jx, jy)
+    want = srgb_float(floor_direct(eye, d, -1.0, light, mat, TINT))
+    assert np.abs(got - want).max() <= 1
+    # the cone is in view: lit pixels, dark pixels outside it, and a soft rim
+    assert (got[..., 0] == 0).sum() > 100 and (got[..., 0] > 60).sum() > 50
+    rim = (got[..., 0] > 0) & (got[..., 0] < 0.5 * got[..., 0].max())
+    assert rim.sum() > 20
+    # the check of the check: no falloff, or a hard cone, changes the picture beyond a level
+    for other in (ng.Light(light.position, 4.0, 0.0, 0.0, light.direction, 0.45, 0.5, light.color),
+                  ng.Light(light.position, 4.0, 0.0, 1.2, light.direction, 0.45, 0.0, light.color)):
+        assert np.abs(got - srgb_float(floor_direct(eye, d, -1.0, other, mat, TINT))).max() > 1
