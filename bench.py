@@ -225,6 +225,9 @@ def bench_lidar(args, torch):
     r = workloads.make_renderer(cfg)
     o = torch.from_numpy(cfg["origins"]).cuda()
     d = torch.from_numpy(cfg["directions"]).cuda()
+    if not args.no_tune:
+        tuned = r.autotune_rays(o, d, cfg["max_distance"], reps=2)
+        print(f"# LiDAR autotuned: {tuned}", file=sys.stderr, flush=True)
     for _ in range(2):
         r.cast_rays(o, d, cfg["max_distance"])
     torch.cuda.synchronize()
@@ -248,6 +251,10 @@ def bench_rgb(args, torch):
     r = workloads.make_renderer(cfg, floor=floor, lights=[Light((0.0, 12.0, -6.0), intensity=1.2, radius=0.5)])
     r.set_ambient(0.1)
     cams = torch.from_numpy(cfg["cameras"]).cuda()
+    tuned = None
+    if not args.no_tune:
+        tuned = r.autotune(cams, cfg["fov"], depth=True, object_ids=True, rgb=True, reps=1, cache=False)
+        print(f"# RGB autotuned: {tuned}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(7)
     pose_sets = [torch.from_numpy(cfg["body_poses"]).cuda()]
     for _ in range(3):
@@ -263,7 +270,7 @@ def bench_rgb(args, torch):
 
     steps = max(5, args.steps // 2)
     sec = timed_steps(step, steps, max(2, args.warmup // 2), torch)
-    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3)
+    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3, trace_config=r.trace_config)
 
 
 # The reference's own published batch-renderer timings on this scene (bench/results/run2/summary.txt:20-22, :41-43:
@@ -337,6 +344,15 @@ def bench_posed_bodies(args, torch):
         (0.5 * (diff * diff).sum()).backward()
         return p.grad
 
+    if not args.no_tune:
+        # the forward's walk and tiles on these cameras, then the backward kernel's variant on a real gradient
+        r.autotune(cams, cfg["fov"], depth=True, object_ids=False, reps=2, cache=False)
+        d = render_depth(r, poses, cams, cfg["fov"]).detach()
+        both = torch.isfinite(d) & torch.isfinite(target)
+        g = torch.where(both, d - target, torch.zeros_like(d)).contiguous()
+        tuned = r.autotune_backward(poses, cams, cfg["fov"], d, g, reps=3)
+        print(f"# backward autotuned: {tuned}", file=sys.stderr, flush=True)
+
     steps = max(5, args.steps // 2)
     t_f = timed_steps(fwd, steps, max(2, args.warmup), torch)
     t_fb = timed_steps(fwd_bwd, steps, max(2, args.warmup), torch)
@@ -350,7 +366,8 @@ def bench_posed_bodies(args, torch):
     nbytes = backward_bytes(cfg["worlds"], cfg["width"], cfg["height"], cfg["worlds"] * len(cfg["instances"]), cfg["worlds"],
                             cfg["bodies"], parts)
     return dict(fwd_images_per_s=cfg["worlds"] / t_f, fwd_bwd_images_per_s=cfg["worlds"] / t_fb, ratio=t_f / t_fb,
-                worlds=cfg["worlds"], backward_kernel_ms=float(np.mean(kernel_ms)), backward_bytes=nbytes)
+                worlds=cfg["worlds"], backward_kernel_ms=float(np.mean(kernel_ms)), backward_bytes=nbytes,
+                trace_config=r.trace_config)
 
 
 def cpu_threads():
@@ -553,7 +570,8 @@ def main():
         line.update(instanced_worlds_64x64_images_per_s=c1["images_per_s"], instanced_worlds_64x64_ms=c1["ms"],
                     instanced_worlds_reference_claim_images_per_s=c1["reference_claim_images_per_s"])
     if c3:
-        line.update(lidar_rays_per_s=c3["rays_per_s"], lidar_ms=c3["ms"], lidar_hit_fraction=c3["hit_frac"])
+        line.update(lidar_rays_per_s=c3["rays_per_s"], lidar_ms=c3["ms"], lidar_hit_fraction=c3["hit_frac"],
+                    lidar_trace_config=c3["trace_config"])
     if c4:
         line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])
@@ -562,8 +580,10 @@ def main():
             "bound": "hbm", "kernel": "depth_backward_kernel", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": bw / HBM_PEAK_GBS, "traffic": _profile_summary("depth_backward_kernel_posed_bodies_traffic.json").get("hbm_bytes_per_launch"),
             "algorithmic_bytes_per_launch": c4["backward_bytes"], "kernel_ms": c4["backward_kernel_ms"]}
+        line["fwd_bwd_trace_config"] = c4["trace_config"]
     if crgb:
-        line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"])
+        line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"],
+                    rgb_depth_id_trace_config=crgb["trace_config"])
     if cscene:
         line["falling_bodies_rgb"] = cscene
     line["cpu_baseline"] = cpu

@@ -264,6 +264,11 @@ const char *ngx_last_error(void);
  *              of their directions, so a wavefront's rays are neighbours; results are in the caller's order. */
 typedef struct {
     int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles, sort_rays;
+    /* Kernel variants, by the workgroups of 256 threads per compute unit each is built for (the registers a lane
+     * may hold follow from it; 0 picks the walk's default): the depth / normal / ID kernel 5, 6 or 7 (default 7),
+     * the RGB kernel 4, 5, 6 or 7 (default 4 for the packet walk, 7 for the hybrid one), the ray-query kernel 5, 6
+     * or 7 (default 6), the depth backward 4, 5 or 6 (default 5). The per-lane walk has one variant of each. */
+    int32_t render_blocks, rgb_blocks, ray_blocks, backward_blocks;
 } ngx_trace_config;
 ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config);
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);
@@ -274,6 +279,16 @@ ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);
  * them. */
 ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
                         void *stream, ngx_trace_config *best, float *best_ms);
+
+/* No counterpart in the reference: the same for ray queries (direction order on or off, the ray kernel's
+ * variant) on the caller's rays, and for the depth backward (its kernel's variant) on the caller's arguments of
+ * ngx_pose_backward_depth; the rest of the configuration is kept. */
+ngx_status ngx_autotune_rays(ngx_ctx *ctx, const float *origins, const float *directions, int32_t rays_per_world,
+                             float max_distance, ngx_memory memory, int32_t reps, void *stream, ngx_trace_config *best,
+                             float *best_ms);
+ngx_status ngx_autotune_backward(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov, const float *depth,
+                                 const float *grad_depth, float *grad_out, int32_t reps, void *stream, ngx_trace_config *best,
+                                 float *best_ms);
 
 /* ---- build intermediates, for checking the top-level build against the reference algorithm ---- */
 

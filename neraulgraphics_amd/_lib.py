@@ -65,7 +65,8 @@ assert ctypes.sizeof(Material) == 28 and ctypes.sizeof(Light) == 56
 class TraceConfig(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("xcd_remap", ctypes.c_int32), ("packet", ctypes.c_int32),
                 ("pair_limit", ctypes.c_int32), ("persistent", ctypes.c_int32), ("lane_min_triangles", ctypes.c_int32),
-                ("sort_rays", ctypes.c_int32)]
+                ("sort_rays", ctypes.c_int32), ("render_blocks", ctypes.c_int32), ("rgb_blocks", ctypes.c_int32),
+                ("ray_blocks", ctypes.c_int32), ("backward_blocks", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -102,6 +103,10 @@ _SIGNATURES = {
     "ngx_instance_count": (ctypes.c_int32, [_VP]),
     "ngx_device_id": (ctypes.c_int32, [_VP]),
     "ngx_last_error": (ctypes.c_char_p, []),
+    "ngx_autotune_rays": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, _VP,
+                                           ctypes.POINTER(TraceConfig), ctypes.POINTER(ctypes.c_float)]),
+    "ngx_autotune_backward": (ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_float, _VP, _VP, _VP, ctypes.c_int32, _VP,
+                                               ctypes.POINTER(TraceConfig), ctypes.POINTER(ctypes.c_float)]),
     "ngx_set_trace_config": (ctypes.c_int32, [_VP, ctypes.POINTER(TraceConfig)]),
     "ngx_get_trace_config": (ctypes.c_int32, [_VP, ctypes.POINTER(TraceConfig)]),
     "ngx_set_shading": (ctypes.c_int32, [_VP, ctypes.POINTER(Shading)]),

@@ -235,13 +235,15 @@ class BatchRenderer:
 
     @_serialized
     def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000,
-                         sort_rays=True):
+                         sort_rays=True, render_blocks=0, rgb_blocks=0, ray_blocks=0, backward_blocks=0):
         """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
         default) or per-lane (False) walks, (ray, triangle) pairs batched before testing (1..64), the
         persistent fixed-grid launch, the triangle count from which a mesh's bottom level is walked lane by lane
-        inside the packet walk, and whether ray queries trace each world's rays in an order of their directions."""
+        inside the packet walk, whether ray queries trace each world's rays in an order of their directions, and
+        the kernel variants by workgroups per compute unit (0: the walk's default; see ngx_trace_config)."""
         c = _lib.TraceConfig(int(tile_w), int(bool(xcd_remap)), int(bool(packet)), int(pair_limit), int(bool(persistent)),
-                             int(lane_min_triangles), int(bool(sort_rays)))
+                             int(lane_min_triangles), int(bool(sort_rays)), int(render_blocks), int(rgb_blocks),
+                             int(ray_blocks), int(backward_blocks))
         _lib.check(self._lib.ngx_set_trace_config(self._h, ctypes.byref(c)))
 
     @property
@@ -251,7 +253,7 @@ class BatchRenderer:
         return c.as_dict()
 
     @_serialized
-    def autotune(self, cameras, fov=1.0, *, depth=True, normals=False, object_ids=True, reps=3, cache=True):
+    def autotune(self, cameras, fov=1.0, *, depth=True, normals=False, object_ids=True, rgb=False, reps=3, cache=True):
         """Times every trace configuration worth trying on these cameras and keeps the fastest (see
         ngx_autotune). With cache, the choice is remembered in a JSON file (NGX_TUNE_CACHE, default
         ~/.cache/neraulgraphics_amd/autotune.json) under a key made of the device, the frame, the scene's sizes
@@ -260,7 +262,7 @@ class BatchRenderer:
         import json
         import os
         outputs = (_lib.OUTPUT_DEPTH if depth else 0) | (_lib.OUTPUT_NORMAL if normals else 0) | \
-                  (_lib.OUTPUT_OBJECT_ID if object_ids else 0)
+                  (_lib.OUTPUT_OBJECT_ID if object_ids else 0) | (_lib.OUTPUT_RGB if rgb else 0)
         path = os.environ.get("NGX_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "neraulgraphics_amd", "autotune.json"))
         key = None
         if cache:
@@ -299,6 +301,44 @@ class BatchRenderer:
             except OSError:
                 pass
         return dict(chosen, ms=float(ms.value))
+
+    @_serialized
+    def autotune_rays(self, origins, directions, max_distance=100.0, reps=3):
+        """Times ray queries on these rays with the direction order on and off and every variant of the ray kernel
+        (ngx_autotune_rays), keeps the fastest and returns the configuration with its time per query (ms)."""
+        w = self._worlds
+        n = origins.numel() if _is_device_tensor(origins) else np.asarray(origins).size
+        if n % (3 * w) != 0:
+            raise ValueError(f"origins must hold 3 floats for each ray of each of the {w} worlds")
+        r = n // (3 * w)
+        po, mo, ko = _as_input(origins, "origins (world_count, rays_per_world, 3)", 3 * w * r)
+        pd, md, kd = _as_input(directions, "directions (world_count, rays_per_world, 3)", 3 * w * r)
+        if mo != md:
+            raise ValueError("origins and directions must both be on the host or both on the GPU")
+        best, ms = _lib.TraceConfig(), ctypes.c_float()
+        _lib.check(self._lib.ngx_autotune_rays(self._h, po, pd, int(r), float(max_distance), mo, int(reps), self._stream_for(mo),
+                                               ctypes.byref(best), ctypes.byref(ms)))
+        return dict(best.as_dict(), ms=float(ms.value))
+
+    @_serialized
+    def autotune_backward(self, body_poses, cameras, fov, depth, grad_depth, reps=3):
+        """Times the depth backward on these arguments (as pose_backward_depth takes them) with every variant of
+        its kernel (ngx_autotune_backward), keeps the fastest and returns the configuration with its time (ms)."""
+        t = _torch()
+        if self._bodies is None:
+            raise ValueError("the pose backward needs bind_bodies first")
+        views, dev = self.view_count, self._device_id
+        pixels = views * self._height * self._width
+        poses = _device_operand(body_poses, "body_poses", self._worlds * self._bodies * 7, dev)
+        cams = _device_operand(cameras, "cameras", views * 12, dev)
+        d = _device_operand(depth, "depth", pixels, dev)
+        gd = _device_operand(grad_depth, "grad_depth", pixels, dev)
+        grad = t.empty((self._worlds, self._bodies, 7), dtype=t.float32, device=poses.device)
+        best, ms = _lib.TraceConfig(), ctypes.c_float()
+        _lib.check(self._lib.ngx_autotune_backward(self._h, poses.data_ptr(), cams.data_ptr(), float(fov), d.data_ptr(),
+                                                   gd.data_ptr(), grad.data_ptr(), int(reps), _stream(), ctypes.byref(best),
+                                                   ctypes.byref(ms)))
+        return dict(best.as_dict(), ms=float(ms.value))
 
     # ---- poses
     @_serialized

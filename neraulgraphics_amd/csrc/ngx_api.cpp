@@ -180,6 +180,8 @@ struct ngx_ctx {
     bool timed = false;
     hipStream_t last_stream = nullptr;
     int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000, sort_rays = 1;
+    // kernel variants by workgroups per compute unit (0: each walk's default)
+    int render_blocks = 0, rgb_blocks = 0, ray_blocks = 0, backward_blocks = 0;
     int compute_units = 0;              // sizes the persistent launch
     int *d_tile_counters = nullptr;
     int *d_overflow = nullptr;          // set by a walk whose stack would pass its capacity; read by ngx_synchronize
@@ -881,6 +883,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
         a.object_id = (outputs & NGX_OUTPUT_OBJECT_ID) ? ctx->d_id : nullptr;
         a.xcd_remap = ctx->xcd_remap;
         a.walk = ctx->walk();
+        a.blocks = ctx->render_blocks;
         uint8_t *rgb = nullptr;
         if (outputs & NGX_OUTPUT_RGB) {
             if (!ctx->d_rgb) device_alloc(ctx->d_rgb, 3 * pixels);
@@ -900,6 +903,7 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
             g.rgb = rgb;
             g.spp = spp;
             g.frame = frame;
+            g.blocks = ctx->rgb_blocks;
             HIP_CHECK(ngx::launch_rgb(g, s, ctx->persistent ? ctx->compute_units : 0));
         }
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
@@ -959,6 +963,7 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
         a.normal = ctx->d_hit_normal;
         a.object_id = ctx->d_hit_id;
         a.walk = ctx->walk();
+        a.blocks = ctx->ray_blocks;
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
         if (ctx->sort_rays && a.walk != 0 && rays_per_world >= 1024 && rays_per_world <= ngx::RAY_ORDER_MAX_RAYS) {
@@ -1030,6 +1035,7 @@ void pose_backward_depth(ngx_ctx *ctx, const float *body_poses, const float *cam
     a.parts = parts;
     a.partial = partial;
     a.walk = ctx->walk();
+    a.blocks = ctx->backward_blocks;
     // the backward kernel alone is what ngx_last_kernel_ms reports after this call
     const bool timed = !ctx->capturing(s);
     if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
@@ -1149,6 +1155,74 @@ int32_t ngx_instances_per_world(const ngx_ctx *ctx) { return ctx ? ctx->per_worl
 int32_t ngx_instance_count(const ngx_ctx *ctx) { return ctx ? ctx->total : 0; }
 int32_t ngx_device_id(const ngx_ctx *ctx) { return ctx ? ctx->device : 0; }
 
+}  // extern "C"
+
+namespace {
+
+template <size_t N>
+bool built_variant(const int blocks, const int (&list)[N]) {
+    if (blocks == 0) return true;
+    for (const int b : list)
+        if (b == blocks) return true;
+    return false;
+}
+
+ngx_trace_config current_config(const ngx_ctx *ctx) {
+    return ngx_trace_config{ctx->tile_w,      ctx->xcd_remap,  ctx->packet,     ctx->pair_limit,
+                            ctx->persistent,  ctx->lane_min_tris, ctx->sort_rays, ctx->render_blocks,
+                            ctx->rgb_blocks,  ctx->ray_blocks, ctx->backward_blocks};
+}
+
+// Times `runs` calls of `call` on the stream after one untimed call; returns ms per call, or -1 when a call fails
+// (the failure's status in *failure).
+template <class Call>
+float time_calls(ngx_ctx *ctx, const hipStream_t s, const int reps, ngx_status *failure, Call &&call) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    float ms = -1.0f;
+    *failure = call();
+    if (*failure == NGX_OK) {
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && *failure == NGX_OK; ++r) *failure = call();
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        if (*failure == NGX_OK) {
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)reps;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)ctx;
+    return ms;
+}
+
+// Tries each configuration with `call` and keeps the fastest; on a failure the context gets back `start` and the
+// failure is raised.
+template <class Call>
+void keep_fastest(ngx_ctx *ctx, const std::vector<ngx_trace_config> &cands, const ngx_trace_config &start, const hipStream_t s,
+                  const int reps, ngx_trace_config &winner, float &best_time, Call &&call) {
+    for (const ngx_trace_config &c : cands) {
+        ngx_set_trace_config(ctx, &c);
+        ngx_status failure = NGX_OK;
+        const float ms = time_calls(ctx, s, reps, &failure, call);
+        if (failure != NGX_OK) {
+            const std::string message = g_last_error;
+            ngx_set_trace_config(ctx, &start);
+            fail(failure, "autotune: " + message);
+        }
+        if (best_time <= 0.0f || ms < best_time) {
+            best_time = ms;
+            winner = c;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
 ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
     return guarded([&] {
         require(ctx != nullptr && config != nullptr, "context and config must not be null");
@@ -1156,6 +1230,10 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
         require(c.tile_w == 8 || c.tile_w == 16 || c.tile_w == 32 || c.tile_w == 64, "tile_w must be 8, 16, 32 or 64");
         require(c.pair_limit >= 1 && c.pair_limit <= 64, "pair_limit must be 1 to 64");
         require(c.lane_min_triangles >= 0, "lane_min_triangles must not be negative");
+        require(built_variant(c.render_blocks, ngx::RENDER_BLOCKS), "render_blocks must be 0, 5, 6 or 7");
+        require(built_variant(c.rgb_blocks, ngx::RGB_BLOCKS), "rgb_blocks must be 0, 4, 5, 6 or 7");
+        require(built_variant(c.ray_blocks, ngx::RAY_BLOCKS), "ray_blocks must be 0, 5, 6 or 7");
+        require(built_variant(c.backward_blocks, ngx::BACKWARD_BLOCKS), "backward_blocks must be 0, 4, 5 or 6");
         ctx->tile_w = c.tile_w;
         ctx->xcd_remap = c.xcd_remap ? 1 : 0;
         ctx->packet = c.packet ? 1 : 0;
@@ -1163,73 +1241,135 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
         ctx->persistent = c.persistent ? 1 : 0;
         ctx->lane_min_tris = c.lane_min_triangles;
         ctx->sort_rays = c.sort_rays ? 1 : 0;
+        ctx->render_blocks = c.render_blocks;
+        ctx->rgb_blocks = c.rgb_blocks;
+        ctx->ray_blocks = c.ray_blocks;
+        ctx->backward_blocks = c.backward_blocks;
     });
 }
 
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config) {
     return guarded([&] {
         require(ctx != nullptr && config != nullptr, "context and config must not be null");
-        *config = ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent, ctx->lane_min_tris,
-                                   ctx->sort_rays};
+        *config = current_config(ctx);
     });
 }
 
 ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
                         void *stream, ngx_trace_config *best, float *best_ms) {
-    const ngx_trace_config start = ctx ? ngx_trace_config{ctx->tile_w, ctx->xcd_remap, ctx->packet, ctx->pair_limit, ctx->persistent,
-                                                          ctx->lane_min_tris, ctx->sort_rays}
-                                       : ngx_trace_config{};
-    const ngx_status st = guarded([&] {
+    return guarded([&] {
         require(ctx != nullptr && cams12 != nullptr, "context and cameras must not be null");
         require(reps >= 1 && reps <= 100, "reps must be 1 to 100");
-        require(!ctx->capturing(static_cast<hipStream_t>(stream)), "autotune cannot run while the stream is being captured");
-        // the per-lane bottom level for: no mesh, meshes of 1000+ triangles, of 300+, every mesh
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        require(!ctx->capturing(s), "autotune cannot run while the stream is being captured");
+        ctx->set_device();
+        const ngx_trace_config start = current_config(ctx);
+        auto render = [&] {
+            ngx_batch_frame f;
+            return ngx_render_batch(ctx, cams12, memory, fov, 1, outputs, 0, stream, &f);
+        };
+        // first the walk and tiles, with each walk's default kernel variants: the per-lane bottom level for no
+        // mesh, meshes of 1000+ triangles, of 300+, every mesh
         const int never = 1 << 30;
         std::vector<ngx_trace_config> cands;
         for (const int tw : {8, 16, 32}) {
             for (const int lanes : {never, 1000, 300, 0})
                 for (const int limit : {32, 64})
-                    for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers, lanes, start.sort_rays});
-            if (tw <= 16) cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never, start.sort_rays});
+                    for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers, lanes, start.sort_rays, 0, 0,
+                                                                                   start.ray_blocks, start.backward_blocks});
+            if (tw <= 16)
+                cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never, start.sort_rays, 0, 0, start.ray_blocks, start.backward_blocks});
         }
-        const hipStream_t s = static_cast<hipStream_t>(stream);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        ctx->set_device();
-        HIP_CHECK(hipEventCreate(&e0));
-        HIP_CHECK(hipEventCreate(&e1));
-        float best_time = 0.0f;
         ngx_trace_config winner = start;
-        ngx_status failure = NGX_OK;
-        for (const ngx_trace_config &c : cands) {
-            ngx_set_trace_config(ctx, &c);
-            ngx_batch_frame f;
-            failure = ngx_render_batch(ctx, cams12, memory, fov, 1, outputs, 0, stream, &f);  // warm-up
-            if (failure != NGX_OK) break;
-            (void)hipEventRecord(e0, s);
-            for (int r = 0; r < reps && failure == NGX_OK; ++r) failure = ngx_render_batch(ctx, cams12, memory, fov, 1, outputs, 0, stream, &f);
-            if (failure != NGX_OK) break;
-            (void)hipEventRecord(e1, s);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            ms /= (float)reps;
-            if (best_time == 0.0f || ms < best_time) {
-                best_time = ms;
-                winner = c;
+        float best_time = 0.0f;
+        keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
+        // then, for the winner's walk, the kernel variants: workgroups per compute unit of the depth / normal / ID
+        // kernel and of the RGB kernel (the per-lane walk has one variant of each)
+        if (winner.packet) {
+            const ngx_trace_config base = winner;
+            cands.clear();
+            if (outputs & (NGX_OUTPUT_DEPTH | NGX_OUTPUT_NORMAL | NGX_OUTPUT_OBJECT_ID))
+                for (const int b : ngx::RENDER_BLOCKS) {
+                    ngx_trace_config c = base;
+                    c.render_blocks = b;
+                    cands.push_back(c);
+                }
+            keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
+            if (outputs & NGX_OUTPUT_RGB) {
+                const ngx_trace_config base2 = winner;
+                cands.clear();
+                for (const int b : ngx::RGB_BLOCKS) {
+                    ngx_trace_config c = base2;
+                    c.rgb_blocks = b;
+                    cands.push_back(c);
+                }
+                keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
             }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (failure != NGX_OK) {
-            const std::string message = g_last_error;
-            ngx_set_trace_config(ctx, &start);
-            fail(failure, "autotune: " + message);
         }
         ngx_set_trace_config(ctx, &winner);
         if (best) *best = winner;
         if (best_ms) *best_ms = best_time;
     });
-    return st;
+}
+
+ngx_status ngx_autotune_rays(ngx_ctx *ctx, const float *origins, const float *directions, int32_t rays_per_world,
+                             float max_distance, ngx_memory memory, int32_t reps, void *stream, ngx_trace_config *best,
+                             float *best_ms) {
+    return guarded([&] {
+        require(ctx != nullptr && origins != nullptr && directions != nullptr, "context and rays must not be null");
+        require(reps >= 1 && reps <= 100, "reps must be 1 to 100");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        require(!ctx->capturing(s), "autotune cannot run while the stream is being captured");
+        ctx->set_device();
+        const ngx_trace_config start = current_config(ctx);
+        std::vector<ngx_trace_config> cands;
+        for (const int sorted : {1, 0})
+            for (const int b : ngx::RAY_BLOCKS) {
+                ngx_trace_config c = start;
+                c.sort_rays = sorted;
+                c.ray_blocks = b;
+                cands.push_back(c);
+            }
+        ngx_trace_config winner = start;
+        float best_time = 0.0f;
+        keep_fastest(ctx, cands, start, s, reps, winner, best_time, [&] {
+            ngx_ray_hits h;
+            return ngx_cast_rays_batch(ctx, origins, directions, rays_per_world, max_distance, memory, stream, &h);
+        });
+        ngx_set_trace_config(ctx, &winner);
+        if (best) *best = winner;
+        if (best_ms) *best_ms = best_time;
+    });
+}
+
+ngx_status ngx_autotune_backward(ngx_ctx *ctx, const float *body_poses, const float *cams12, float fov, const float *depth,
+                                 const float *grad_depth, float *grad_out, int32_t reps, void *stream, ngx_trace_config *best,
+                                 float *best_ms) {
+    return guarded([&] {
+        require(ctx != nullptr, "context is null");
+        require(reps >= 1 && reps <= 100, "reps must be 1 to 100");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        require(!ctx->capturing(s), "autotune cannot run while the stream is being captured");
+        ctx->set_device();
+        const ngx_trace_config start = current_config(ctx);
+        std::vector<ngx_trace_config> cands;
+        if (start.packet)
+            for (const int b : ngx::BACKWARD_BLOCKS) {
+                ngx_trace_config c = start;
+                c.backward_blocks = b;
+                cands.push_back(c);
+            }
+        else
+            cands.push_back(start);
+        ngx_trace_config winner = start;
+        float best_time = 0.0f;
+        keep_fastest(ctx, cands, start, s, reps, winner, best_time, [&] {
+            return ngx_pose_backward_depth(ctx, body_poses, cams12, fov, depth, grad_depth, grad_out, stream);
+        });
+        ngx_set_trace_config(ctx, &winner);
+        if (best) *best = winner;
+        if (best_ms) *best_ms = best_time;
+    });
 }
 
 ngx_status ngx_debug_keep_build(ngx_ctx *ctx, int32_t keep) {

@@ -104,6 +104,7 @@ struct RenderArgs {
     int xcd_remap;             // 1: consecutive tiles stay on one XCD
     int walk;                  // 0 per-lane, 1 packet, 2 packet with fine meshes lane by lane
     int *tile_counters;        // persistent mode (non-null): TILE_COUNTER_INTS ints, zeroed by the launch
+    int blocks;                // the kernel variant: workgroups per compute unit it is built for (0: the walk's default)
 };
 
 // The RGB picture's shading inputs, on the device: materials as two float4 (diffuse rgb, diffuse albedo;
@@ -127,6 +128,7 @@ struct RgbArgs {
     ShadeScene shade;
     uint8_t *rgb;                      // [views][H][W][3]
     int spp, frame;
+    int blocks;                        // the RGB kernel's variant (0: the walk's default)
 };
 
 struct RayArgs {
@@ -142,6 +144,7 @@ struct RayArgs {
     int walk;
     const int *order;          // null, or per world the order to trace its rays in (indices within the world)
     int xcd_remap;             // 1: each XCD takes a contiguous range of the waves
+    int blocks;                // the kernel variant (0: the default)
 };
 
 // The pose backward of depth: dL/d(body pose) for every world, through each pixel's fixed hit triangle.
@@ -159,6 +162,7 @@ struct DepthBackwardArgs {
     int parts;                 // workgroups per view, each taking every parts-th group of tiles
     float *partial;            // [views][parts][bodies][7]
     int walk;                  // the re-trace's walk, as the forward's
+    int blocks;                // the kernel variant (0: the default)
 };
 
 struct RangeBackwardArgs {
@@ -184,6 +188,13 @@ struct PoseGradFinishArgs {
     const float *body_poses;   // [worlds][bodies][7]
     float *grad;               // [worlds][bodies][7]
 };
+
+// The variants each kernel is built in, by workgroups of 256 threads per compute unit (its register budget
+// follows: 512 / (blocks x 4 / 4 SIMDs) registers a lane). 0 in a trace configuration picks the default.
+constexpr int RENDER_BLOCKS[] = {5, 6, 7}, RENDER_BLOCKS_DEFAULT = 7;
+constexpr int RGB_BLOCKS[] = {4, 5, 6, 7}, RGB_BLOCKS_DEFAULT_PACKET = 4, RGB_BLOCKS_DEFAULT_HYBRID = 7;
+constexpr int RAY_BLOCKS[] = {5, 6, 7}, RAY_BLOCKS_DEFAULT = 6;
+constexpr int BACKWARD_BLOCKS[] = {4, 5, 6}, BACKWARD_BLOCKS_DEFAULT = 5;
 
 // The pose backward sums gradients per body in LDS: at most this many bodies per world.
 constexpr int BACKWARD_MAX_BODIES = 256;

@@ -880,15 +880,13 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
     if (A.object_id) A.object_id[idx] = object;
 }
 
-// Workgroups per compute unit the hybrid render kernels are built for (their registers follow: 7 gives at most 72
-// VGPRs, with more spilled; body_worlds 10.49 ms at 6, 10.20 ms at 7).
-#ifndef NGX_HYBRID_MIN_BLOCKS
-#define NGX_HYBRID_MIN_BLOCKS 7
-#endif
+// BLOCKS: workgroups per compute unit the kernel is built for (their registers follow: 7 gives at most 72 VGPRs,
+// with more spilled; body_worlds 10.49 ms at 6, 10.20 ms at 7). The packet and hybrid walks are built at 5, 6 and 7
+// and the autotuner picks one; the per-lane walk at 1.
 
 // One tile per wavefront, tiles dealt by the dispatcher.
-template <int WALK>
-__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? NGX_HYBRID_MIN_BLOCKS : 1) void render_kernel(const RenderArgs A) {
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void render_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(A.xcd_remap, A.total_tiles);
     if (tile >= A.total_tiles) return;
@@ -935,8 +933,8 @@ __device__ __forceinline__ void take_tiles(int *counters, const long long total,
     }
 }
 
-template <int WALK>
-__global__ __launch_bounds__(256, WALK == WALK_PACKET ? 7 : WALK == WALK_HYBRID ? NGX_HYBRID_MIN_BLOCKS : 1) void render_persistent_kernel(const RenderArgs A) {
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void render_persistent_kernel(const RenderArgs A) {
     NGX_PAIR_LDS(L);
     take_tiles(A.tile_counters, A.total_tiles, [&](const long long tile) { render_tile<WALK>(A, L, tile); });
 }
@@ -1101,13 +1099,8 @@ __device__ __forceinline__ float srgb_channel(const float v) {
 // packet and per-lane walks, 7 (at most 72 VGPRs, many more spills) for the hybrid walk, whose scenes have more
 // latency to hide (body_worlds RGB: 25.7 ms at 5, 24.3 at 6, 23.5 at 7, 33.0 at 8; the 8-body falling-bodies scene,
 // packet walk, is 3.1 ms at 4 and 4.2 at 6).
-#ifndef NGX_RGB_MIN_BLOCKS
-#define NGX_RGB_MIN_BLOCKS 4
-#endif
-#ifndef NGX_RGB_MIN_BLOCKS_HYBRID
-#define NGX_RGB_MIN_BLOCKS_HYBRID 7
-#endif
-#define NGX_RGB_BLOCKS(W) ((W) == WALK_HYBRID ? NGX_RGB_MIN_BLOCKS_HYBRID : NGX_RGB_MIN_BLOCKS)
+// The packet and hybrid walks' RGB kernels are built at 4, 5, 6 and 7 workgroups per compute unit, the per-lane
+// walk's at 4; the autotuner picks.
 template <int WALK>
 __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const long long tile) {
     const RenderArgs &A = G.r;
@@ -1210,8 +1203,8 @@ __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const lon
     out[2] = (uint8_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
 }
 
-template <int WALK>
-__global__ __launch_bounds__(256, NGX_RGB_BLOCKS(WALK)) void rgb_kernel(const RgbArgs G) {
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void rgb_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
     const long long tile = wave_tile(G.r.xcd_remap, G.r.total_tiles);
     if (tile >= G.r.total_tiles) return;
@@ -1220,8 +1213,8 @@ __global__ __launch_bounds__(256, NGX_RGB_BLOCKS(WALK)) void rgb_kernel(const Rg
 
 // The same tiles from a fixed grid taking them from the per-XCD slice counters, as the persistent depth
 // kernel does.
-template <int WALK>
-__global__ __launch_bounds__(256, NGX_RGB_BLOCKS(WALK)) void rgb_persistent_kernel(const RgbArgs G) {
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void rgb_persistent_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
     take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_tile<WALK>(G, L, tile); });
 }
@@ -1231,11 +1224,9 @@ __global__ __launch_bounds__(256, NGX_RGB_BLOCKS(WALK)) void rgb_persistent_kern
 // wave whose rays fall in several worlds walks each world's rays as its own packet.
 // 6 workgroups (24 wavefronts) per compute unit, at most 84 VGPRs: the ray queries wait on memory more than the
 // render does, and a sixth wavefront per SIMD hides more of it (lidar_sweeps 22.7 -> 21.5 ms; 7 gives 22.6)
-#ifndef NGX_RAY_MIN_BLOCKS
-#define NGX_RAY_MIN_BLOCKS 6
-#endif
-template <int WALK>
-__global__ __launch_bounds__(256, NGX_RAY_MIN_BLOCKS) void ray_kernel(const RayArgs A) {
+// (built at 5, 6 and 7 for the packet and hybrid walks, the autotuner picking; the per-lane walk's at 6)
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void ray_kernel(const RayArgs A) {
     NGX_PAIR_LDS(L);
     const long long base = wave_tile(A.xcd_remap, 0) * 64;
     if (base >= A.count) return;
@@ -1366,11 +1357,9 @@ constexpr int BWD_WAVES = 4;
 
 // 5 workgroups of 4 wavefronts per compute unit (at most 100 VGPRs; posed_bodies forward + backward 746k -> 794k
 // images/s; 6 gives 781k)
-#ifndef NGX_BWD_MIN_BLOCKS
-#define NGX_BWD_MIN_BLOCKS 5
-#endif
-template <int WALK>
-__global__ __launch_bounds__(64 * BWD_WAVES, NGX_BWD_MIN_BLOCKS) void depth_backward_kernel(const DepthBackwardArgs A) {
+// (built at 4, 5 and 6 for the packet and hybrid walks, the autotuner picking; the per-lane walk's at 5)
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(64 * BWD_WAVES, BLOCKS) void depth_backward_kernel(const DepthBackwardArgs A) {
     extern __shared__ float acc_all[];  // [BWD_WAVES][bodies][7]
     NGX_PAIR_LDS(L);
     const int view = blockIdx.x / A.parts, part = blockIdx.x % A.parts;
@@ -1480,11 +1469,19 @@ __global__ void pose_grad_finish_kernel(const PoseGradFinishArgs A) {
 
 }  // namespace
 
-#define NGX_BY_WALK(walk, launch) \
+// A kernel variant by walk and workgroups per compute unit: the per-lane walk has one variant (LANE_BLOCKS), the
+// packet and hybrid walks one per value of the kernel's list; any other value takes the default.
+#define NGX_BY_VARIANT3(walk, blocks, def, lane_blocks, v0, v1, v2, launch) \
     switch (walk) { \
-    case WALK_LANES: launch(WALK_LANES); break; \
-    case WALK_HYBRID: launch(WALK_HYBRID); break; \
-    default: launch(WALK_PACKET); break; \
+    case WALK_LANES: launch(WALK_LANES, lane_blocks); break; \
+    case WALK_HYBRID: \
+        switch (blocks ? blocks : def) { case v0: launch(WALK_HYBRID, v0); break; case v1: launch(WALK_HYBRID, v1); break; \
+                                         default: launch(WALK_HYBRID, v2); break; } \
+        break; \
+    default: \
+        switch (blocks ? blocks : def) { case v0: launch(WALK_PACKET, v0); break; case v1: launch(WALK_PACKET, v1); break; \
+                                         default: launch(WALK_PACKET, v2); break; } \
+        break; \
     }
 
 // The persistent grid: as many workgroups as the device holds at once (the kernel's occupancy times the
@@ -1495,11 +1492,11 @@ static int capped_grid(const int resident, const long long tiles) {
     return (int)(need < resident ? need : resident);
 }
 
-template <int WALK>
+template <int WALK, int BLOCKS>
 static int persistent_grid(const int compute_units, const long long tiles) {
     static const int per_cu = [] {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, render_persistent_kernel<WALK>, 256, 0) != hipSuccess || n < 1) n = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, render_persistent_kernel<WALK, BLOCKS>, 256, 0) != hipSuccess || n < 1) n = 1;
         return n;
     }();
     return capped_grid((per_cu * compute_units + 7) / 8 * 8, tiles);
@@ -1510,14 +1507,14 @@ hipError_t launch_render(const RenderArgs &a, hipStream_t stream, const int comp
     if (a.tile_counters && compute_units > 0) {
         if (a.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.tile_counters);
-#define NGX_LAUNCH(W) render_persistent_kernel<W><<<persistent_grid<W>(compute_units, a.total_tiles), 256, 0, stream>>>(a)
-        NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#define NGX_LAUNCH(W, B) render_persistent_kernel<W, B><<<persistent_grid<W, B>(compute_units, a.total_tiles), 256, 0, stream>>>(a)
+        NGX_BY_VARIANT3(a.walk, a.blocks, RENDER_BLOCKS_DEFAULT, 1, 5, 6, 7, NGX_LAUNCH)
 #undef NGX_LAUNCH
         return hipGetLastError();
     }
     const long long blocks = (a.total_tiles + 3) / 4;
-#define NGX_LAUNCH(W) render_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
-    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#define NGX_LAUNCH(W, B) render_kernel<W, B><<<(unsigned)blocks, 256, 0, stream>>>(a)
+    NGX_BY_VARIANT3(a.walk, a.blocks, RENDER_BLOCKS_DEFAULT, 1, 5, 6, 7, NGX_LAUNCH)
 #undef NGX_LAUNCH
     return hipGetLastError();
 }
@@ -1527,21 +1524,40 @@ hipError_t launch_render_stats(const RenderArgs &a, int *out, hipStream_t stream
     if (a.total_tiles <= 0) return hipSuccess;
     const long long blocks = (a.total_tiles + 3) / 4;
 #define NGX_LAUNCH(W) render_stats_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a, out)
-    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+    switch (a.walk) {
+    case WALK_LANES: NGX_LAUNCH(WALK_LANES); break;
+    case WALK_HYBRID: NGX_LAUNCH(WALK_HYBRID); break;
+    default: NGX_LAUNCH(WALK_PACKET); break;
+    }
 #undef NGX_LAUNCH
     return hipGetLastError();
 }
 #endif
 
-template <int WALK>
+template <int WALK, int BLOCKS>
 static int rgb_persistent_grid(const int compute_units, const long long tiles) {
     static const int per_cu = [] {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rgb_persistent_kernel<WALK>, 256, 0) != hipSuccess || n < 1) n = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rgb_persistent_kernel<WALK, BLOCKS>, 256, 0) != hipSuccess || n < 1) n = 1;
         return n;
     }();
     return capped_grid((per_cu * compute_units + 7) / 8 * 8, tiles);
 }
+
+// The RGB kernels' variants: 4, 5, 6 or 7 workgroups per compute unit for the packet and hybrid walks, 4 for the
+// per-lane walk; defaults 4 for the packet walk and 7 for the hybrid walk.
+#define NGX_RGB_VARIANT(walk, blocks, launch) \
+    switch (walk) { \
+    case WALK_LANES: launch(WALK_LANES, 4); break; \
+    case WALK_HYBRID: \
+        switch (blocks ? blocks : RGB_BLOCKS_DEFAULT_HYBRID) { case 4: launch(WALK_HYBRID, 4); break; \
+            case 5: launch(WALK_HYBRID, 5); break; case 6: launch(WALK_HYBRID, 6); break; default: launch(WALK_HYBRID, 7); break; } \
+        break; \
+    default: \
+        switch (blocks ? blocks : RGB_BLOCKS_DEFAULT_PACKET) { case 5: launch(WALK_PACKET, 5); break; \
+            case 6: launch(WALK_PACKET, 6); break; case 7: launch(WALK_PACKET, 7); break; default: launch(WALK_PACKET, 4); break; } \
+        break; \
+    }
 
 hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_units) {
     if (a.r.total_tiles <= 0) return hipSuccess;
@@ -1549,14 +1565,14 @@ hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_un
     if (a.r.tile_counters && compute_units > 0) {
         if (a.r.total_tiles >= INT32_MAX / 2) return hipErrorInvalidValue;
         zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
-#define NGX_LAUNCH(W) rgb_persistent_kernel<W><<<rgb_persistent_grid<W>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
-        NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
+#define NGX_LAUNCH(W, B) rgb_persistent_kernel<W, B><<<rgb_persistent_grid<W, B>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
+        NGX_RGB_VARIANT(a.r.walk, a.blocks, NGX_LAUNCH)
 #undef NGX_LAUNCH
         return hipGetLastError();
     }
     const long long blocks = (a.r.total_tiles + 3) / 4;
-#define NGX_LAUNCH(W) rgb_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
-    NGX_BY_WALK(a.r.walk, NGX_LAUNCH)
+#define NGX_LAUNCH(W, B) rgb_kernel<W, B><<<(unsigned)blocks, 256, 0, stream>>>(a)
+    NGX_RGB_VARIANT(a.r.walk, a.blocks, NGX_LAUNCH)
 #undef NGX_LAUNCH
     return hipGetLastError();
 }
@@ -1565,8 +1581,8 @@ hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
     if (a.count <= 0) return hipSuccess;
     const long long waves = (a.count + 63) / 64;
     const long long blocks = (waves + 3) / 4;
-#define NGX_LAUNCH(W) ray_kernel<W><<<(unsigned)blocks, 256, 0, stream>>>(a)
-    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#define NGX_LAUNCH(W, B) ray_kernel<W, B><<<(unsigned)blocks, 256, 0, stream>>>(a)
+    NGX_BY_VARIANT3(a.walk, a.blocks, RAY_BLOCKS_DEFAULT, 6, 5, 6, 7, NGX_LAUNCH)
 #undef NGX_LAUNCH
     return hipGetLastError();
 }
@@ -1575,8 +1591,8 @@ hipError_t launch_depth_backward(const DepthBackwardArgs &a, hipStream_t stream)
     if (a.views <= 0) return hipSuccess;
     if (a.bodies_per_world <= 0 || a.bodies_per_world > BACKWARD_MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
-#define NGX_LAUNCH(W) depth_backward_kernel<W><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a)
-    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+#define NGX_LAUNCH(W, B) depth_backward_kernel<W, B><<<a.views * a.parts, 64 * BWD_WAVES, lds, stream>>>(a)
+    NGX_BY_VARIANT3(a.walk, a.blocks, BACKWARD_BLOCKS_DEFAULT, 5, 4, 5, 6, NGX_LAUNCH)
 #undef NGX_LAUNCH
     return hipGetLastError();
 }
@@ -1586,7 +1602,11 @@ hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream)
     if (a.bodies_per_world <= 0 || a.bodies_per_world > BACKWARD_MAX_BODIES) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * BWD_WAVES * 7 * (size_t)a.bodies_per_world;
 #define NGX_LAUNCH(W) range_backward_kernel<W><<<a.worlds * a.chunks, 64 * BWD_WAVES, lds, stream>>>(a)
-    NGX_BY_WALK(a.walk, NGX_LAUNCH)
+    switch (a.walk) {
+    case WALK_LANES: NGX_LAUNCH(WALK_LANES); break;
+    case WALK_HYBRID: NGX_LAUNCH(WALK_HYBRID); break;
+    default: NGX_LAUNCH(WALK_PACKET); break;
+    }
 #undef NGX_LAUNCH
     return hipGetLastError();
 }

@@ -135,10 +135,54 @@ def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_l
     r = workloads.make_renderer(cfg)
     r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
     assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent),
-                                  lane_min_triangles=1000, sort_rays=1)
+                                  lane_min_triangles=1000, sort_rays=1, render_blocks=0, rgb_blocks=0, ray_blocks=0,
+                                  backward_blocks=0)
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
     _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "body_worlds"))
+
+
+def test_every_kernel_variant_gives_the_same_results(gpu):
+    """The kernels built for 4 to 7 workgroups per compute unit differ only in how many registers they may hold (and so
+    in what they spill): depth, normal, ID and RGB images, ray-query results and pose gradients are bit-identical
+    across every variant of both packet walks, and a variant that is not built is refused."""
+    import torch
+    import neraulgraphics_amd as ng
+    _say("kernel variants: body_worlds 12 worlds, rays, backward")
+    cfg = workloads.make("body_worlds", worlds=12)
+    r = workloads.make_renderer(cfg, floor=ng.Plane(y=-6.0, half_extent_x=40.0, z_min=-60.0, z_max=20.0),
+                                lights=[ng.Light((0.0, 12.0, -6.0), intensity=1.2, radius=0.5)])
+    cams = torch.from_numpy(cfg["cameras"]).cuda()
+    rng = np.random.default_rng(17)
+    o = torch.from_numpy(np.repeat(cfg["cameras"].reshape(12, 3, 4)[:, None, :, 3], 4096, axis=1).astype(np.float32)).cuda()
+    d = torch.from_numpy(rng.normal(size=(12, 4096, 3)).astype(np.float32)).cuda()
+    poses = torch.from_numpy(cfg["body_poses"]).cuda()
+
+    def results(**variant):
+        out = {}
+        for walk in ("packet", "hybrid"):
+            r.set_trace_config(**WALKS[walk], **variant)
+            img = r.render(cfg["cameras"], cfg["fov"], 1, rgb=True, depth=True, normals=True, object_ids=True, frame=2)
+            rays = r.cast_rays(o, d, 50.0)
+            depth = torch.from_numpy(img["depth"]).cuda()
+            g = torch.where(torch.isfinite(depth), torch.ones_like(depth), torch.zeros_like(depth))
+            grad = r.pose_backward_depth(poses, cams, cfg["fov"], depth, g)
+            out[walk] = [img["rgb"], img["depth"], img["normals"], img["object_ids"], rays["distance"].cpu().numpy(),
+                         rays["object_id"].cpu().numpy(), grad.cpu().numpy()]
+        return out
+
+    base = results()
+    for key, values in (("render_blocks", (5, 6, 7)), ("rgb_blocks", (4, 5, 6, 7)), ("ray_blocks", (5, 6, 7)),
+                        ("backward_blocks", (4, 5, 6))):
+        for v in values:
+            got = results(**{key: v})
+            for walk in base:
+                for a, b in zip(base[walk], got[walk]):
+                    assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), (key, v, walk)
+    with pytest.raises(ValueError, match="render_blocks"):
+        r.set_trace_config(render_blocks=8)
+    with pytest.raises(ValueError, match="backward_blocks"):
+        r.set_trace_config(backward_blocks=7)
 
 
 def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, monkeypatch):
@@ -669,17 +713,25 @@ def test_range_gradient_on_a_lidar_sweeps_sample(gpu, oracle):
     moved[..., :3] += np.random.default_rng(43).normal(0.0, 0.02, moved[..., :3].shape).astype(np.float32)
     r.set_body_poses(moved)
     target = r.cast_rays(lc["origins"], lc["directions"], lc["max_distance"])["distance"].copy()
+    # the oracle's hits, and the rays whose hit is a tie (two triangles met at one distance, through their shared
+    # edge: the same distance, another normal), where the two walks may keep different triangles; their loss weight is
+    # set to zero on both sides, so both gradients go through the same triangles
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], lc["origins"], lc["directions"], lc["max_distance"])
+    r.set_body_poses(poses)
+    fwd = r.cast_rays(lc["origins"], lc["directions"], lc["max_distance"])
+    rep = compare_rays(fwd, ref, "lidar_sweeps sample, 3 worlds")
+    tie = np.any(np.abs(fwd["normal"] - ref["normal"]) > 1e-5, axis=-1) | (fwd["object_id"] != ref["object_id"])
+    _say(f"{rep}; tie rays left out of the loss: {int(tie.sum())}")
     p = torch.from_numpy(poses).cuda().requires_grad_(True)
     dist = cast_range(r, p, torch.from_numpy(lc["origins"]).cuda(), torch.from_numpy(lc["directions"]).cuda(), lc["max_distance"])
     tgt = torch.from_numpy(target).cuda()
-    both = torch.isfinite(dist) & torch.isfinite(tgt)
+    both = torch.isfinite(dist) & torch.isfinite(tgt) & ~torch.from_numpy(tie).cuda()
     diff = torch.where(both, dist - tgt, torch.zeros_like(dist))
     (0.5 * (diff * diff).sum()).backward()
     g_gpu = p.grad.cpu().numpy().astype(np.float64)
     assert int(both.sum()) > W * R // 10
-    st = forward(cfg, oracle, outputs=())
-    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
-                           st["meshes"]["attrs"], st["world_off"], lc["origins"], lc["directions"], lc["max_distance"])
     d32 = lc["directions"].reshape(-1, 3)
     unit = (d32 * (np.float32(1.0) / np.sqrt((d32 * d32).sum(axis=1, dtype=np.float32)))[:, None]).astype(np.float64)
     args = (lc["origins"].reshape(-1, 3).astype(np.float64), unit, diff.detach().cpu().numpy().reshape(-1).astype(np.float64),
@@ -777,7 +829,8 @@ def test_rgb_depth_id_of_the_full_body_worlds_batch_on_a_sample_of_worlds(gpu, o
     _say(f"{rep}; mean colour difference per channel {mean}; bit-identical pixels {float(np.mean(np.all(diff == 0, axis=-1))):.5f}")
     assert np.all(mean < 0.05)
     assert rep["hits"] > 0.3 * rep["pixels"]
-    assert len(np.unique(ref["rgb"].reshape(-1, 3), axis=0)) > 1000
+    # a picture: grey bodies in light and shadow over a checker floor under the sky gradient, not a flat colour
+    assert len(np.unique(ref["rgb"].reshape(-1, 3), axis=0)) > 200
 
 
 def test_rgb_known_answers_on_the_device(gpu, oracle):
