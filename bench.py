@@ -331,17 +331,20 @@ def bench_posed_bodies(args, torch):
     r.set_body_poses(cfg["target_poses"])
     target = r.render(cams, cfg["fov"], depth=True, zero_copy=True)["depth"].clone()
     poses = torch.from_numpy(cfg["body_poses"]).cuda()
+    ones = torch.ones_like(target)
 
     def fwd():
         with torch.no_grad():
             render_depth(r, poses, cams, cfg["fov"])
 
     def fwd_bwd():
+        # L = 0.5 sum (depth - target)^2 over pixels that hit in both; its gradient is seeded per pixel (ones), so no
+        # reduction runs (a reduction's memset nodes are kept out of the captured graph)
         p = poses.detach().requires_grad_(True)
         d = render_depth(r, p, cams, cfg["fov"])
         both = torch.isfinite(d) & torch.isfinite(target)
         diff = torch.where(both, d - target, torch.zeros_like(d))
-        (0.5 * (diff * diff).sum()).backward()
+        (0.5 * (diff * diff)).backward(ones)
         return p.grad
 
     if not args.no_tune:
@@ -353,9 +356,39 @@ def bench_posed_bodies(args, torch):
         tuned = r.autotune_backward(poses, cams, cfg["fov"], d, g, reps=3)
         print(f"# backward autotuned: {tuned}", file=sys.stderr, flush=True)
 
+    # both steps replayed from HIP graphs (one forward, or one forward, loss and backward), captured after eager warm-up
+    # runs; the replayed gradient is checked against the eager one before timing
+    graphs = {}
+    if not args.no_graph:
+        try:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                for _ in range(2):
+                    fwd()
+                    want = fwd_bwd().clone()
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            for name, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd)):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    out = fn()
+                graphs[name] = (g, out)
+            for _ in range(3):  # every replay, with eager work between, gives the eager gradient
+                graphs["fwd_bwd"][0].replay()
+                torch.cuda.synchronize()
+                if not torch.equal(graphs["fwd_bwd"][1], want):
+                    raise RuntimeError("the replayed gradient differs from the eager one")
+                fwd()
+                torch.cuda.synchronize()
+        except Exception as e:  # graph capture is an optimization; the eager steps are the same work
+            print(f"# fwd+bwd hip graph capture unavailable ({e}); timing eager steps", file=sys.stderr, flush=True)
+            graphs = {}
+    step_f = (lambda: graphs["fwd"][0].replay()) if graphs else fwd
+    step_fb = (lambda: graphs["fwd_bwd"][0].replay()) if graphs else fwd_bwd
     steps = max(5, args.steps // 2)
-    t_f = timed_steps(fwd, steps, max(2, args.warmup), torch)
-    t_fb = timed_steps(fwd_bwd, steps, max(2, args.warmup), torch)
+    t_f = timed_steps(step_f, steps, max(2, args.warmup), torch)
+    t_fb = timed_steps(step_fb, steps, max(2, args.warmup), torch)
     # the backward kernel alone, timed by the library's events on the stream it runs on
     kernel_ms = []
     for _ in range(steps):
@@ -367,7 +400,7 @@ def bench_posed_bodies(args, torch):
                             cfg["bodies"], parts)
     return dict(fwd_images_per_s=cfg["worlds"] / t_f, fwd_bwd_images_per_s=cfg["worlds"] / t_fb, ratio=t_f / t_fb,
                 worlds=cfg["worlds"], backward_kernel_ms=float(np.mean(kernel_ms)), backward_bytes=nbytes,
-                trace_config=r.trace_config)
+                trace_config=r.trace_config, hip_graph=bool(graphs))
 
 
 def cpu_threads():
@@ -581,6 +614,7 @@ def main():
             "frac": bw / HBM_PEAK_GBS, "traffic": _profile_summary("depth_backward_kernel_posed_bodies_traffic.json").get("hbm_bytes_per_launch"),
             "algorithmic_bytes_per_launch": c4["backward_bytes"], "kernel_ms": c4["backward_kernel_ms"]}
         line["fwd_bwd_trace_config"] = c4["trace_config"]
+        line["fwd_bwd_hip_graph"] = c4["hip_graph"]
     if crgb:
         line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"],
                     rgb_depth_id_trace_config=crgb["trace_config"])
