@@ -28,7 +28,14 @@
  *                              src/device_scene.h:114-119; src/cpu_backend.cpp:94-120, :239-272
  *   orc_cast_rays_batch        src/render_core.h:331-349; src/cpu_backend.cpp:274-287
  *                              pinned (both): bit-exact per-ray hits vs the reference traversal core
- *                              (mesh_dev.h) compiled into oracle/_ref; known answers of tests/test_batch.cpp.
+ *                              (mesh_dev.h) compiled into oracle/_ref; known answers of tests/test_batch.cpp
+ *                              and tests/test_renderer.cpp:739-775 (sphere, wall, floor depths and normals);
+ *                              normals (mesh_surface, not linkable here) by the inverse-transpose face normals
+ *                              of sheared boxes and the smooth normals of a unit icosphere.
+ *   orc_render_rgb             src/render_core.h:146-283; src/mesh_dev.h:208-227; src/geom_dev.h:114-184;
+ *                              src/device_scene.h:98-111
+ *                              pinned: closed forms (sky, Lambert, shadow, mirror, Phong highlight, spotlight
+ *                              cone, distance falloff) with each pixel's jitter from an independent PCG stream.
  *   orc_pose_grad_*            no reference (the backward pass does not exist there): FP64 closed form,
  *                              pinned by FP64 central finite differences of the ray-plane distance.
  */
