@@ -179,10 +179,42 @@ def test_every_kernel_variant_gives_the_same_results(gpu):
             for walk in base:
                 for a, b in zip(base[walk], got[walk]):
                     assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), (key, v, walk)
+    r.set_trace_config(render_blocks=6, ray_blocks=5)
+    before = r.trace_config
     with pytest.raises(ValueError, match="render_blocks"):
         r.set_trace_config(render_blocks=8)
     with pytest.raises(ValueError, match="backward_blocks"):
-        r.set_trace_config(backward_blocks=7)
+        r.set_trace_config(render_blocks=5, backward_blocks=7)
+    assert r.trace_config == before  # a refused configuration changes nothing
+
+
+def test_ray_and_backward_autotuners_pick_built_variants_and_keep_results(gpu, oracle):
+    """ngx_autotune_rays and ngx_autotune_backward: each returns a configuration of built variants with a time, leaves
+    it set, and the results under it are the oracle's rays and the untuned gradient, bit for bit."""
+    import torch
+    _say("autotune: rays on lidar_sweeps 8 worlds x 4096 rays; backward on posed_bodies 16 worlds")
+    cfg = workloads.make("lidar_sweeps", worlds=8, rays_per_world=4096)
+    r = workloads.make_renderer(cfg)
+    o, d = torch.from_numpy(cfg["origins"]).cuda(), torch.from_numpy(cfg["directions"]).cuda()
+    got = r.autotune_rays(o, d, cfg["max_distance"], reps=1)
+    assert got["ms"] > 0 and got["ray_blocks"] in (5, 6, 7) and got["sort_rays"] in (0, 1)
+    assert {k: got[k] for k in r.trace_config} == r.trace_config
+    out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
+    st = forward(cfg, oracle, outputs=())
+    ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                           st["meshes"]["attrs"], st["world_off"], cfg["origins"], cfg["directions"], cfg["max_distance"])
+    compare_rays(out, ref, "rays after autotune_rays")
+    pc = workloads.make("posed_bodies", worlds=16)
+    rb = workloads.make_renderer(pc)
+    poses = torch.from_numpy(pc["body_poses"]).cuda()
+    cams = torch.from_numpy(pc["cameras"]).cuda()
+    depth = rb.render(cams, pc["fov"], depth=True, zero_copy=True)["depth"].clone()
+    g = torch.where(torch.isfinite(depth), torch.full_like(depth, 0.5), torch.zeros_like(depth))
+    want = rb.pose_backward_depth(poses, cams, pc["fov"], depth, g).cpu().numpy()
+    tuned = rb.autotune_backward(poses, cams, pc["fov"], depth, g, reps=1)
+    assert tuned["ms"] > 0 and tuned["backward_blocks"] in (4, 5, 6)
+    again = rb.pose_backward_depth(poses, cams, pc["fov"], depth, g).cpu().numpy()
+    assert np.array_equal(want.view(np.uint32), again.view(np.uint32))
 
 
 def test_autotune_picks_a_configuration_and_keeps_parity(gpu, oracle, tmp_path, monkeypatch):
