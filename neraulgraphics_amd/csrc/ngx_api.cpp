@@ -119,7 +119,7 @@ struct ngx_ctx {
     bool top_level_stale = false;       // a mesh moved since the last pose update
     ngx::SkipNode *d_skip = nullptr;
     int *d_mesh_skip_root = nullptr, *d_mesh_tris = nullptr;
-    int max_mesh_tris = 0;
+    int max_mesh_tris = 0, min_mesh_tris = 0;
     ngx::MeshRoot *d_mesh_roots = nullptr;
     float *d_tris = nullptr;            // 9 floats per triangle
     float4 *d_attrs = nullptr;
@@ -292,6 +292,9 @@ struct ngx_ctx {
     // The kernels' walk: per-lane, packet, or packet with the bottom level of meshes of lane_min_tris or more
     // triangles walked lane by lane.
     int walk() const { return !packet ? 0 : (max_mesh_tris >= lane_min_tris ? 2 : 1); }
+    // Ray queries and their backward: when every mesh is walked lane by lane, the walk built without the packet
+    // bottom level (3), which holds fewer registers; the same results.
+    int walk_rays() const { return packet && total > 0 && min_mesh_tris >= lane_min_tris ? 3 : walk(); }
 
     bool capturing(hipStream_t s) const {
         hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
@@ -481,6 +484,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             c.mesh_tri_count = built.tri_counts;
             upload(c.d_mesh_tris, built.tri_counts.data(), built.tri_counts.size());
             for (const int t : built.tri_counts) c.max_mesh_tris = std::max(c.max_mesh_tris, t);
+            c.min_mesh_tris = *std::min_element(built.tri_counts.begin(), built.tri_counts.end());
             {
                 // v0, e1, e2 without the padding float of each: 36 bytes a triangle on the device
                 std::vector<float> packed;
@@ -962,7 +966,7 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
         a.distance = ctx->d_hit_dist;
         a.normal = ctx->d_hit_normal;
         a.object_id = ctx->d_hit_id;
-        a.walk = ctx->walk();
+        a.walk = ctx->walk_rays();
         a.blocks = ctx->ray_blocks;
         const bool timed = !ctx->capturing(s);
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev0, s));
@@ -1087,7 +1091,7 @@ ngx_status ngx_pose_backward_range(ngx_ctx *ctx, const float *body_poses, const 
         a.dirs = directions;
         a.rays_per_world = rays_per_world;
         a.chunks = chunks;
-        a.walk = ctx->walk();
+        a.walk = ctx->walk_rays();
         a.max_distance = max_distance;
         a.distance = distance;
         a.grad_distance = grad_distance;

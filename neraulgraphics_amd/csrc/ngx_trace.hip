@@ -374,7 +374,9 @@ __device__ void walk_shared(const TraceScene &S, PairLds *L, const int g, int to
 
 // One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
 // renormalized so t is shared), then the mesh's own tree.
-template <bool LANES, class Stats>
+// BLAS: how bottom levels are walked: 0 as a packet; 1 lane by lane for meshes of lane_min_tris triangles or more,
+// as a packet otherwise; 2 lane by lane for every mesh (the packet code is then not built, which frees registers).
+template <int BLAS, class Stats>
 __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, const uint64_t mask, const float3 o,
                                const float3 d, float &tmax, LaneHit &h, Stats &stats) {
     stats.add(ST_INSTANCES, 1);
@@ -392,7 +394,8 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     if (root_mask == 0) return;
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     int shared = 0;  // subtrees given to the shared per-lane walk
-    if (LANES && meta.z >= S.lane_min_tris) {
+    constexpr bool LANES = BLAS != 0;
+    if (BLAS == 2 || (BLAS == 1 && meta.z >= S.lane_min_tris)) {
         share_subtree(L, shared, root_mask, meta.w);
         walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
         return;
@@ -474,7 +477,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
 // distance to beat on entry and its closest distance on return.
-template <bool LANES, class Stats>
+template <int BLAS, class Stats>
 __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
                             const uint64_t live, float &tmax, LaneHit &h, Stats &stats) {
     if (live == 0) return;
@@ -483,7 +486,7 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
     // the origin in vector registers: for camera rays it is the same in every lane
     const float3 ov = in_vgpr3(o);
     if (root >= S.total - 1) {
-        visit_instance<LANES>(S, L, root - (S.total - 1), live, ov, d, tmax, h, stats);
+        visit_instance<BLAS>(S, L, root - (S.total - 1), live, ov, d, tmax, h, stats);
         return;
     }
     const float3 inv_d = safe_inverse(d);
@@ -521,7 +524,7 @@ __device__ void trace_world(const TraceScene &S, PairLds *L, const int world, co
             next_ref = 0; next_mask = 0;
         }
         while (next_mask == 0 || next_ref < 0) {
-            if (next_mask != 0) visit_instance<LANES>(S, L, ~next_ref, next_mask, ov, d, tmax, h, stats);
+            if (next_mask != 0) visit_instance<BLAS>(S, L, ~next_ref, next_mask, ov, d, tmax, h, stats);
             if (st.sp == 0) return;
             st.pop(next_ref, next_mask);
         }
@@ -715,7 +718,12 @@ __device__ void trace_world_lanes(const TraceScene &S, const int world, const fl
 // Returns the object, or OBJECT_NONE; t and h describe the hit.
 // The walks: per-lane (both levels lane by lane), packet (both levels as a packet), and hybrid (the packet
 // walk with fine meshes' bottom levels walked lane by lane).
-enum Walk { WALK_LANES = 0, WALK_PACKET = 1, WALK_HYBRID = 2 };
+// The walks: per-lane (both levels lane by lane), packet (both levels as a packet), hybrid (the packet walk with fine
+// meshes' bottom levels walked lane by lane), and shared (the packet top level, every bottom level lane by lane: what
+// the hybrid walk does when every mesh is fine, built without the packet bottom level; ray queries only).
+enum Walk { WALK_LANES = 0, WALK_PACKET = 1, WALK_HYBRID = 2, WALK_SHARED = 3 };
+template <int WALK>
+constexpr int blas_mode() { return WALK == WALK_SHARED ? 2 : WALK == WALK_HYBRID ? 1 : 0; }
 
 template <int WALK, class Stats>
 __device__ __forceinline__ int scene_hit(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
@@ -754,7 +762,7 @@ __device__ __forceinline__ int scene_hit(const TraceScene &S, PairLds *L, const 
         const int lane = lane_id();
         L->hit_inst[lane] = -1;
         wave_lds_order();
-        trace_world<WALK == WALK_HYBRID>(S, L, world, o, d, live, best, h, stats);
+        trace_world<blas_mode<WALK>()>(S, L, world, o, d, live, best, h, stats);
         wave_lds_order();
         h.inst = L->hit_inst[lane];
         if (h.inst >= 0) {
@@ -1067,7 +1075,7 @@ __device__ __forceinline__ bool occluded(const TraceScene &S, PairLds *L, const 
             const int lane = lane_id();
             L->hit_inst[lane] = -1;
             wave_lds_order();
-            trace_world<WALK == WALK_HYBRID>(S, L, world, o, d, rest, best, h, stats);
+            trace_world<blas_mode<WALK>()>(S, L, world, o, d, rest, best, h, stats);
             wave_lds_order();
             if (in_mask(rest) && L->hit_inst[lane] >= 0) blocked = true;
         } else {
@@ -1582,7 +1590,15 @@ hipError_t launch_rays(const RayArgs &a, hipStream_t stream) {
     const long long waves = (a.count + 63) / 64;
     const long long blocks = (waves + 3) / 4;
 #define NGX_LAUNCH(W, B) ray_kernel<W, B><<<(unsigned)blocks, 256, 0, stream>>>(a)
-    NGX_BY_VARIANT3(a.walk, a.blocks, RAY_BLOCKS_DEFAULT, 6, 5, 6, 7, NGX_LAUNCH)
+    if (a.walk == WALK_SHARED) {
+        switch (a.blocks ? a.blocks : RAY_BLOCKS_DEFAULT) {
+        case 5: NGX_LAUNCH(WALK_SHARED, 5); break;
+        case 7: NGX_LAUNCH(WALK_SHARED, 7); break;
+        default: NGX_LAUNCH(WALK_SHARED, 6); break;
+        }
+    } else {
+        NGX_BY_VARIANT3(a.walk, a.blocks, RAY_BLOCKS_DEFAULT, 6, 5, 6, 7, NGX_LAUNCH)
+    }
 #undef NGX_LAUNCH
     return hipGetLastError();
 }
@@ -1605,6 +1621,7 @@ hipError_t launch_range_backward(const RangeBackwardArgs &a, hipStream_t stream)
     switch (a.walk) {
     case WALK_LANES: NGX_LAUNCH(WALK_LANES); break;
     case WALK_HYBRID: NGX_LAUNCH(WALK_HYBRID); break;
+    case WALK_SHARED: NGX_LAUNCH(WALK_SHARED); break;
     default: NGX_LAUNCH(WALK_PACKET); break;
     }
 #undef NGX_LAUNCH
