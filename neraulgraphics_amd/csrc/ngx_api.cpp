@@ -1180,7 +1180,7 @@ ngx_trace_config current_config(const ngx_ctx *ctx) {
 // Times `runs` calls of `call` on the stream after one untimed call; returns ms per call, or -1 when a call fails
 // (the failure's status in *failure).
 template <class Call>
-float time_calls(ngx_ctx *ctx, const hipStream_t s, const int reps, ngx_status *failure, Call &&call) {
+float time_calls(const hipStream_t s, const int reps, ngx_status *failure, Call &&call) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -1198,7 +1198,6 @@ float time_calls(ngx_ctx *ctx, const hipStream_t s, const int reps, ngx_status *
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)ctx;
     return ms;
 }
 
@@ -1210,7 +1209,7 @@ void keep_fastest(ngx_ctx *ctx, const std::vector<ngx_trace_config> &cands, cons
     for (const ngx_trace_config &c : cands) {
         ngx_set_trace_config(ctx, &c);
         ngx_status failure = NGX_OK;
-        const float ms = time_calls(ctx, s, reps, &failure, call);
+        const float ms = time_calls(s, reps, &failure, call);
         if (failure != NGX_OK) {
             const std::string message = g_last_error;
             ngx_set_trace_config(ctx, &start);
