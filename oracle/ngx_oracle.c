@@ -24,6 +24,9 @@
  *                              pinned: spread_bits == loop interleave; sort == numpy stable argsort;
  *                              topology invariants; brute-force split search on tiny inputs.
  *   orc_sah_tlas_*             src/instances.cpp:31-121 (per-world SAH TLAS, refit or rebuild, flatten)
+ *                              pinned: forest node for node vs the reference's bvh.cpp build/refit/cost
+ *                              (oracle/_ref) under the 1.3x rule on walks that straddle it; moving equals
+ *                              building (tests/test_renderer.cpp:481-550).
  *   orc_render_batch           src/render_core.h:78-143, :264-273, :313-327; src/mesh_dev.h:58-205, :238-262;
  *                              src/device_scene.h:114-119; src/cpu_backend.cpp:94-120, :239-272
  *   orc_cast_rays_batch        src/render_core.h:331-349; src/cpu_backend.cpp:274-287
