@@ -146,7 +146,9 @@ typedef struct {
     float sky_up[3], sky_color_up[3], sky_color_down[3], sky_intensity;
 } ngx_shading;
 
-/* make_cuda_backend (src/backend.h:88) with BatchRenderer's constructor checks (src/batch.cpp:47-75). */
+/* make_cuda_backend (src/backend.h:88) with BatchRenderer's constructor checks (src/batch.cpp:47-75). More than
+ * NGX_MAX_TRIANGLES triangles over all meshes are refused with NGX_UNSUPPORTED. */
+#define NGX_MAX_TRIANGLES 16777216
 ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out);
 void ngx_destroy(ngx_ctx *ctx);
 

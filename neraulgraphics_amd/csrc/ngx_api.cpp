@@ -14,6 +14,7 @@
 #include "ngx_kernels.h"
 
 static_assert(ngx::BACKWARD_MAX_BODIES == NGX_BACKWARD_MAX_BODIES, "the header states the backward's body limit");
+static_assert(ngx::MAX_TRIANGLES == NGX_MAX_TRIANGLES, "the header states the triangle limit");
 static_assert(ngx::NOALLOC_SCRATCH_FLOATS == (size_t)NGX_NOALLOC_SCRATCH_FLOATS, "the header states the scratch array's size");
 
 namespace {
@@ -364,6 +365,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
         const float fl[4] = {s.floor_y, s.floor_half_extent_x, s.floor_z_min, s.floor_z_max};
         require(finite_all(fl, 4), "floor has a NaN or infinite value");
         require(s.mesh_count >= 0 && s.instances_per_world >= 0, "mesh and instance counts must not be negative");
+        long long all_triangles = 0;
         for (int m = 0; m < s.mesh_count; ++m) {
             const std::string name = "mesh " + std::to_string(m);
             const int nv = s.vertex_offsets[m + 1] - s.vertex_offsets[m];
@@ -371,6 +373,7 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             require(nv >= 0 && ni > 0, name + " has no triangles");
             require(ni % 3 == 0, name + " has " + std::to_string(ni) + " indices, which is not a multiple of 3");
             require(ni / 3 <= 100000000, name + " has too many triangles");
+            all_triangles += ni / 3;
             require(finite_all(s.vertices + 3 * (size_t)s.vertex_offsets[m], 3 * (size_t)nv), name + " has a NaN or infinite vertex");
             for (int i = 0; i < ni; ++i)
                 require(s.indices[s.index_offsets[m] + i] < (uint32_t)nv,
@@ -379,6 +382,9 @@ ngx_status ngx_create(const ngx_scene_desc *scene, ngx_ctx **out) {
             if (s.normals && s.mesh_has_normals && s.mesh_has_normals[m])
                 require(finite_all(s.normals + 3 * (size_t)s.vertex_offsets[m], 3 * (size_t)nv), name + " has a NaN or infinite normal");
         }
+        if (all_triangles > ngx::MAX_TRIANGLES)
+            fail(NGX_UNSUPPORTED, "the meshes hold " + std::to_string(all_triangles) + " triangles; at most " +
+                                      std::to_string(ngx::MAX_TRIANGLES) + " over all meshes are supported");
         for (int i = 0; i < s.instances_per_world; ++i) {
             const std::string name = "instance " + std::to_string(i);
             require(s.instance_mesh[i] >= 0 && s.instance_mesh[i] < s.mesh_count,

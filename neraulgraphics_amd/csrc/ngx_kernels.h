@@ -90,6 +90,10 @@ struct TraceScene {
 // level for the per-lane walk.
 constexpr int WALK_STACK = 64;
 
+// The batched walks keep a triangle or node index and a lane number in one 32-bit word (index << 6 | lane), so
+// indices stay below 2^25; a tree over n triangles has fewer than 2n nodes, so at most 2^24 triangles over all meshes.
+constexpr long long MAX_TRIANGLES = 1ll << 24;
+
 struct RenderArgs {
     TraceScene s;
     const float *cams;         // [views][12]

@@ -954,6 +954,7 @@ __global__ void zero_counters_kernel(int *c) {
 #ifdef NGX_TRACE_STATS
 // The render kernel's traversal with its counters: each wave writes ST_COUNT ints to its own slot, lane k
 // storing counter k, so no atomics are involved.
+static_assert(ST_COUNT == TRACE_STAT_COUNT && ST_COUNT <= 64, "one counter a lane, as many as the host reads");
 template <int WALK>
 __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, int *out) {
     NGX_PAIR_LDS(L);

@@ -93,6 +93,15 @@ def test_bad_scenes_are_rejected_before_any_device_work(kw, match):
         ng.BatchRenderer(**_scene(**kw))
 
 
+def test_more_triangles_than_the_walks_can_index_are_refused():
+    """The batched walks pack a triangle or node index with a lane number in 32 bits; two meshes of 2^23 + 1 triangles
+    (two more than 2^24 in all) are refused before any tree is built."""
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    big = ng.Mesh(tri, np.tile(np.arange(3, dtype=np.uint32), (1 << 23) + 1))
+    with pytest.raises(NotImplementedError, match="16777218 triangles; at most 16777216"):
+        ng.BatchRenderer(**_scene(meshes=[big, big], instances=[ng.Instance(0), ng.Instance(1)]))
+
+
 def test_textures_are_refused_plainly():
     with pytest.raises(NotImplementedError, match="textured"):
         ng.BatchRenderer(**_scene(textures=[object()]))
