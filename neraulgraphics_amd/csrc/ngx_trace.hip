@@ -36,22 +36,25 @@ enum StatIndex { ST_TLAS_NODES, ST_TLAS_LANES, ST_INSTANCES, ST_INSTANCE_LANES, 
 // at its first hit, so its lane leaves every box after it and the walk ends as soon as every ray is blocked or
 // done; whether a ray is blocked is all that is read (mesh_dev.h:208-227 stops at the first hit too).
 // kHandoff says whether the hybrid walk may hand thin subtrees to the shared per-lane walk; the RGB kernel, whose
-// registers are already spent on shading, walks without it.
+// registers are already spent on shading, walks without it. kTwoChild says which shared per-lane walk runs: the one
+// testing both children of a node a step (depth, IDs, ray queries, the backward), or the one-box-a-step walk of the
+// skip layout, which needs fewer registers (the RGB kernel, whose spills the other raises: body_worlds RGB 34.7 ->
+// 44.4 ms, where the depth kernel gains 10.1 -> 9.3 ms and the LiDAR query 21.1 -> 16.4 ms).
 struct NoStats {
-    static constexpr bool kAnyHit = false, kHandoff = true;
+    static constexpr bool kAnyHit = false, kHandoff = true, kTwoChild = true;
     __device__ __forceinline__ void add(int, int) {}
 };
 struct ShadeRays {
-    static constexpr bool kAnyHit = false, kHandoff = false;
+    static constexpr bool kAnyHit = false, kHandoff = false, kTwoChild = false;
     __device__ __forceinline__ void add(int, int) {}
 };
 struct Occlusion {
-    static constexpr bool kAnyHit = true, kHandoff = false;
+    static constexpr bool kAnyHit = true, kHandoff = false, kTwoChild = false;
     __device__ __forceinline__ void add(int, int) {}
 };
 #ifdef NGX_TRACE_STATS
 struct WaveStats {
-    static constexpr bool kAnyHit = false, kHandoff = true;
+    static constexpr bool kAnyHit = false, kHandoff = true, kTwoChild = true;
     int c[ST_COUNT];
     __device__ __forceinline__ void add(const int k, const int v) { c[k] += v; }
 };
@@ -226,7 +229,8 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 // read from the rays' own lanes. So the few rays that enter an instance keep the whole wave busy. Leaves are
 // queued, and their triangles are tested in batches of up to 64 (ray, triangle) pairs once 64 leaves or 64
 // pairs wait, and at the end; after each batch every walking lane re-reads its ray's closest distance. Every
-// node a ray enters is walked by exactly one lane, so each ray's closest hit is its own.
+// node a ray enters is walked by exactly one lane, so each ray's closest hit is its own. (The RGB kernel walks so;
+// the other kernels use the two-child walk further below, which shares its work the same way.)
 constexpr int SHARE_CAP = 128, LEAF_RING = 128;
 // Queued leaves that trigger a batch of triangle tests (at most 64 pairs a batch; fewer leaves wait for more).
 #ifndef NGX_LEAF_BATCH
@@ -372,10 +376,116 @@ __device__ void walk_shared(const TraceScene &S, PairLds *L, const int g, int to
     test_queued_leaves<true>(S, L, head, queued, g, lo, ld, tmax, stats);
 }
 
+// The shared per-lane walk over the two-child layout the packet walk uses: a step loads one node and tests both of
+// its children, as the reference does (mesh_dev.h:133-150), so a ray's chain of dependent node loads is half as long
+// as in the walk above, whose arrival at a node is its box test. A work item is a subtree whose box its ray is known to
+// enter. A lane entering both children of its node continues with the nearer one and gives the farther to the shared
+// stack in LDS, or, when that is full, keeps it on its own stack; an idle lane first resumes its own entries, then
+// takes the newest given subtree (with its ray's mesh-space data read from the ray's lane). A lane's own entries lie on
+// one path down a tree, one a level at most, and the host refuses bottom-level trees of WALK_STACK levels or more, so
+// WALK_STACK entries hold them. Leaves are queued and tested in batches as in the walk above.
+template <class Stats>
+__device__ void walk_shared_two_child(const TraceScene &S, PairLds *L, const int g, int top, const float3 lo,
+                                      const float3 ld, const float3 inv_ld, float &tmax, Stats &stats) {
+    const int lane = lane_id();
+    int head = 0, queued = 0;
+    int ray = -1, cur = -1;
+    float3 wlo = lo, winv = inv_ld;
+    float wlim = tmax;
+    int own[WALK_STACK];
+    int own_n = 0;
+    while (true) {
+        const uint64_t idle = ballot(ray < 0);
+        const uint64_t resume = idle & ballot(own_n > 0);
+        if (idle != 0 && (top > 0 || resume != 0)) {
+            const uint64_t takers = idle & ~resume;
+            const int n = min(popc64(takers), top);
+            const int rk = rank_in(takers);
+            wave_lds_order();
+            int e = -1;
+            if (in_mask(takers) && rk < n) e = L->share[top - 1 - rk];
+            if (in_mask(resume)) e = own[--own_n];
+            top -= n;
+            if (e >= 0) {
+                ray = e & 63;
+                cur = e >> 6;
+            }
+            const int src = ray >= 0 ? ray : lane;
+            wlo = mk3(__shfl(lo.x, src, 64), __shfl(lo.y, src, 64), __shfl(lo.z, src, 64));
+            winv = mk3(__shfl(inv_ld.x, src, 64), __shfl(inv_ld.y, src, 64), __shfl(inv_ld.z, src, 64));
+            wlim = __shfl(tmax, src, 64);
+        }
+        const uint64_t busy = ballot(ray >= 0);
+        if (busy == 0) break;
+        stats.add(ST_BLAS_NODES, 1);
+        stats.add(ST_BLAS_LANES, popc64(busy));
+        // idle lanes see two empty boxes, and their results are masked out
+        float4 f0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), f1 = f0, f2 = f0;
+        int4 f3 = make_int4(0, 0, 0, 0);
+        if (in_mask(busy)) {
+            const float4 *np = reinterpret_cast<const float4 *>(S.blas_wide + cur);
+            f0 = np[0];
+            f1 = np[1];
+            f2 = np[2];
+            f3 = *reinterpret_cast<const int4 *>(np + 3);
+        }
+        const float lim = box_limit(wlim);
+        float dl, dr;
+        const uint64_t hl = busy & ballot(box_hit(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, wlo, winv, lim, dl));
+        const uint64_t hr = busy & ballot(box_hit(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, wlo, winv, lim, dr));
+        const uint64_t lleaf = ballot(f3.x < 0), rleaf = ballot(f3.y < 0);
+        const uint64_t ll = hl & lleaf, rl = hr & rleaf;    // leaves entered
+        const uint64_t li = hl & ~lleaf, ri = hr & ~rleaf;  // interior children entered
+        const int nl = popc64(ll), nr = popc64(rl);
+        if (nl + nr != 0) {
+            // a step queues at most 128 leaves, so the ring holds them once what waits is tested
+            if (queued + nl + nr > LEAF_RING) {
+                test_queued_leaves<true>(S, L, head, queued, g, lo, ld, tmax, stats);
+                wlim = __shfl(tmax, ray >= 0 ? ray : lane, 64);
+            }
+            if (in_mask(ll)) {
+                const int slot = (head + queued + rank_in(ll)) & (LEAF_RING - 1);
+                L->leaf_first[slot] = ~f3.x;
+                L->leaf_ray_count[slot] = ray | (f3.z << 6);
+            }
+            if (in_mask(rl)) {
+                const int slot = (head + queued + nl + rank_in(rl)) & (LEAF_RING - 1);
+                L->leaf_first[slot] = ~f3.y;
+                L->leaf_ray_count[slot] = ray | (f3.w << 6);
+            }
+            queued += nl + nr;
+        }
+        const uint64_t both = li & ri;
+        const bool left_near = dl <= dr;
+        const int near = in_mask(both) ? (left_near ? f3.x : f3.y) : in_mask(li) ? f3.x : in_mask(ri) ? f3.y : -1;
+        if (both != 0) {
+            const int far = left_near ? f3.y : f3.x;
+            // the lowest-ranked lanes fill the shared stack's room; the others keep their far child
+            const uint64_t given = both & ballot(rank_in(both) < SHARE_CAP - top);
+            if (in_mask(given)) L->share[top + rank_in(given)] = ray | (far << 6);
+            top += popc64(given);
+            if (in_mask(both & ~given)) {
+                if (own_n < WALK_STACK) own[own_n++] = ray | (far << 6);
+                else *S.overflow = 1;
+            }
+        }
+        if (in_mask(busy)) {
+            cur = near;
+            if (near < 0) ray = -1;
+        }
+        if (queued >= NGX_LEAF_BATCH) {
+            test_queued_leaves<false>(S, L, head, queued, g, lo, ld, tmax, stats);
+            wlim = __shfl(tmax, ray >= 0 ? ray : lane, 64);
+        }
+    }
+    test_queued_leaves<true>(S, L, head, queued, g, lo, ld, tmax, stats);
+}
+
 // One instance for the lanes in mask: the ray moved into mesh space (mesh_dev.h:164-171, direction not
 // renormalized so t is shared), then the mesh's own tree.
 // BLAS: how bottom levels are walked: 0 as a packet; 1 lane by lane for meshes of lane_min_tris triangles or more,
-// as a packet otherwise; 2 lane by lane for every mesh (the packet code is then not built, which frees registers).
+// as a packet otherwise; 2 lane by lane for every mesh (the packet code is then not built, which frees registers; a
+// mesh of a single leaf is still tested as one batch).
 template <int BLAS, class Stats>
 __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, const uint64_t mask, const float3 o,
                                const float3 d, float &tmax, LaneHit &h, Stats &stats) {
@@ -395,12 +505,21 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
     stats.add(ST_INSTANCE_ROOT_LANES, popc64(root_mask));
     int shared = 0;  // subtrees given to the shared per-lane walk
     constexpr bool LANES = BLAS != 0;
-    if (BLAS == 2 || (BLAS == 1 && meta.z >= S.lane_min_tris)) {
-        share_subtree(L, shared, root_mask, meta.w);
-        walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
-        return;
-    }
     const int root_ref = __float_as_int(m0.w);
+    if (BLAS == 2 || (BLAS == 1 && meta.z >= S.lane_min_tris)) {
+        if constexpr (Stats::kTwoChild) {
+            // the two-child layout starts below the root's box; a mesh of one leaf is tested as a batch below
+            if (root_ref >= 0) {
+                share_subtree(L, shared, root_mask, root_ref);
+                walk_shared_two_child(S, L, g, shared, lo, ld, inv_d, tmax, stats);
+                return;
+            }
+        } else {
+            share_subtree(L, shared, root_mask, meta.w);
+            walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
+            return;
+        }
+    }
     int pairs = 0;
     if (root_ref < 0) {
         add_leaf(S, L, pairs, ~root_ref, __float_as_int(m1.w), root_mask, g, lo, ld, tmax, h, stats);
@@ -448,7 +567,7 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
             if (next_mask != 0 && next_ref >= 0) {
                 const int n = popc64(next_mask);
                 if (!LANES || !Stats::kHandoff || n > NGX_HANDOFF_LANES || shared + n > SHARE_CAP) break;
-                share_subtree(L, shared, next_mask, S.blas_wide_node[next_ref]);
+                share_subtree(L, shared, next_mask, Stats::kTwoChild ? next_ref : S.blas_wide_node[next_ref]);
             } else if (next_mask != 0) {
                 add_leaf(S, L, pairs, ~next_ref, next_count, next_mask, g, lo, ld, tmax, h, stats);
             }
@@ -472,7 +591,10 @@ __device__ void visit_instance(const TraceScene &S, PairLds *L, const int g, con
         cm = next_mask;
     }
     flush_pairs(S, L, pairs, g, lo, ld, tmax, stats);
-    if (LANES && shared > 0) walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
+    if (LANES && shared > 0) {
+        if constexpr (Stats::kTwoChild) walk_shared_two_child(S, L, g, shared, lo, ld, inv_d, tmax, stats);
+        else walk_shared(S, L, g, shared, lo, ld, inv_d, tmax, stats);
+    }
 }
 
 // The closest instance hit in one world for the lanes in live (mesh_dev.h:177-205). tmax is each lane's
