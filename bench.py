@@ -16,7 +16,8 @@ and the reference CPU algorithm (this repository's C restatement of the referenc
 poses, per-world SAH top level with refit, the same traversal, rows of views in chunks of 4 over threads)
 timed on this host on a bounded sample.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph] [--no-tune] [--no-lidar] [--no-rgb] [--no-bwd]"""
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-graph] [--no-tune] [--no-lidar] [--no-rgb] [--no-bwd]
+                [--only LEG] [--trace-config JSON]"""
 import argparse
 import json
 import os
@@ -84,6 +85,17 @@ def backward_bytes(views, width, height, instances, worlds, bodies, parts):
     return 8 * views * width * height + 48 * views + 128 * instances + 28 * worlds * bodies + 28 * views * parts * bodies
 
 
+def fixed_config(r, args):
+    """Applies --trace-config (a JSON object of BatchRenderer.set_trace_config's arguments, such as the trace_config
+    a bench line reports) in place of autotuning; True when it did."""
+    if not args.trace_config:
+        return False
+    c = {k: v for k, v in json.loads(args.trace_config).items() if k != "ms"}
+    r.set_trace_config(**c)
+    print(f"# trace configuration given: {r.trace_config}", file=sys.stderr, flush=True)
+    return True
+
+
 def bench_body_worlds(args, torch):
     from neraulgraphics_amd import workloads
     cfg = workloads.make("body_worlds")
@@ -92,6 +104,8 @@ def bench_body_worlds(args, torch):
     tuned = None
     if args.tile_w:
         r.set_trace_config(args.tile_w, not args.no_xcd)
+    elif fixed_config(r, args):
+        pass
     elif not args.no_tune:
         # the autotuner times its candidates on these cameras (outside the timed region) and keeps the fastest
         tuned = r.autotune(cams, cfg["fov"], depth=True, object_ids=True, reps=2, cache=False)
@@ -176,7 +190,7 @@ def bench_instanced_worlds(args, torch):
     cfg = workloads.make("instanced_worlds")
     r = workloads.make_renderer(cfg)
     cams = torch.from_numpy(cfg["cameras"]).cuda()
-    if not args.no_tune:
+    if not fixed_config(r, args) and not args.no_tune:
         r.autotune(cams, cfg["fov"], depth=True, object_ids=True, reps=2, cache=False)
     rng = np.random.default_rng(8)
     sets = []
@@ -225,7 +239,7 @@ def bench_lidar(args, torch):
     r = workloads.make_renderer(cfg)
     o = torch.from_numpy(cfg["origins"]).cuda()
     d = torch.from_numpy(cfg["directions"]).cuda()
-    if not args.no_tune:
+    if not fixed_config(r, args) and not args.no_tune:
         tuned = r.autotune_rays(o, d, cfg["max_distance"], reps=2)
         print(f"# LiDAR autotuned: {tuned}", file=sys.stderr, flush=True)
     for _ in range(2):
@@ -252,7 +266,7 @@ def bench_rgb(args, torch):
     r.set_ambient(0.1)
     cams = torch.from_numpy(cfg["cameras"]).cuda()
     tuned = None
-    if not args.no_tune:
+    if not fixed_config(r, args) and not args.no_tune:
         tuned = r.autotune(cams, cfg["fov"], depth=True, object_ids=True, rgb=True, reps=1, cache=False)
         print(f"# RGB autotuned: {tuned}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(7)
@@ -347,7 +361,7 @@ def bench_posed_bodies(args, torch):
         (0.5 * (diff * diff)).backward(ones)
         return p.grad
 
-    if not args.no_tune:
+    if not fixed_config(r, args) and not args.no_tune:
         # the forward's walk and tiles on these cameras, then the backward kernel's variant on a real gradient
         r.autotune(cams, cfg["fov"], depth=True, object_ids=False, reps=2, cache=False)
         d = render_depth(r, poses, cams, cfg["fov"]).detach()
@@ -499,6 +513,8 @@ def main():
     ap.add_argument("--tile-w", type=int, default=0)
     ap.add_argument("--no-xcd", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default trace configuration instead of autotuning")
+    ap.add_argument("--trace-config", default="", help="a JSON object of set_trace_config's arguments (e.g. a line's "
+                                                        "trace_config) used in place of autotuning, for profiling it")
     ap.add_argument("--no-lidar", action="store_true")
     ap.add_argument("--no-instanced", action="store_true", help="skip the 64x64 instanced_worlds leg")
     ap.add_argument("--no-rgb", action="store_true")
