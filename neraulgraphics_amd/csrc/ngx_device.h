@@ -187,6 +187,20 @@ __device__ __forceinline__ float in_vgpr(const float v) {
     return r;
 }
 __device__ __forceinline__ float3 in_vgpr3(const float3 v) { return make_float3(in_vgpr(v.x), in_vgpr(v.y), in_vgpr(v.z)); }
+// A vector register holding some value the compiler neither computes nor knows, for the operands of lanes whose results
+// are masked out (idle lanes of a per-lane walk): no instruction is spent on it. An uninitialized variable would not do:
+// its value is undefined to the compiler, which may then fold it through the masks (an undefined value ANDed with a mask
+// may be taken as 0), while an empty asm statement's output is a definite value it cannot see.
+__device__ __forceinline__ float unknown_f() {
+    float r;
+    asm volatile("" : "=v"(r));
+    return r;
+}
+__device__ __forceinline__ int unknown_i() {
+    int r;
+    asm volatile("" : "=v"(r));
+    return r;
+}
 
 // Loads through the scalar data cache: the address must be the same in every lane of the wave. The
 // constant address space is how the compiler is told the data is read-only for the whole kernel.

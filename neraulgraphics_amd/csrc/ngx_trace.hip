@@ -331,14 +331,16 @@ __device__ void walk_shared(const TraceScene &S, PairLds *L, const int g, int to
         stats.add(ST_BLAS_NODES, 1);
         stats.add(ST_BLAS_LANES, popc64(busy));
         // One step in every walking lane; the decisions are taken as wave masks, and only the node loads are
-        // confined to the walking lanes (idle lanes see an empty box and their results are masked out).
-        float4 f0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), f1 = f0;
-        int esc = -1;
+        // confined to the walking lanes (idle lanes test whatever their registers hold, and their results are masked out).
+        float4 f0 = make_float4(unknown_f(), unknown_f(), unknown_f(), unknown_f());
+        float4 f1 = make_float4(unknown_f(), unknown_f(), unknown_f(), unknown_f());
+        int esc = unknown_i();
         if (in_mask(busy)) {
-            const float4 *np = reinterpret_cast<const float4 *>(S.skip + cur);
-            f0 = np[0];
-            f1 = np[1];
-            esc = S.skip[cur].escape[oct];
+            // a 32-bit byte offset from the array's base (node indices stay below 2^25)
+            const char *np = reinterpret_cast<const char *>(S.skip) + (uint32_t)cur * (uint32_t)sizeof(SkipNode);
+            f0 = reinterpret_cast<const float4 *>(np)[0];
+            f1 = reinterpret_cast<const float4 *>(np)[1];
+            esc = reinterpret_cast<const SkipNode *>(np)->escape[oct];
         }
         const int a = __float_as_int(f0.w), meta = __float_as_int(f1.w);
         float tn;
@@ -419,11 +421,15 @@ __device__ void walk_shared_two_child(const TraceScene &S, PairLds *L, const int
         if (busy == 0) break;
         stats.add(ST_BLAS_NODES, 1);
         stats.add(ST_BLAS_LANES, popc64(busy));
-        // idle lanes see two empty boxes, and their results are masked out
-        float4 f0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), f1 = f0, f2 = f0;
-        int4 f3 = make_int4(0, 0, 0, 0);
+        // idle lanes test whatever their registers hold, and their results are masked out
+        float4 f0 = make_float4(unknown_f(), unknown_f(), unknown_f(), unknown_f());
+        float4 f1 = make_float4(unknown_f(), unknown_f(), unknown_f(), unknown_f());
+        float4 f2 = make_float4(unknown_f(), unknown_f(), unknown_f(), unknown_f());
+        int4 f3 = make_int4(unknown_i(), unknown_i(), unknown_i(), unknown_i());
         if (in_mask(busy)) {
-            const float4 *np = reinterpret_cast<const float4 *>(S.blas_wide + cur);
+            // a 32-bit byte offset from the array's base (node indices stay below 2^25)
+            const float4 *np = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(S.blas_wide) +
+                                                                (uint32_t)cur * (uint32_t)sizeof(WideNode));
             f0 = np[0];
             f1 = np[1];
             f2 = np[2];
