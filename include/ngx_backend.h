@@ -316,6 +316,11 @@ ngx_status ngx_build_skip_host(const float *vertices, int32_t vertex_count, cons
 ngx_status ngx_debug_check_reciprocal(int32_t device_id, uint64_t *mismatches_in_range, uint64_t *mismatches_anywhere,
                                       uint32_t *example);
 
+/* No counterpart in the reference: the inclusive prefix sum over a wavefront that the batched walks use to place (ray,
+ * triangle) pairs, run on device device_id over host arrays of 64 x waves values (each run of 64 scanned by one
+ * wavefront), for checking its lane layout. At most 2^20 waves. */
+ngx_status ngx_debug_wave_prefix_sum(int32_t device_id, const int32_t *values, int32_t waves, int32_t *sums);
+
 /* The host-side bottom-level build of one mesh (src/mesh_build.cpp:91-157 with src/bvh.cpp:46-174), no device
  * needed: nodes (capacity 2 x triangles) x 32 bytes in the reference's Node layout, triangles
  * (triangles x 12 floats, v0 e1 e2 padded), order (triangles). Returns the node count in *node_count. */

@@ -187,6 +187,18 @@ __device__ __forceinline__ float in_vgpr(const float v) {
     return r;
 }
 __device__ __forceinline__ float3 in_vgpr3(const float3 v) { return make_float3(in_vgpr(v.x), in_vgpr(v.y), in_vgpr(v.z)); }
+// Inclusive prefix sum over the wavefront (every lane active), one DPP add a stage: within each row of 16 lanes by
+// shifts of 1, 2, 4 and 8 (a lane shifted in from outside the row reads 0), then lane 15's sum added to rows 1 and 3 and
+// lane 31's to rows 2 and 3.
+__device__ __forceinline__ int wave_prefix_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+    return v;
+}
 // A vector register holding some value the compiler neither computes nor knows, for the operands of lanes whose results
 // are masked out (idle lanes of a per-lane walk): no instruction is spent on it. An uninitialized variable would not do:
 // its value is undefined to the compiler, which may then fold it through the masks (an undefined value ANDed with a mask

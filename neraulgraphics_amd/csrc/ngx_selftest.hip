@@ -48,6 +48,14 @@ __global__ __launch_bounds__(256) void reciprocal_check_kernel(uint32_t *out) {
     }
 }
 
+// The walks' wave-wide inclusive prefix sum on given values, 64 a wave, four waves a workgroup.
+__global__ __launch_bounds__(256) void wave_scan_kernel(const int *in, int *out, const int waves) {
+    const int wave = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (wave >= waves) return;  // whole waves leave together, so a scanning wave has every lane active
+    const size_t i = (size_t)wave * 64 + (threadIdx.x & 63);
+    out[i] = wave_prefix_sum(in[i]);
+}
+
 }  // namespace
 }  // namespace ngx
 
@@ -75,4 +83,20 @@ extern "C" ngx_status ngx_debug_check_reciprocal(int32_t device_id, uint64_t *mi
     *mismatches_anywhere = b;
     *example = ex;
     return NGX_OK;
+}
+
+extern "C" ngx_status ngx_debug_wave_prefix_sum(int32_t device_id, const int32_t *values, int32_t waves, int32_t *sums) {
+    if (waves < 0 || (waves > 0 && (!values || !sums)) || waves > (1 << 20)) return NGX_INVALID_ARGUMENT;
+    if (waves == 0) return NGX_OK;
+    if (hipSetDevice(device_id) != hipSuccess) return NGX_DEVICE_ERROR;
+    const size_t n = (size_t)waves * 64;
+    int *d = nullptr;
+    if (hipMalloc(&d, 2 * n * sizeof(int)) != hipSuccess) return NGX_OUT_OF_MEMORY;
+    bool ok = hipMemcpy(d, values, n * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        ngx::wave_scan_kernel<<<(unsigned)((waves + 3) / 4), 256>>>(d, d + n, waves);
+        ok = hipGetLastError() == hipSuccess && hipMemcpy(sums, d + n, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    hipFree(d);
+    return ok ? NGX_OK : NGX_DEVICE_ERROR;
 }

@@ -209,15 +209,6 @@ __device__ __forceinline__ void add_leaf(const TraceScene &S, PairLds *L, int &p
     }
 }
 
-// Inclusive prefix sum over the wavefront.
-__device__ __forceinline__ int wave_prefix_sum(int v) {
-    const int lane = lane_id();
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(v, o, 64);
-        if (lane >= o) v += y;
-    }
-    return v;
-}
 
 // The bottom level of one instance walked lane by lane, for meshes whose triangles are small next to a
 // wavefront's tile: there each ray of the packet needs its own leaves, so a shared walk visits the union of
