@@ -163,8 +163,8 @@ struct ngx_ctx {
     long long ray_in_cap = 0;
     float *d_hit_dist = nullptr, *d_hit_normal = nullptr;
     int32_t *d_hit_id = nullptr;
-    uint32_t *d_ray_keys = nullptr;     // the direction order's scratch: 2 x rays
-    int *d_ray_vals = nullptr;
+    uint32_t *d_ray_keys = nullptr;     // the direction order's scratch: 2 x rays (cell and index in one word)
+    int *d_ray_vals = nullptr;          // the direction order: rays
     long long ray_cap = 0;
     float *d_partial = nullptr;
     size_t partial_cap = 0;
@@ -959,7 +959,7 @@ ngx_status ngx_cast_rays_batch(ngx_ctx *ctx, const float *origins, const float *
             device_alloc(ctx->d_hit_normal, 3 * (size_t)count);
             device_alloc(ctx->d_hit_id, (size_t)count);
             device_alloc(ctx->d_ray_keys, 2 * (size_t)count);
-            device_alloc(ctx->d_ray_vals, 2 * (size_t)count);
+            device_alloc(ctx->d_ray_vals, (size_t)count);
             ctx->ray_cap = count;
         }
         ngx::RayArgs a{};

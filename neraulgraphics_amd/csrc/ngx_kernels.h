@@ -226,8 +226,8 @@ struct BigBuildScratch {
     int *tile_counts;          // [16 * ceil(capacity / 1024)]
 };
 
-// Per world, a stable order of its rays by direction cell, for coherent wavefronts: keys and vals are scratch
-// of 2 x rays_per_world x worlds each; *order points into vals.
+// Per world, a stable order of its rays by direction cell, for coherent wavefronts: keys is scratch of
+// 2 x rays_per_world x worlds words, vals receives the order (rays_per_world x worlds), and *order is vals.
 constexpr int RAY_ORDER_MAX_RAYS = 262144;
 hipError_t launch_ray_order(const float *dirs, int rays_per_world, int worlds, uint32_t *keys, int *vals, int **order,
                             hipStream_t stream);

@@ -656,8 +656,11 @@ __global__ __launch_bounds__(256) void big_write_kernel(const TlasBuildArgs A, c
 // world: a stable LSD radix sort over chunks of 1024 rays, in two passes of 6-bit digits for up to 65,536 rays a
 // world (the counts of every chunk and digit fit the workgroup's LDS) and three of 4-bit digits above that; both
 // give the same stable order. The tracer then gives a wavefront 64 consecutive rays of that order; results still
-// go to each ray's own place.
+// go to each ray's own place. A ray's cell and its index travel as one 32-bit word (cell above bit 20, index below), so
+// a pass moves 4 bytes a ray; the last pass writes the index alone.
 constexpr int RAY_ORDER_MAX_CHUNKS = RAY_ORDER_MAX_RAYS / 1024;
+constexpr int RAY_ORDER_INDEX_BITS = 20;
+static_assert(RAY_ORDER_MAX_RAYS <= (1 << RAY_ORDER_INDEX_BITS), "a world's ray index fits below the cell");
 constexpr int RAY_ORDER_WIDE_MAX_RAYS = 65536;
 
 // tile_digit for a BITS-bit digit: the digit of each key of a 1024-key tile, its rank among the equal digits of
@@ -700,7 +703,7 @@ __device__ __forceinline__ uint32_t direction_cell(const float x, const float y,
 }
 
 template <int BITS, int MAX_CHUNKS>
-__global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, const int R, uint32_t *ka, uint32_t *kb, int *va, int *vb) {
+__global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, const int R, uint32_t *ka, uint32_t *kb, int *order) {
     constexpr int NW = 16, NB = 1 << BITS, PASSES = 12 / BITS;
     static_assert(PASSES * BITS == 12 && NB * NW <= 1024, "12-bit keys in whole digits, one histogram entry a thread");
     __shared__ int counts[NB * MAX_CHUNKS];
@@ -712,14 +715,13 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
     const int chunks = (R + 1023) / 1024;
     for (int i = tid; i < R; i += 1024) {
         const float *p = dirs + 3 * (base + i);
-        ka[base + i] = direction_cell(p[0], p[1], p[2]);
-        va[base + i] = i;
+        ka[base + i] = (direction_cell(p[0], p[1], p[2]) << RAY_ORDER_INDEX_BITS) | (uint32_t)i;
     }
     __syncthreads();
     uint32_t *ks = ka, *kd = kb;
-    int *vs = va, *vd = vb;
     for (int pass = 0; pass < PASSES; ++pass) {
-        const int shift = BITS * pass;
+        const int shift = RAY_ORDER_INDEX_BITS + BITS * pass;
+        const bool last = pass == PASSES - 1;
         for (int c = 0; c < chunks; ++c) {
             const int i = c * 1024 + tid;
             const bool has = i < R;
@@ -762,7 +764,6 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
             const int i = c * 1024 + tid;
             const bool has = i < R;
             const uint32_t k = has ? ks[base + i] : 0u;
-            const int v = has ? vs[base + i] : 0;
             if (tid < NB * NW) hist[tid] = 0;
             __syncthreads();
             uint32_t digit;
@@ -773,13 +774,12 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
             block_exclusive_scan(hist, NB * NW, wave_tot);
             if (has) {
                 const int dest = counts[digit * chunks + c] + hist[digit * NW + wave] - hist[digit * NW] + rank;
-                kd[base + dest] = k;
-                vd[base + dest] = v;
+                if (last) order[base + dest] = (int)(k & ((1u << RAY_ORDER_INDEX_BITS) - 1u));
+                else kd[base + dest] = k;
             }
             __syncthreads();
         }
         uint32_t *tk = ks; ks = kd; kd = tk;
-        int *tv = vs; vs = vd; vd = tv;
     }
 }
 
@@ -820,13 +820,11 @@ hipError_t launch_ray_order(const float *dirs, const int rays_per_world, const i
     if (rays_per_world <= 0 || rays_per_world > RAY_ORDER_MAX_RAYS || worlds <= 0) return hipErrorInvalidValue;
     const size_t n = (size_t)rays_per_world * worlds;
     if (rays_per_world <= RAY_ORDER_WIDE_MAX_RAYS) {
-        ray_order_kernel<6, RAY_ORDER_WIDE_MAX_RAYS / 1024><<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals,
-                                                                                       vals + n);
-        *order = vals;  // two passes end in the first buffers
+        ray_order_kernel<6, RAY_ORDER_WIDE_MAX_RAYS / 1024><<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals);
     } else {
-        ray_order_kernel<4, RAY_ORDER_MAX_CHUNKS><<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals, vals + n);
-        *order = vals + n;  // three passes end in the second buffers
+        ray_order_kernel<4, RAY_ORDER_MAX_CHUNKS><<<worlds, 1024, 0, stream>>>(dirs, rays_per_world, keys, keys + n, vals);
     }
+    *order = vals;
     return hipGetLastError();
 }
 
