@@ -321,6 +321,11 @@ ngx_status ngx_debug_check_reciprocal(int32_t device_id, uint64_t *mismatches_in
  * wavefront), for checking its lane layout. At most 2^20 waves. */
 ngx_status ngx_debug_wave_prefix_sum(int32_t device_id, const int32_t *values, int32_t waves, int32_t *sums);
 
+/* No counterpart in the reference: how many entries (1 to 128, the default) of the LDS stack on which the shared
+ * per-lane bottom-level walk gives far subtrees to idle lanes this context's later calls use; the lanes keep what does
+ * not fit on their own stacks. Results do not depend on it (a ray's ties aside); a test lowers it to run that path. */
+ngx_status ngx_debug_set_share_capacity(ngx_ctx *ctx, int32_t entries);
+
 /* The host-side bottom-level build of one mesh (src/mesh_build.cpp:91-157 with src/bvh.cpp:46-174), no device
  * needed: nodes (capacity 2 x triangles) x 32 bytes in the reference's Node layout, triangles
  * (triangles x 12 floats, v0 e1 e2 padded), order (triangles). Returns the node count in *node_count. */

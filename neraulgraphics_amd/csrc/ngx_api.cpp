@@ -181,6 +181,7 @@ struct ngx_ctx {
     bool timed = false;
     hipStream_t last_stream = nullptr;
     int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000, sort_rays = 1;
+    int share_cap = ngx::SHARE_STACK;   // the two-child shared walk's LDS stack entries (ngx_debug_set_share_capacity)
     // kernel variants by workgroups per compute unit (0: each walk's default)
     int render_blocks = 0, rgb_blocks = 0, ray_blocks = 0, backward_blocks = 0;
     int compute_units = 0;              // sizes the persistent launch
@@ -230,6 +231,7 @@ struct ngx_ctx {
         s.mesh_tris = d_mesh_tris;
         s.lane_min_tris = lane_min_tris;
         s.overflow = d_overflow;
+        s.share_cap = share_cap;
         return s;
     }
 
@@ -1474,6 +1476,15 @@ ngx_status ngx_build_blas_host(const float *vertices, int32_t vertex_count, cons
                                       c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.0f};
             std::memcpy(o, packed, sizeof(packed));
         }
+    });
+}
+
+ngx_status ngx_debug_set_share_capacity(ngx_ctx *ctx, int32_t entries) {
+    return guarded([&] {
+        require(ctx != nullptr, "context must not be null");
+        require(entries >= 1 && entries <= ngx::SHARE_STACK,
+                "entries must be 1 to " + std::to_string(ngx::SHARE_STACK) + ", got " + std::to_string(entries));
+        ctx->share_cap = entries;
     });
 }
 

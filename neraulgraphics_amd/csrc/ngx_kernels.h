@@ -84,11 +84,15 @@ struct TraceScene {
     int lane_min_tris;         // packet walk: meshes of at least this many triangles are walked lane by lane
     int *overflow;             // set to 1 by a walk whose stack or trail would pass its capacity (a tree deeper than
                                // the walks allow); the walk then drops what does not fit, so its images are wrong
+    int share_cap;             // entries of the LDS stack the two-child shared walk gives subtrees to, 1..128 (128;
+                               // lower only to test the lanes' own stacks, which take the rest)
 };
 
 // The walks' depth limits: a wave-uniform stack of WALK_STACK entries per level for the packet walk, a 64-bit trail per
 // level for the per-lane walk.
 constexpr int WALK_STACK = 64;
+// Entries of the LDS stack on which the shared per-lane walks give subtrees to idle lanes.
+constexpr int SHARE_STACK = 128;
 
 // The batched walks keep a triangle or node index and a lane number in one 32-bit word (index << 6 | lane), so
 // indices stay below 2^25; a tree over n triangles has fewer than 2n nodes, so at most 2^24 triangles over all meshes.

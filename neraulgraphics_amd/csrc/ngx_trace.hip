@@ -222,7 +222,7 @@ __device__ __forceinline__ void add_leaf(const TraceScene &S, PairLds *L, int &p
 // pairs wait, and at the end; after each batch every walking lane re-reads its ray's closest distance. Every
 // node a ray enters is walked by exactly one lane, so each ray's closest hit is its own. (The RGB kernel walks so;
 // the other kernels use the two-child walk further below, which shares its work the same way.)
-constexpr int SHARE_CAP = 128, LEAF_RING = 128;
+constexpr int SHARE_CAP = SHARE_STACK, LEAF_RING = 128;
 // Queued leaves that trigger a batch of triangle tests (at most 64 pairs a batch; fewer leaves wait for more).
 #ifndef NGX_LEAF_BATCH
 #define NGX_LEAF_BATCH 64
@@ -458,7 +458,7 @@ __device__ void walk_shared_two_child(const TraceScene &S, PairLds *L, const int
         if (both != 0) {
             const int far = left_near ? f3.y : f3.x;
             // the lowest-ranked lanes fill the shared stack's room; the others keep their far child
-            const uint64_t given = both & ballot(rank_in(both) < SHARE_CAP - top);
+            const uint64_t given = both & ballot(rank_in(both) < S.share_cap - top);
             if (in_mask(given)) L->share[top + rank_in(given)] = ray | (far << 6);
             top += popc64(given);
             if (in_mask(both & ~given)) {

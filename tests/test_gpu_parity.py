@@ -128,6 +128,36 @@ def test_depth_normal_id_match_the_oracle(gpu, oracle, name, worlds, walk):
     assert rep["hits"] > 0.05 * rep["pixels"]
 
 
+@pytest.mark.parametrize("name,worlds", [("body_worlds", 16), ("lidar_sweeps", 4)])
+def test_lanes_own_stacks_give_the_oracle_results(gpu, oracle, name, worlds):
+    """With the shared walk's LDS stack cut to one entry, nearly every far subtree a lane meets goes to the lane's own
+    stack, the path a full LDS stack takes: depth, normal and ID images, and LiDAR ray results, still match the oracle, and
+    no walk reports an overflow (a lane's own entries lie on one path down a tree). Capacities outside 1..128 are
+    refused."""
+    from neraulgraphics_amd import _lib
+    _say(f"own-stack parity: {name}, {worlds} worlds, an LDS share stack of 1 entry")
+    cfg = workloads.make(name, worlds=worlds, rays_per_world=8192)
+    r = workloads.make_renderer(cfg)
+    r.set_trace_config(**WALKS["hybrid"])
+    for bad in (0, 129):
+        with pytest.raises(ValueError, match="entries must be 1 to 128"):
+            _lib.check(r._lib.ngx_debug_set_share_capacity(r._h, bad))
+    _lib.check(r._lib.ngx_debug_set_share_capacity(r._h, 1))
+    if name == "lidar_sweeps":
+        out = r.cast_rays(cfg["origins"], cfg["directions"], cfg["max_distance"])
+        st = forward(cfg, oracle, outputs=())
+        ref = oracle.cast_rays(st["nodes"], st["roots"], st["inv"], st["blas_root"], st["meshes"]["nodes"], st["meshes"]["tris"],
+                               st["meshes"]["attrs"], st["world_off"], cfg["origins"], cfg["directions"], cfg["max_distance"])
+        rep = compare_rays(out, ref, name)
+    else:
+        out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
+        ref = forward(cfg, oracle)
+        rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, name)
+    r.synchronize()
+    _say(rep)
+    assert rep["hits"] > 0
+
+
 @pytest.mark.parametrize("tile_w,pair_limit,persistent", [(16, 16, True), (32, 1, False), (8, 64, True)])
 def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_limit, persistent):
     _say(f"render parity: body_worlds, 16 worlds, tile_w {tile_w}, pair_limit {pair_limit}, persistent {persistent}")
