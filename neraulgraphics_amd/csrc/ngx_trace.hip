@@ -1297,14 +1297,24 @@ __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const lon
                     light_dir = mul3(to_light, rcp_rn(light_distance));
                     strength = lp0.w * light_factor(lq, ldir, light_dir, light_distance);
                 }
+                // The light's terms come first: where both are zero (a surface facing away from the light, and no
+                // highlight) adding them changes nothing whether or not the light is blocked (the sums start at +0 and
+                // only grow), so that lane casts no shadow ray, and a wave with no such lane left skips the walk.
                 const bool test = hit && strength > 0.0f;
-                if (!ballot(test)) continue;
-                const bool blocked = occluded<WALK>(A.s, L, world, offset_origin(point, normal, light_dir), light_dir, ballot(test),
-                                                    light_distance);
-                if (test && !blocked) {
+                float3 cd = mk3(0.0f, 0.0f, 0.0f), cs = cd;
+                if (test) {
                     const float3 lit = mul3(xyz(lcol), strength);
-                    diffuse = add3(diffuse, mul3(lit, fmaxf(0.0f, dot3(light_dir, normal))));
-                    specular = add3(specular, mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z)));
+                    cd = mul3(lit, fmaxf(0.0f, dot3(light_dir, normal)));
+                    cs = mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z));
+                }
+                const bool need = test && (cd.x != 0.0f || cd.y != 0.0f || cd.z != 0.0f || cs.x != 0.0f || cs.y != 0.0f || cs.z != 0.0f);
+                const uint64_t shadow = ballot(need);
+                if (!shadow) continue;
+                const bool blocked = occluded<WALK>(A.s, L, world, offset_origin(point, normal, light_dir), light_dir, shadow,
+                                                    light_distance);
+                if (need && !blocked) {
+                    diffuse = add3(diffuse, cd);
+                    specular = add3(specular, cs);
                 }
             }
             if (hit) {
