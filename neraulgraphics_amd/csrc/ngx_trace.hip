@@ -932,6 +932,11 @@ __device__ __forceinline__ void surface(const TraceScene &S, const int kind, con
     }
 }
 
+// The object ID surface() gives, without the normal: what a frame of depth and IDs alone needs.
+__device__ __forceinline__ int object_of(const TraceScene &S, const int kind, const LaneHit &h) {
+    return kind == OBJECT_INSTANCE ? OBJECT_INSTANCE + S.inst[h.inst].tmpl : kind;
+}
+
 struct Cam {
     float3 pos, right, up, fwd;
 };
@@ -995,16 +1000,18 @@ __device__ __forceinline__ void render_tile(const RenderArgs &A, PairLds *L, con
     if (!valid) return;
     const size_t idx = ((size_t)view * A.height + y) * A.width + x;
     const bool found = kind != OBJECT_NONE;
-    float3 normal = mk3(0.0f, 0.0f, 0.0f);
-    int object = OBJECT_NONE;
-    if (found) surface(A.s, kind, cam.pos, d, best, h, normal, object);
     if (A.depth) A.depth[idx] = found ? best * dot3(d, cam.fwd) : __int_as_float(0x7f800000);
     if (A.normal) {
+        float3 normal = mk3(0.0f, 0.0f, 0.0f);
+        int object = OBJECT_NONE;
+        if (found) surface(A.s, kind, cam.pos, d, best, h, normal, object);
         A.normal[3 * idx] = normal.x;
         A.normal[3 * idx + 1] = normal.y;
         A.normal[3 * idx + 2] = normal.z;
+        if (A.object_id) A.object_id[idx] = object;
+    } else if (A.object_id) {
+        A.object_id[idx] = found ? object_of(A.s, kind, h) : OBJECT_NONE;
     }
-    if (A.object_id) A.object_id[idx] = object;
 }
 
 // BLOCKS: workgroups per compute unit the kernel is built for (their registers follow: 7 gives at most 72 VGPRs,

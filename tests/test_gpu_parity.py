@@ -126,6 +126,10 @@ def test_depth_normal_id_match_the_oracle(gpu, oracle, name, worlds, walk):
     rep = compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, name)
     _say(rep)
     assert rep["hits"] > 0.05 * rep["pixels"]
+    # without the normal plane the kernel takes the object ID alone: the same depth and IDs, bit for bit
+    lean = r.render(cfg["cameras"], cfg["fov"], depth=True, object_ids=True)
+    assert np.array_equal(np.asarray(lean["depth"]).view(np.uint32), np.asarray(out["depth"]).view(np.uint32))
+    assert np.array_equal(np.asarray(lean["object_ids"]), np.asarray(out["object_ids"]))
 
 
 @pytest.mark.parametrize("name,worlds", [("body_worlds", 16), ("lidar_sweeps", 4)])
