@@ -609,6 +609,9 @@ def main():
             "hit_fraction": c2["hit_frac"],
             "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": summary.get("hbm_bytes_per_launch"),
+                         # traffic was counted under the profile's trace configuration; False when this run's
+                         # autotuner chose another (spills, and so traffic, differ between builds)
+                         "traffic_same_config": summary.get("trace_config") == c2["trace_config"],
                          "algorithmic_bytes_per_launch": rb,
                          # the bound that holds: the fraction of cycles every SIMD issues a vector instruction
                          # (rocprof, same kernel)
@@ -625,9 +628,11 @@ def main():
         line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])
         bw = c4["backward_bytes"] / (c4["backward_kernel_ms"] * 1e-3) / 1e9
+        summary = _profile_summary("depth_backward_kernel_posed_bodies_traffic.json")
         line["fwd_bwd_roofline"] = {
             "bound": "hbm", "kernel": "depth_backward_kernel", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": bw / HBM_PEAK_GBS, "traffic": _profile_summary("depth_backward_kernel_posed_bodies_traffic.json").get("hbm_bytes_per_launch"),
+            "frac": bw / HBM_PEAK_GBS, "traffic": summary.get("hbm_bytes_per_launch"),
+            "traffic_same_config": summary.get("trace_config") == c4["trace_config"],
             "algorithmic_bytes_per_launch": c4["backward_bytes"], "kernel_ms": c4["backward_kernel_ms"]}
         line["fwd_bwd_trace_config"] = c4["trace_config"]
         line["fwd_bwd_hip_graph"] = c4["hip_graph"]
