@@ -263,7 +263,11 @@ const char *ngx_last_error(void);
  *  lane_min_triangles  packet walk: the bottom level of meshes with at least this many triangles is walked
  *              lane by lane (each ray its own path) instead of as a packet (default 1000);
  *  sort_rays   ray queries of 1024 to 262,144 rays a world: 1 (the default) traces each world's rays in an order
- *              of their directions, so a wavefront's rays are neighbours; results are in the caller's order. */
+ *              of their directions, so a wavefront's rays are neighbours; results are in the caller's order;
+ *  rgb_two_pass  1 (the default): the RGB picture at one sample a pixel, where no surface is reflective enough to
+ *              bounce a ray (every reflectivity below 1e-3), with the persistent launch and the packet walk, is drawn
+ *              in two passes: the camera rays' hits (16 bytes a pixel kept on the device, 24 with vertex normals),
+ *              then their shading and shadow rays; the same bytes as the one pass (0). */
 typedef struct {
     int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles, sort_rays;
     /* Kernel variants, by the workgroups of 256 threads per compute unit each is built for (the registers a lane
@@ -271,6 +275,7 @@ typedef struct {
      * the RGB kernel 4, 5, 6 or 7 (default 4 for the packet walk, 7 for the hybrid one), the ray-query kernel 5, 6
      * or 7 (default 6), the depth backward 4, 5 or 6 (default 5). The per-lane walk has one variant of each. */
     int32_t render_blocks, rgb_blocks, ray_blocks, backward_blocks;
+    int32_t rgb_two_pass;
 } ngx_trace_config;
 ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config);
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);

@@ -66,7 +66,7 @@ class TraceConfig(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("xcd_remap", ctypes.c_int32), ("packet", ctypes.c_int32),
                 ("pair_limit", ctypes.c_int32), ("persistent", ctypes.c_int32), ("lane_min_triangles", ctypes.c_int32),
                 ("sort_rays", ctypes.c_int32), ("render_blocks", ctypes.c_int32), ("rgb_blocks", ctypes.c_int32),
-                ("ray_blocks", ctypes.c_int32), ("backward_blocks", ctypes.c_int32)]
+                ("ray_blocks", ctypes.c_int32), ("backward_blocks", ctypes.c_int32), ("rgb_two_pass", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

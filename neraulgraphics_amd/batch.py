@@ -235,15 +235,16 @@ class BatchRenderer:
 
     @_serialized
     def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000,
-                         sort_rays=True, render_blocks=0, rgb_blocks=0, ray_blocks=0, backward_blocks=0):
+                         sort_rays=True, render_blocks=0, rgb_blocks=0, ray_blocks=0, backward_blocks=0, rgb_two_pass=True):
         """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
         default) or per-lane (False) walks, (ray, triangle) pairs batched before testing (1..64), the
         persistent fixed-grid launch, the triangle count from which a mesh's bottom level is walked lane by lane
-        inside the packet walk, whether ray queries trace each world's rays in an order of their directions, and
-        the kernel variants by workgroups per compute unit (0: the walk's default; see ngx_trace_config)."""
+        inside the packet walk, whether ray queries trace each world's rays in an order of their directions, the
+        kernel variants by workgroups per compute unit (0: the walk's default), and whether the RGB picture of a
+        scene without mirror bounces is drawn in two passes (see ngx_trace_config)."""
         c = _lib.TraceConfig(int(tile_w), int(bool(xcd_remap)), int(bool(packet)), int(pair_limit), int(bool(persistent)),
                              int(lane_min_triangles), int(bool(sort_rays)), int(render_blocks), int(rgb_blocks),
-                             int(ray_blocks), int(backward_blocks))
+                             int(ray_blocks), int(backward_blocks), int(bool(rgb_two_pass)))
         _lib.check(self._lib.ngx_set_trace_config(self._h, ctypes.byref(c)))
 
     @property

@@ -154,6 +154,9 @@ struct ngx_ctx {
     ngx::ShadeScene shade{};
     float4 *d_inst_mat = nullptr, *d_sphere_mat = nullptr, *d_lights = nullptr;
     uint8_t *d_rgb = nullptr;
+    bool shade_no_bounce = false;       // every surface's reflectivity below the path's cut-off: paths end at their first hit
+    int4 *d_rgb_hits = nullptr;         // the two-pass picture's hits, pixels
+    float2 *d_rgb_uv = nullptr;         // and their barycentrics (meshes with vertex normals)
 
     // outputs and staging, grown as needed
     float *d_depth = nullptr, *d_normal = nullptr;
@@ -184,6 +187,7 @@ struct ngx_ctx {
     int share_cap = ngx::SHARE_STACK;   // the two-child shared walk's LDS stack entries (ngx_debug_set_share_capacity)
     // kernel variants by workgroups per compute unit (0: each walk's default)
     int render_blocks = 0, rgb_blocks = 0, ray_blocks = 0, backward_blocks = 0;
+    int rgb_two_pass = 1;
     int compute_units = 0;              // sizes the persistent launch
     int *d_tile_counters = nullptr;
     int *d_overflow = nullptr;          // set by a walk whose stack would pass its capacity; read by ngx_synchronize
@@ -199,7 +203,7 @@ struct ngx_ctx {
         for (float *p : floats) if (p) (void)hipFree(p);
         void *others[] = {d_root_at_end, d_blas_wide, d_mesh_roots, d_tris, d_attrs, d_spheres, d_inst, d_nodes, d_wide,
                           d_id, d_hit_id, d_dbg_keys, d_dbg_sorted_keys, d_skip, d_ray_keys, d_ray_vals, d_inst_mat, d_sphere_mat,
-                          d_lights, d_rgb, big.key[0], big.key[1], big.val[0], big.val[1],
+                          d_lights, d_rgb, d_rgb_hits, d_rgb_uv, big.key[0], big.key[1], big.val[0], big.val[1],
                           big.box, big.node_box, big.left, big.right, big.parent, big.arrivals, big.bounds, big.tile_counts};
         for (void *p : others) if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -834,6 +838,12 @@ ngx_status ngx_set_shading(ngx_ctx *ctx, const ngx_shading *shading) {
         upload(ctx->d_inst_mat, im.data(), im.size());
         upload(ctx->d_sphere_mat, sm.data(), sm.size());
         upload(ctx->d_lights, lights.data(), lights.size());
+        // a path's throughput starts at 1 and is multiplied by each surface's reflectivity; below the cut-off the
+        // path ends, so with every reflectivity below it no path goes past its first hit
+        bool no_bounce = h.floor_material.reflectivity < ngx::MIN_THROUGHPUT;
+        for (int i = 0; i < ctx->per_world; ++i) no_bounce = no_bounce && h.instance_materials[i].reflectivity < ngx::MIN_THROUGHPUT;
+        for (int i = 0; i < ctx->sphere_count; ++i) no_bounce = no_bounce && h.sphere_materials[i].reflectivity < ngx::MIN_THROUGHPUT;
+        ctx->shade_no_bounce = no_bounce;
         ngx::ShadeScene &e = ctx->shade;
         e.instance_materials = ctx->d_inst_mat;
         e.sphere_materials = ctx->d_sphere_mat;
@@ -916,7 +926,15 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
             g.spp = spp;
             g.frame = frame;
             g.blocks = ctx->rgb_blocks;
-            HIP_CHECK(ngx::launch_rgb(g, s, ctx->persistent ? ctx->compute_units : 0));
+            if (ctx->rgb_two_pass && spp == 1 && ctx->shade_no_bounce && ctx->persistent && ctx->compute_units > 0 && a.walk != 0) {
+                if (!ctx->d_rgb_hits) device_alloc(ctx->d_rgb_hits, pixels);
+                if (ctx->d_attrs && !ctx->d_rgb_uv) device_alloc(ctx->d_rgb_uv, pixels);
+                g.hits = ctx->d_rgb_hits;
+                g.hit_uv = ctx->d_attrs ? ctx->d_rgb_uv : nullptr;
+                HIP_CHECK(ngx::launch_rgb_two_pass(g, ctx->render_blocks, s, ctx->compute_units));
+            } else {
+                HIP_CHECK(ngx::launch_rgb(g, s, ctx->persistent ? ctx->compute_units : 0));
+            }
         }
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
         ctx->timed = timed;
@@ -1182,7 +1200,7 @@ bool built_variant(const int blocks, const int (&list)[N]) {
 ngx_trace_config current_config(const ngx_ctx *ctx) {
     return ngx_trace_config{ctx->tile_w,      ctx->xcd_remap,  ctx->packet,     ctx->pair_limit,
                             ctx->persistent,  ctx->lane_min_tris, ctx->sort_rays, ctx->render_blocks,
-                            ctx->rgb_blocks,  ctx->ray_blocks, ctx->backward_blocks};
+                            ctx->rgb_blocks,  ctx->ray_blocks, ctx->backward_blocks, ctx->rgb_two_pass};
 }
 
 // Times `runs` calls of `call` on the stream after one untimed call; returns ms per call, or -1 when a call fails
@@ -1256,6 +1274,7 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config) {
         ctx->rgb_blocks = c.rgb_blocks;
         ctx->ray_blocks = c.ray_blocks;
         ctx->backward_blocks = c.backward_blocks;
+        ctx->rgb_two_pass = c.rgb_two_pass ? 1 : 0;
     });
 }
 
@@ -1287,9 +1306,11 @@ ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, fl
             for (const int lanes : {never, 1000, 300, 0})
                 for (const int limit : {32, 64})
                     for (const int pers : {0, 1}) cands.push_back(ngx_trace_config{tw, 1, 1, limit, pers, lanes, start.sort_rays, 0, 0,
-                                                                                   start.ray_blocks, start.backward_blocks});
+                                                                                   start.ray_blocks, start.backward_blocks,
+                                                                                   start.rgb_two_pass});
             if (tw <= 16)
-                cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never, start.sort_rays, 0, 0, start.ray_blocks, start.backward_blocks});
+                cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never, start.sort_rays, 0, 0, start.ray_blocks, start.backward_blocks,
+                                                 start.rgb_two_pass});
         }
         ngx_trace_config winner = start;
         float best_time = 0.0f;
@@ -1309,10 +1330,15 @@ ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, fl
             if (outputs & NGX_OUTPUT_RGB) {
                 const ngx_trace_config base2 = winner;
                 cands.clear();
-                for (const int b : ngx::RGB_BLOCKS) {
-                    ngx_trace_config c = base2;
-                    c.rgb_blocks = b;
-                    cands.push_back(c);
+                // the two-pass picture where it applies (one sample, no bounces), else the one-pass kernel alone
+                for (const int two : {1, 0}) {
+                    if (two && !ctx->shade_no_bounce) continue;
+                    for (const int b : ngx::RGB_BLOCKS) {
+                        ngx_trace_config c = base2;
+                        c.rgb_blocks = b;
+                        c.rgb_two_pass = two;
+                        cands.push_back(c);
+                    }
                 }
                 keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
             }

@@ -119,6 +119,9 @@ struct RenderArgs {
 // specular albedo, reflectivity, specular exponent, 0), lights as four (position, intensity; radius, falloff,
 // cos_outer, cos_inner; direction, 0; colour, 0).
 constexpr int MAX_LIGHTS = 8;
+// A path of the RGB picture ends where its throughput (the product of the reflectivities met) falls below this
+// (render_core.h:16, :238-239).
+constexpr float MIN_THROUGHPUT = 1e-3f;
 struct ShadeScene {
     const float4 *instance_materials;  // [template instances][2]
     const float4 *sphere_materials;    // [spheres][2]
@@ -137,6 +140,10 @@ struct RgbArgs {
     uint8_t *rgb;                      // [views][H][W][3]
     int spp, frame;
     int blocks;                        // the RGB kernel's variant (0: the walk's default)
+    // the two-pass picture's hits: [views][H][W] (t's bits, what was hit, instance, triangle), and their
+    // barycentrics when a mesh has vertex normals (else null)
+    int4 *hits;
+    float2 *hit_uv;
 };
 
 struct RayArgs {
@@ -259,6 +266,9 @@ hipError_t launch_mesh_refit(const MeshRefitArgs &a, hipStream_t stream);
 
 // The RGB picture; with a.r.tile_counters set and compute_units > 0, from a persistent grid as launch_render.
 hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, int compute_units = 0);
+// One sample a pixel, no surface reflective enough to bounce, persistent launch, packet or hybrid walk: the hit pass,
+// then the shading pass; the same bytes as launch_rgb.
+hipError_t launch_rgb_two_pass(const RgbArgs &a, int hit_blocks, hipStream_t stream, int compute_units);
 
 // Renders every view; with a.tile_counters set and compute_units > 0, a fixed grid of as many workgroups as the
 // device holds at once takes tiles from per-XCD counters until none are left (the counters are zeroed by this

@@ -52,6 +52,15 @@ struct Occlusion {
     static constexpr bool kAnyHit = true, kHandoff = false, kTwoChild = false;
     __device__ __forceinline__ void add(int, int) {}
 };
+// Shadow rays of the two-pass picture, whose second pass holds no primary walk: the depth kernel's walk, stopping
+// each ray at its first hit.
+struct OcclusionTwoChild {
+    static constexpr bool kAnyHit = true, kHandoff = true, kTwoChild = true;
+    __device__ __forceinline__ void add(int, int) {}
+};
+#ifndef NGX_SPLIT_SHADOW
+#define NGX_SPLIT_SHADOW OcclusionTwoChild
+#endif
 #ifdef NGX_TRACE_STATS
 struct WaveStats {
     static constexpr bool kAnyHit = false, kHandoff = true, kTwoChild = true;
@@ -1118,7 +1127,6 @@ __global__ __launch_bounds__(256) void render_stats_kernel(const RenderArgs A, i
 // primary rays are a packet; after a bounce the lanes' rays go their own ways and the packet walk still
 // serves them (each lane tests with its own ray).
 constexpr int MAX_BOUNCES = 4;
-constexpr float MIN_THROUGHPUT = 1e-3f;
 constexpr float SURFACE_EPS = 1e-3f;
 
 __device__ __forceinline__ uint32_t pcg_hash(const uint32_t x) {
@@ -1167,7 +1175,7 @@ __device__ __forceinline__ float light_factor(const float4 q, const float4 dir, 
 }
 
 // Whether anything lies between o and o + max_dist d, for the lanes in live (render_core.h:146-160).
-template <int WALK>
+template <int WALK, class Walk = Occlusion>
 __device__ __forceinline__ bool occluded(const TraceScene &S, PairLds *L, const int world, const float3 o, const float3 d,
                                          const uint64_t live, const float max_dist) {
     const bool mine = in_mask(live);
@@ -1197,7 +1205,7 @@ __device__ __forceinline__ bool occluded(const TraceScene &S, PairLds *L, const 
     if (rest != 0) {
         float best = max_dist;
         LaneHit h;
-        Occlusion stats;
+        Walk stats;
         if (WALK != WALK_LANES) {
             const int lane = lane_id();
             L->hit_inst[lane] = -1;
@@ -1230,6 +1238,122 @@ __device__ __forceinline__ float srgb_channel(const float v) {
     return x <= 0.0031308f ? 12.92f * x : 1.055f * exp2f(__log2f(x) * (1.0f / 2.4f)) - 0.055f;
 }
 
+// A wavefront's pixels: an 8x8 (or tile_w x 64 / tile_w) tile of one view.
+struct TilePixel {
+    int view, x, y, world;
+    bool valid;
+    size_t index;  // into [views][H][W]
+};
+__device__ __forceinline__ TilePixel tile_pixel(const RenderArgs &A, const long long tile) {
+    const int lane = lane_id();
+    TilePixel p;
+    p.view = (int)(tile / A.tiles_per_view);
+    const int tt = (int)(tile % A.tiles_per_view);
+    p.x = (tt % A.tiles_x) * A.tile_w + lane % A.tile_w;
+    p.y = (tt / A.tiles_x) * A.tile_h + lane / A.tile_w;
+    p.valid = p.x < A.width && p.y < A.height;
+    p.world = p.view / A.cameras_per_world;
+    p.index = ((size_t)p.view * A.height + p.y) * A.width + p.x;
+    return p;
+}
+// The pixel's random stream, seeded by the pixel and the frame (geom_dev.h:122-126).
+__device__ __forceinline__ uint32_t pixel_stream(const RenderArgs &A, const TilePixel &p, const int frame) {
+    return pcg_hash((uint32_t)(p.y * A.width + p.x) ^ pcg_hash((uint32_t)frame + 1u));
+}
+// The pixel's bytes from the sum of its samples (render_core.h:276-283).
+__device__ __forceinline__ void write_rgb(const RgbArgs &G, const size_t index, const float3 sum) {
+    const float3 c = mul3(sum, 1.0f / (float)G.spp);
+    uint8_t *out = G.rgb + 3 * index;
+    out[0] = (uint8_t)(saturate(srgb_channel(c.x)) * 255.0f + 0.5f);
+    out[1] = (uint8_t)(saturate(srgb_channel(c.y)) * 255.0f + 0.5f);
+    out[2] = (uint8_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
+}
+
+// What one traced segment of a path adds (render_core.h:206-240): the lanes in active traced o + t d, and kind
+// says what each hit (OBJECT_NONE: nothing, so the sky), best and h where. radiance and throughput accumulate as
+// in the reference; active is cleared where the path ends, and elsewhere o and d become the mirror bounce. Every
+// lane of the wavefront calls it (the shadow walk is a wavefront's); Shadow is the shadow rays' walk.
+template <int WALK, class Shadow>
+__device__ __forceinline__ void shade_hit(const RgbArgs &G, PairLds *L, const int world, const int kind, const float best,
+                                          const LaneHit &h, bool &active, float3 &o, float3 &d, uint32_t &rng,
+                                          float3 &radiance, float &throughput) {
+    const RenderArgs &A = G.r;
+    const ShadeScene &E = G.shade;
+    const bool hit = active && kind != OBJECT_NONE;
+    if (active && !hit) {
+        radiance = add3(radiance, mul3(sky(E, d), throughput));
+        active = false;
+    }
+    float3 normal = mk3(0.0f, 1.0f, 0.0f), point = o;
+    float4 m0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m1 = m0;
+    if (hit) {
+        point = add3(o, mul3(d, best));
+        int object;
+        surface(A.s, kind, o, d, best, h, normal, object);
+        if (kind == OBJECT_INSTANCE) {
+            const float4 *mp = E.instance_materials + 2 * (size_t)A.s.inst[h.inst].tmpl;
+            m0 = mp[0];
+            m1 = mp[1];
+        } else if (kind == OBJECT_FLOOR) {
+            m0 = E.floor_material[0];
+            m1 = E.floor_material[1];
+            const int cx = (int)floorf(point.x * E.floor_checker_scale), cz = (int)floorf(point.z * E.floor_checker_scale);
+            const float3 tint = ((cx + cz) & 1) ? mk3(1.0f, 1.0f, 1.0f) : mk3(1.0f, 0.7f, 0.3f);
+            m0.x = tint.x * 0.3f;
+            m0.y = tint.y * 0.3f;
+            m0.z = tint.z * 0.3f;
+        } else {
+            const float4 *mp = E.sphere_materials + 2 * (size_t)(kind - OBJECT_SPHERE);
+            m0 = mp[0];
+            m1 = mp[1];
+        }
+    }
+    float3 diffuse = mk3(0.0f, 0.0f, 0.0f), specular = mk3(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < E.light_count; ++i) {
+        const float4 lp0 = E.lights[4 * i], lq = E.lights[4 * i + 1], ldir = E.lights[4 * i + 2], lcol = E.lights[4 * i + 3];
+        float3 light_dir = mk3(0.0f, 1.0f, 0.0f);
+        float light_distance = 0.0f, strength = 0.0f;
+        if (hit) {
+            const float3 lp = sample_sphere_light(xyz(lp0), lq.x, rng);
+            const float3 to_light = sub3(lp, point);
+            light_distance = sqrtf(dot3(to_light, to_light));
+            light_dir = mul3(to_light, rcp_rn(light_distance));
+            strength = lp0.w * light_factor(lq, ldir, light_dir, light_distance);
+        }
+        // The light's terms come first: where both are zero (a surface facing away from the light, and no
+        // highlight) adding them changes nothing whether or not the light is blocked (the sums start at +0 and
+        // only grow), so that lane casts no shadow ray, and a wave with no such lane left skips the walk.
+        const bool test = hit && strength > 0.0f;
+        float3 cd = mk3(0.0f, 0.0f, 0.0f), cs = cd;
+        if (test) {
+            const float3 lit = mul3(xyz(lcol), strength);
+            cd = mul3(lit, fmaxf(0.0f, dot3(light_dir, normal)));
+            cs = mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z));
+        }
+        const bool need = test && (cd.x != 0.0f || cd.y != 0.0f || cd.z != 0.0f || cs.x != 0.0f || cs.y != 0.0f || cs.z != 0.0f);
+        const uint64_t shadow = ballot(need);
+        if (!shadow) continue;
+        const bool blocked = occluded<WALK, Shadow>(A.s, L, world, offset_origin(point, normal, light_dir), light_dir, shadow,
+                                                    light_distance);
+        if (need && !blocked) {
+            diffuse = add3(diffuse, cd);
+            specular = add3(specular, cs);
+        }
+    }
+    if (hit) {
+        const float3 dc = xyz(m0);
+        const float3 direct = add3(mul3v(dc, mul3(diffuse, m0.w)), mul3(specular, m1.x));
+        radiance = add3(radiance, mul3(add3(direct, mul3(dc, E.ambient * m0.w)), throughput));
+        throughput *= m1.y;
+        if (throughput < MIN_THROUGHPUT) {
+            active = false;
+        } else {
+            d = reflect3(d, normal);
+            o = offset_origin(point, normal, d);
+        }
+    }
+}
+
 // Workgroups per compute unit the RGB kernels are compiled for: 4 (16 wavefronts, at most 128 VGPRs) for the
 // packet and per-lane walks, 7 (at most 72 VGPRs, many more spills) for the hybrid walk, whose scenes have more
 // latency to hide (body_worlds RGB: 25.7 ms at 5, 24.3 at 6, 23.5 at 7, 33.0 at 8; the 8-body falling-bodies scene,
@@ -1239,113 +1363,28 @@ __device__ __forceinline__ float srgb_channel(const float v) {
 template <int WALK>
 __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const long long tile) {
     const RenderArgs &A = G.r;
-    const ShadeScene &E = G.shade;
-    const int lane = lane_id();
-    const int view = (int)(tile / A.tiles_per_view);
-    const int tt = (int)(tile % A.tiles_per_view);
-    const int x = (tt % A.tiles_x) * A.tile_w + lane % A.tile_w;
-    const int y = (tt / A.tiles_x) * A.tile_h + lane / A.tile_w;
-    const bool valid = x < A.width && y < A.height;
-    const int world = view / A.cameras_per_world;
-    const Cam cam = load_camera(A.cams + 12 * (size_t)view);
-    uint32_t rng = pcg_hash((uint32_t)(y * A.width + x) ^ pcg_hash((uint32_t)G.frame + 1u));  // geom_dev.h:122-126
+    const TilePixel p = tile_pixel(A, tile);
+    const Cam cam = load_camera(A.cams + 12 * (size_t)p.view);
+    uint32_t rng = pixel_stream(A, p, G.frame);
     float3 sum = mk3(0.0f, 0.0f, 0.0f);
     for (int sample = 0; sample < G.spp; ++sample) {
         const float jx = rand01(rng), jy = rand01(rng);
         float3 o = cam.pos;
-        float3 d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)x + jx, (float)y + jy, A.width, A.height);
+        float3 d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)p.x + jx, (float)p.y + jy, A.width, A.height);
         float3 radiance = mk3(0.0f, 0.0f, 0.0f);
         float throughput = 1.0f;
-        bool active = valid;
+        bool active = p.valid;
         for (int depth = 0; ballot(active); ++depth) {
             float best;
             LaneHit h;
             ShadeRays stats;
             int kind = OBJECT_NONE;
-            if (depth <= MAX_BOUNCES) kind = scene_hit<WALK>(A.s, L, world, o, d, ballot(active), best, h, stats);
-            const bool hit = active && kind != OBJECT_NONE;
-            if (active && !hit) {
-                radiance = add3(radiance, mul3(sky(E, d), throughput));
-                active = false;
-            }
-            float3 normal = mk3(0.0f, 1.0f, 0.0f), point = o;
-            float4 m0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m1 = m0;
-            if (hit) {
-                point = add3(o, mul3(d, best));
-                int object;
-                surface(A.s, kind, o, d, best, h, normal, object);
-                if (kind == OBJECT_INSTANCE) {
-                    const float4 *mp = E.instance_materials + 2 * (size_t)A.s.inst[h.inst].tmpl;
-                    m0 = mp[0];
-                    m1 = mp[1];
-                } else if (kind == OBJECT_FLOOR) {
-                    m0 = E.floor_material[0];
-                    m1 = E.floor_material[1];
-                    const int cx = (int)floorf(point.x * E.floor_checker_scale), cz = (int)floorf(point.z * E.floor_checker_scale);
-                    const float3 tint = ((cx + cz) & 1) ? mk3(1.0f, 1.0f, 1.0f) : mk3(1.0f, 0.7f, 0.3f);
-                    m0.x = tint.x * 0.3f;
-                    m0.y = tint.y * 0.3f;
-                    m0.z = tint.z * 0.3f;
-                } else {
-                    const float4 *mp = E.sphere_materials + 2 * (size_t)(kind - OBJECT_SPHERE);
-                    m0 = mp[0];
-                    m1 = mp[1];
-                }
-            }
-            float3 diffuse = mk3(0.0f, 0.0f, 0.0f), specular = mk3(0.0f, 0.0f, 0.0f);
-            for (int i = 0; i < E.light_count; ++i) {
-                const float4 lp0 = E.lights[4 * i], lq = E.lights[4 * i + 1], ldir = E.lights[4 * i + 2], lcol = E.lights[4 * i + 3];
-                float3 light_dir = mk3(0.0f, 1.0f, 0.0f);
-                float light_distance = 0.0f, strength = 0.0f;
-                if (hit) {
-                    const float3 lp = sample_sphere_light(xyz(lp0), lq.x, rng);
-                    const float3 to_light = sub3(lp, point);
-                    light_distance = sqrtf(dot3(to_light, to_light));
-                    light_dir = mul3(to_light, rcp_rn(light_distance));
-                    strength = lp0.w * light_factor(lq, ldir, light_dir, light_distance);
-                }
-                // The light's terms come first: where both are zero (a surface facing away from the light, and no
-                // highlight) adding them changes nothing whether or not the light is blocked (the sums start at +0 and
-                // only grow), so that lane casts no shadow ray, and a wave with no such lane left skips the walk.
-                const bool test = hit && strength > 0.0f;
-                float3 cd = mk3(0.0f, 0.0f, 0.0f), cs = cd;
-                if (test) {
-                    const float3 lit = mul3(xyz(lcol), strength);
-                    cd = mul3(lit, fmaxf(0.0f, dot3(light_dir, normal)));
-                    cs = mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z));
-                }
-                const bool need = test && (cd.x != 0.0f || cd.y != 0.0f || cd.z != 0.0f || cs.x != 0.0f || cs.y != 0.0f || cs.z != 0.0f);
-                const uint64_t shadow = ballot(need);
-                if (!shadow) continue;
-                const bool blocked = occluded<WALK>(A.s, L, world, offset_origin(point, normal, light_dir), light_dir, shadow,
-                                                    light_distance);
-                if (need && !blocked) {
-                    diffuse = add3(diffuse, cd);
-                    specular = add3(specular, cs);
-                }
-            }
-            if (hit) {
-                const float3 dc = xyz(m0);
-                const float3 direct = add3(mul3v(dc, mul3(diffuse, m0.w)), mul3(specular, m1.x));
-                radiance = add3(radiance, mul3(add3(direct, mul3(dc, E.ambient * m0.w)), throughput));
-                throughput *= m1.y;
-                if (throughput < MIN_THROUGHPUT) {
-                    active = false;
-                } else {
-                    d = reflect3(d, normal);
-                    o = offset_origin(point, normal, d);
-                }
-            }
+            if (depth <= MAX_BOUNCES) kind = scene_hit<WALK>(A.s, L, p.world, o, d, ballot(active), best, h, stats);
+            shade_hit<WALK, Occlusion>(G, L, p.world, kind, best, h, active, o, d, rng, radiance, throughput);
         }
         sum = add3(sum, radiance);
     }
-    if (!valid) return;
-    const float3 c = mul3(sum, 1.0f / (float)G.spp);
-    const size_t idx = ((size_t)view * A.height + y) * A.width + x;
-    uint8_t *out = G.rgb + 3 * idx;
-    out[0] = (uint8_t)(saturate(srgb_channel(c.x)) * 255.0f + 0.5f);
-    out[1] = (uint8_t)(saturate(srgb_channel(c.y)) * 255.0f + 0.5f);
-    out[2] = (uint8_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
+    if (p.valid) write_rgb(G, p.index, sum);
 }
 
 template <int WALK, int BLOCKS>
@@ -1362,6 +1401,75 @@ template <int WALK, int BLOCKS>
 __global__ __launch_bounds__(256, BLOCKS) void rgb_persistent_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
     take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_tile<WALK>(G, L, tile); });
+}
+
+// The picture in two passes, for one sample a pixel where no surface reflects (every reflectivity below
+// MIN_THROUGHPUT, so every path ends at its first segment). The first pass traces the jittered camera rays alone,
+// with the depth kernel's walk and register budget, and keeps each pixel's hit (16 bytes, and 8 more of
+// barycentrics when a mesh has vertex normals); the second draws the same ray and random numbers again and shades
+// the hit as the one-pass kernel does, shadow rays and all, so the bytes are the same.
+template <int WALK>
+__device__ __forceinline__ void rgb_hit_tile(const RgbArgs &G, PairLds *L, const long long tile) {
+    const RenderArgs &A = G.r;
+    const TilePixel p = tile_pixel(A, tile);
+    const Cam cam = load_camera(A.cams + 12 * (size_t)p.view);
+    uint32_t rng = pixel_stream(A, p, G.frame);
+    const float jx = rand01(rng), jy = rand01(rng);
+    const float3 d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)p.x + jx, (float)p.y + jy, A.width, A.height);
+    float best;
+    LaneHit h;
+    NoStats stats;
+    const int kind = scene_hit<WALK>(A.s, L, p.world, cam.pos, d, ballot(p.valid), best, h, stats);
+    if (!p.valid) return;
+    G.hits[p.index] = make_int4(__float_as_int(best), kind, h.inst, h.tri);
+    if (G.hit_uv != nullptr) G.hit_uv[p.index] = kind == OBJECT_INSTANCE ? make_float2(h.u, h.v) : make_float2(0.0f, 0.0f);
+}
+
+template <int WALK>
+__device__ __forceinline__ void rgb_shade_tile(const RgbArgs &G, PairLds *L, const long long tile) {
+    const RenderArgs &A = G.r;
+    const TilePixel p = tile_pixel(A, tile);
+    const Cam cam = load_camera(A.cams + 12 * (size_t)p.view);
+    uint32_t rng = pixel_stream(A, p, G.frame);
+    const float jx = rand01(rng), jy = rand01(rng);
+    float3 o = cam.pos;
+    float3 d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)p.x + jx, (float)p.y + jy, A.width, A.height);
+    int kind = OBJECT_NONE;
+    float best = 0.0f;
+    LaneHit h;
+    h.inst = -1;
+    h.tri = -1;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    if (p.valid) {
+        const int4 r = G.hits[p.index];
+        best = __int_as_float(r.x);
+        kind = r.y;
+        h.inst = r.z;
+        h.tri = r.w;
+        if (G.hit_uv != nullptr) {
+            const float2 uv = G.hit_uv[p.index];
+            h.u = uv.x;
+            h.v = uv.y;
+        }
+    }
+    float3 radiance = mk3(0.0f, 0.0f, 0.0f);
+    float throughput = 1.0f;
+    bool active = p.valid;
+    shade_hit<WALK, NGX_SPLIT_SHADOW>(G, L, p.world, kind, best, h, active, o, d, rng, radiance, throughput);
+    if (p.valid) write_rgb(G, p.index, add3(mk3(0.0f, 0.0f, 0.0f), radiance));
+}
+
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void rgb_hit_kernel(const RgbArgs G) {
+    NGX_PAIR_LDS(L);
+    take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_hit_tile<WALK>(G, L, tile); });
+}
+
+template <int WALK, int BLOCKS>
+__global__ __launch_bounds__(256, BLOCKS) void rgb_shade_kernel(const RgbArgs G) {
+    NGX_PAIR_LDS(L);
+    take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_shade_tile<WALK>(G, L, tile); });
 }
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave (with xcd_remap, each XCD takes a
@@ -1718,6 +1826,45 @@ hipError_t launch_rgb(const RgbArgs &a, hipStream_t stream, const int compute_un
     const long long blocks = (a.r.total_tiles + 3) / 4;
 #define NGX_LAUNCH(W, B) rgb_kernel<W, B><<<(unsigned)blocks, 256, 0, stream>>>(a)
     NGX_RGB_VARIANT(a.r.walk, a.blocks, NGX_LAUNCH)
+#undef NGX_LAUNCH
+    return hipGetLastError();
+}
+
+template <auto KERNEL>
+static int resident_grid(const int compute_units, const long long tiles) {
+    static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KERNEL, 256, 0) != hipSuccess || n < 1) n = 1;
+        return n;
+    }();
+    return capped_grid((per_cu * compute_units + 7) / 8 * 8, tiles);
+}
+
+// The two-pass picture, persistent launches of the packet and hybrid walks only: the hit pass built like the depth
+// kernel (5, 6 or 7 workgroups per compute unit: hit_blocks, 0 for the default), the shading pass like the RGB
+// kernel (4 to 7: a.blocks).
+hipError_t launch_rgb_two_pass(const RgbArgs &a, const int hit_blocks, hipStream_t stream, const int compute_units) {
+    if (a.r.total_tiles <= 0) return hipSuccess;
+    if (a.spp != 1 || a.shade.light_count < 0 || a.shade.light_count > MAX_LIGHTS || a.hits == nullptr ||
+        a.r.tile_counters == nullptr || compute_units <= 0 || a.r.walk == WALK_LANES || a.r.total_tiles >= INT32_MAX / 2)
+        return hipErrorInvalidValue;
+    const bool hybrid = a.r.walk == WALK_HYBRID;
+    zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
+#define NGX_LAUNCH(W, B) rgb_hit_kernel<W, B><<<resident_grid<rgb_hit_kernel<W, B>>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
+    switch (hit_blocks ? hit_blocks : RENDER_BLOCKS_DEFAULT) {
+    case 5: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 5); else NGX_LAUNCH(WALK_PACKET, 5); break;
+    case 6: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 6); else NGX_LAUNCH(WALK_PACKET, 6); break;
+    default: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 7); else NGX_LAUNCH(WALK_PACKET, 7); break;
+    }
+#undef NGX_LAUNCH
+    zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
+#define NGX_LAUNCH(W, B) rgb_shade_kernel<W, B><<<resident_grid<rgb_shade_kernel<W, B>>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
+    switch (a.blocks ? a.blocks : hybrid ? RGB_BLOCKS_DEFAULT_HYBRID : RGB_BLOCKS_DEFAULT_PACKET) {
+    case 4: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 4); else NGX_LAUNCH(WALK_PACKET, 4); break;
+    case 5: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 5); else NGX_LAUNCH(WALK_PACKET, 5); break;
+    case 6: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 6); else NGX_LAUNCH(WALK_PACKET, 6); break;
+    default: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 7); else NGX_LAUNCH(WALK_PACKET, 7); break;
+    }
 #undef NGX_LAUNCH
     return hipGetLastError();
 }

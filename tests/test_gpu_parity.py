@@ -170,7 +170,7 @@ def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_l
     r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
     assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent),
                                   lane_min_triangles=1000, sort_rays=1, render_blocks=0, rgb_blocks=0, ray_blocks=0,
-                                  backward_blocks=0)
+                                  backward_blocks=0, rgb_two_pass=1)
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
     _say(compare_images(dict(depth=out["depth"], object_id=out["object_ids"], normal=out["normals"]), ref, "body_worlds"))
@@ -897,6 +897,43 @@ def test_rgb_depth_id_of_the_full_body_worlds_batch_on_a_sample_of_worlds(gpu, o
     assert rep["hits"] > 0.3 * rep["pixels"]
     # a picture: grey bodies in light and shadow over a checker floor under the sky gradient, not a flat colour
     assert len(np.unique(ref["rgb"].reshape(-1, 3), axis=0)) > 200
+
+
+def test_two_pass_picture_gives_the_one_pass_bytes(gpu):
+    """The picture drawn in two passes (the camera rays' hits first, then their shading and shadow rays: one sample a
+    pixel, no surface reflective enough to bounce a ray) against the one-pass kernel, for the packet and hybrid walks:
+    the body_worlds scene with its floor and soft light (24 worlds), and 8 worlds of meshes with vertex normals, glossy
+    materials, two spheres, a soft light and a spotlight. The hit pass walks the trees in another order, so a ray
+    through an edge two triangles share may keep the other triangle (the parity bar's one exception): all but a
+    handful of pixels are the same bytes."""
+    import neraulgraphics_amd as ng
+    from .test_gpu_normals import _smooth_meshes
+    _say("two-pass picture against the one-pass kernel")
+    bw = workloads.make("body_worlds", worlds=24)
+    r1 = workloads.make_renderer(bw, floor=ng.Plane(y=-6.0, half_extent_x=40.0, z_min=-60.0, z_max=20.0),
+                                 lights=[ng.Light((0.0, 12.0, -6.0), intensity=1.2, radius=0.5)])
+    r1.set_ambient(0.1)
+    iw = workloads.make("instanced_worlds", worlds=8)
+    mats = [ng.Material((0.9, 0.2, 0.2), 0.9, 0.1, 0.0, 50.0), ng.Material((0.2, 0.8, 0.3), 0.7, 0.5, 0.0, 200.0)]
+    iw["meshes"] = _smooth_meshes()
+    iw["instances"] = [ng.Instance(k % 4, material=mats[k % 2]) for k in range(len(iw["instances"]))]
+    r2 = workloads.make_renderer(iw, spheres=[ng.Sphere((0.5, -1.0, -9.0), 1.0), ng.Sphere((-2.0, 1.0, -7.0), 0.7)],
+                                 floor=ng.Plane(y=-4.0, half_extent_x=30.0, z_min=-60.0, z_max=5.0, checker_scale=0.5),
+                                 lights=[ng.Light((2.0, 6.0, -4.0), intensity=1.2, radius=0.8),
+                                         ng.Light((-3.0, 4.0, -2.0), intensity=2.0, falloff=8.0, direction=(0.3, -1.0, -0.8),
+                                                  spot_angle=0.6, color=(1.0, 0.8, 0.6))])
+    r2.set_ambient(0.15)
+    for name, r, cfg in (("body_worlds", r1, bw), ("vertex normals and spheres", r2, iw)):
+        for walk in ("packet", "hybrid"):
+            out = {}
+            for two in (True, False):
+                r.set_trace_config(**WALKS[walk], rgb_two_pass=two)
+                out[two] = r.render(cfg["cameras"], cfg["fov"], 1, rgb=True, frame=5)["rgb"].astype(np.int32)
+            differ = np.any(out[True] != out[False], axis=-1)
+            mean = float(np.abs(out[True] - out[False]).mean())
+            _say(f"{name}, {walk} walk: {int(differ.sum())} of {differ.size} pixels differ; mean difference {mean:.6f}")
+            assert int(differ.sum()) <= max(2, differ.size // 20000) and mean < 0.05, (name, walk)
+            assert len(np.unique(out[True].reshape(-1, 3), axis=0)) > 200
 
 
 def test_rgb_known_answers_on_the_device(gpu, oracle):
