@@ -502,6 +502,28 @@ def _profile_summary(name):
         return json.load(f)
 
 
+# The trace-configuration fields a profiled kernel's traffic depends on, and its variant field with the variant a 0
+# stands for (the kernels' built-in defaults).
+_TRAFFIC_FIELDS = {
+    "render": (("tile_w", "xcd_remap", "packet", "pair_limit", "persistent", "lane_min_triangles"), "render_blocks", 7),
+    "backward": (("packet", "pair_limit", "lane_min_triangles"), "backward_blocks", 5),
+}
+
+
+def _traffic(summary, kind, chosen):
+    """HBM bytes per launch from the summary (or one of its `variants`) profiled under the configuration this run
+    chose, compared on the fields the kernel depends on; else the summary's own figure. Returns (bytes, matched)."""
+    fields, variant, default = _TRAFFIC_FIELDS[kind]
+
+    def key(c):
+        return tuple(int(c.get(f, 0)) for f in fields) + (int(c.get(variant, 0)) or default,)
+
+    for entry in [summary] + list(summary.get("variants", [])):
+        if entry.get("trace_config") is not None and key(entry["trace_config"]) == key(chosen):
+            return entry.get("hbm_bytes_per_launch"), True
+    return summary.get("hbm_bytes_per_launch"), False
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -595,6 +617,7 @@ def main():
         achieved = rb / (c2["render_ms"] * 1e-3) / 1e9
         kernel = "render_persistent_kernel" if c2["trace_config"].get("persistent") else "render_kernel"
         summary = _profile_summary(f"{kernel}_body_worlds_traffic.json")
+        traffic, same = _traffic(summary, "render", c2["trace_config"])
         line.update({
             "value": c2["views"] / sec,
             "ms_per_step": sec * 1e3,
@@ -608,10 +631,10 @@ def main():
             "render_ms": c2["render_ms"],
             "hit_fraction": c2["hit_frac"],
             "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": summary.get("hbm_bytes_per_launch"),
-                         # traffic was counted under the profile's trace configuration; False when this run's
-                         # autotuner chose another (spills, and so traffic, differ between builds)
-                         "traffic_same_config": summary.get("trace_config") == c2["trace_config"],
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                         # whether the traffic was counted under the configuration this run's autotuner chose (spills,
+                         # and so traffic, differ between builds)
+                         "traffic_same_config": same,
                          "algorithmic_bytes_per_launch": rb,
                          # the bound that holds: the fraction of cycles every SIMD issues a vector instruction
                          # (rocprof, same kernel)
@@ -628,11 +651,11 @@ def main():
         line.update(fwd_bwd_images_per_s=c4["fwd_bwd_images_per_s"], fwd_images_per_s=c4["fwd_images_per_s"],
                     fwd_bwd_over_fwd=c4["ratio"])
         bw = c4["backward_bytes"] / (c4["backward_kernel_ms"] * 1e-3) / 1e9
-        summary = _profile_summary("depth_backward_kernel_posed_bodies_traffic.json")
+        traffic, same = _traffic(_profile_summary("depth_backward_kernel_posed_bodies_traffic.json"), "backward",
+                                 c4["trace_config"])
         line["fwd_bwd_roofline"] = {
             "bound": "hbm", "kernel": "depth_backward_kernel", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": bw / HBM_PEAK_GBS, "traffic": summary.get("hbm_bytes_per_launch"),
-            "traffic_same_config": summary.get("trace_config") == c4["trace_config"],
+            "frac": bw / HBM_PEAK_GBS, "traffic": traffic, "traffic_same_config": same,
             "algorithmic_bytes_per_launch": c4["backward_bytes"], "kernel_ms": c4["backward_kernel_ms"]}
         line["fwd_bwd_trace_config"] = c4["trace_config"]
         line["fwd_bwd_hip_graph"] = c4["hip_graph"]
