@@ -326,6 +326,12 @@ ngx_status ngx_debug_check_reciprocal(int32_t device_id, uint64_t *mismatches_in
  * wavefront), for checking its lane layout. At most 2^20 waves. */
 ngx_status ngx_debug_wave_prefix_sum(int32_t device_id, const int32_t *values, int32_t waves, int32_t *sums);
 
+/* No counterpart in the reference: the per-world order in which a ray query with sort_rays traces its rays, run on
+ * device device_id over host arrays: directions (worlds x rays_per_world x 3) in, order (worlds x rays_per_world, each
+ * world's ray indices) out. rays_per_world at most 262,144, at most 2^26 rays in all; for checking that the order is the
+ * stable order of the rays' direction cells. */
+ngx_status ngx_debug_ray_order(int32_t device_id, const float *directions, int32_t rays_per_world, int32_t worlds, int32_t *order);
+
 /* No counterpart in the reference: how many entries (1 to 128, the default) of the LDS stack on which the shared
  * per-lane bottom-level walk gives far subtrees to idle lanes this context's later calls use; the lanes keep what does
  * not fit on their own stacks. Results do not depend on it (a ray's ties aside); a test lowers it to run that path. */

@@ -85,6 +85,35 @@ extern "C" ngx_status ngx_debug_check_reciprocal(int32_t device_id, uint64_t *mi
     return NGX_OK;
 }
 
+extern "C" ngx_status ngx_debug_ray_order(int32_t device_id, const float *directions, int32_t rays_per_world, int32_t worlds,
+                                          int32_t *order) {
+    if (rays_per_world < 0 || worlds < 0 || rays_per_world > ngx::RAY_ORDER_MAX_RAYS ||
+        (long long)rays_per_world * worlds > (1ll << 26))
+        return NGX_INVALID_ARGUMENT;
+    const size_t n = (size_t)rays_per_world * worlds;
+    if (n == 0) return NGX_OK;
+    if (!directions || !order) return NGX_INVALID_ARGUMENT;
+    if (hipSetDevice(device_id) != hipSuccess) return NGX_DEVICE_ERROR;
+    float *d = nullptr;
+    uint32_t *keys = nullptr;
+    int *vals = nullptr, *out = nullptr;
+    bool ok = hipMalloc(&d, 3 * n * sizeof(float)) == hipSuccess && hipMalloc(&keys, 2 * n * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&vals, n * sizeof(int)) == hipSuccess;
+    if (!ok) {
+        hipFree(d);
+        hipFree(keys);
+        hipFree(vals);
+        return NGX_OUT_OF_MEMORY;
+    }
+    ok = hipMemcpy(d, directions, 3 * n * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         ngx::launch_ray_order(d, rays_per_world, worlds, keys, vals, &out, nullptr) == hipSuccess &&
+         hipMemcpy(order, out, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    hipFree(d);
+    hipFree(keys);
+    hipFree(vals);
+    return ok ? NGX_OK : NGX_DEVICE_ERROR;
+}
+
 extern "C" ngx_status ngx_debug_wave_prefix_sum(int32_t device_id, const int32_t *values, int32_t waves, int32_t *sums) {
     if (waves < 0 || (waves > 0 && (!values || !sums)) || waves > (1 << 20)) return NGX_INVALID_ARGUMENT;
     if (waves == 0) return NGX_OK;
