@@ -618,6 +618,7 @@ def main():
         kernel = "render_persistent_kernel" if c2["trace_config"].get("persistent") else "render_kernel"
         summary = _profile_summary(f"{kernel}_body_worlds_traffic.json")
         traffic, same = _traffic(summary, "render", c2["trace_config"])
+        bb = build_bytes(4096, 64, c2["instances"])
         line.update({
             "value": c2["views"] / sec,
             "ms_per_step": sec * 1e3,
@@ -640,6 +641,12 @@ def main():
                          # (rocprof, same kernel)
                          "valu_busy": summary.get("kernels", {}).get("render_kernel_sq", {}).get("valu_busy")},
             "step_hbm_frac": (rb + build_bytes(4096, 64, c2["instances"])) / sec / 1e9 / HBM_PEAK_GBS,
+            # the top-level build alone, the step's one bandwidth-shaped phase: its algorithmic bytes over its event time
+            "build_roofline": {"bound": "hbm", "kernel": "tlas_build_kernel",
+                               "achieved": bb / (c2["build_ms"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": bb / (c2["build_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "traffic": summary.get("kernels", {}).get("tlas_build_kernel", {}).get("hbm_bytes_per_launch"),
+                               "algorithmic_bytes_per_launch": bb},
         })
     if c1:
         line.update(instanced_worlds_64x64_images_per_s=c1["images_per_s"], instanced_worlds_64x64_ms=c1["ms"],
