@@ -281,8 +281,9 @@ ngx_status ngx_set_trace_config(ngx_ctx *ctx, const ngx_trace_config *config);
 ngx_status ngx_get_trace_config(const ngx_ctx *ctx, ngx_trace_config *config);
 
 /* Tries every trace configuration worth trying on this scene and these cameras (each rendered once to warm
- * up, then `reps` times timed with events on `stream`), keeps the fastest, and returns it with its time per
- * render. Synchronous; not while `stream` is being captured. The outputs are left as the last render wrote
+ * up, then `reps` times timed with events on `stream`; the four fastest walk and tile choices again over 4 x reps
+ * renders, then the kernel variants of the winner over 2 x reps), keeps the fastest, and returns it with its time
+ * per render. Synchronous; not while `stream` is being captured. The outputs are left as the last render wrote
  * them. */
 ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, float fov, uint32_t outputs, int32_t reps,
                         void *stream, ngx_trace_config *best, float *best_ms);

@@ -1227,6 +1227,25 @@ float time_calls(const hipStream_t s, const int reps, ngx_status *failure, Call 
     return ms;
 }
 
+// Times each configuration with `call`; on a failure the context gets back `start` and the failure is raised.
+template <class Call>
+std::vector<float> time_each(ngx_ctx *ctx, const std::vector<ngx_trace_config> &cands, const ngx_trace_config &start,
+                             const hipStream_t s, const int reps, Call &&call) {
+    std::vector<float> times;
+    for (const ngx_trace_config &c : cands) {
+        ngx_set_trace_config(ctx, &c);
+        ngx_status failure = NGX_OK;
+        const float ms = time_calls(s, reps, &failure, call);
+        if (failure != NGX_OK) {
+            const std::string message = g_last_error;
+            ngx_set_trace_config(ctx, &start);
+            fail(failure, "autotune: " + message);
+        }
+        times.push_back(ms);
+    }
+    return times;
+}
+
 // Tries each configuration with `call` and keeps the fastest; on a failure the context gets back `start` and the
 // failure is raised.
 template <class Call>
@@ -1312,9 +1331,17 @@ ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, fl
                 cands.push_back(ngx_trace_config{tw, 1, 0, 64, 0, never, start.sort_rays, 0, 0, start.ray_blocks, start.backward_blocks,
                                                  start.rgb_two_pass});
         }
+        // every candidate timed briefly, then the four fastest again over four times the renders: configurations a
+        // percent or two apart are otherwise told apart by noise
+        const std::vector<float> first = time_each(ctx, cands, start, s, reps, render);
+        std::vector<int> rank(cands.size());
+        for (size_t i = 0; i < rank.size(); ++i) rank[i] = (int)i;
+        std::stable_sort(rank.begin(), rank.end(), [&](const int a, const int b) { return first[a] < first[b]; });
+        std::vector<ngx_trace_config> finalists;
+        for (size_t i = 0; i < rank.size() && i < 4; ++i) finalists.push_back(cands[rank[i]]);
         ngx_trace_config winner = start;
         float best_time = 0.0f;
-        keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
+        keep_fastest(ctx, finalists, start, s, 4 * reps, winner, best_time, render);
         // then, for the winner's walk, the kernel variants: workgroups per compute unit of the depth / normal / ID
         // kernel and of the RGB kernel (the per-lane walk has one variant of each)
         if (winner.packet) {
@@ -1326,7 +1353,7 @@ ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, fl
                     c.render_blocks = b;
                     cands.push_back(c);
                 }
-            keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
+            keep_fastest(ctx, cands, start, s, 2 * reps, winner, best_time, render);
             if (outputs & NGX_OUTPUT_RGB) {
                 const ngx_trace_config base2 = winner;
                 cands.clear();
@@ -1340,7 +1367,7 @@ ngx_status ngx_autotune(ngx_ctx *ctx, const float *cams12, ngx_memory memory, fl
                         cands.push_back(c);
                     }
                 }
-                keep_fastest(ctx, cands, start, s, reps, winner, best_time, render);
+                keep_fastest(ctx, cands, start, s, 2 * reps, winner, best_time, render);
             }
         }
         ngx_set_trace_config(ctx, &winner);
