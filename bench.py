@@ -228,7 +228,7 @@ def bench_instanced_worlds(args, torch):
         except Exception as e:  # graph capture is an optimization; direct launches are the same work
             print(f"# hip graph capture unavailable ({e}); timing direct launches", file=sys.stderr, flush=True)
     sec = timed_steps(step, max(10, args.steps), args.warmup, torch)
-    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3, reference_claim_images_per_s=1.0e6)
+    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3, reference_claim_images_per_s=1.0e6, trace_config=r.trace_config)
 
 
 def bench_lidar(args, torch):
@@ -650,7 +650,8 @@ def main():
         })
     if c1:
         line.update(instanced_worlds_64x64_images_per_s=c1["images_per_s"], instanced_worlds_64x64_ms=c1["ms"],
-                    instanced_worlds_reference_claim_images_per_s=c1["reference_claim_images_per_s"])
+                    instanced_worlds_reference_claim_images_per_s=c1["reference_claim_images_per_s"],
+                    instanced_worlds_trace_config=c1["trace_config"])
     if c3:
         line.update(lidar_rays_per_s=c3["rays_per_s"], lidar_ms=c3["ms"], lidar_hit_fraction=c3["hit_frac"],
                     lidar_trace_config=c3["trace_config"])
