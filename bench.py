@@ -232,8 +232,9 @@ def bench_instanced_worlds(args, torch):
 
 
 def bench_lidar(args, torch):
-    """LiDAR: 1024 worlds x 65,536 rays over 100,000 instances of a 138,000-triangle grid; rays resident on the
-    device, results left there (zero-copy)."""
+    """LiDAR: 1024 worlds x 65,536 rays over 100,000 moving instances of a 138,000-triangle grid; a step places the
+    instances (new matrices from the device, four resident sets cycled, every instance moved a little) and answers the
+    query; rays resident on the device, results left there (zero-copy)."""
     from neraulgraphics_amd import workloads
     cfg = workloads.make("lidar_sweeps")
     r = workloads.make_renderer(cfg)
@@ -242,12 +243,27 @@ def bench_lidar(args, torch):
     if not fixed_config(r, args) and not args.no_tune:
         tuned = r.autotune_rays(o, d, cfg["max_distance"], reps=2)
         print(f"# LiDAR autotuned: {tuned}", file=sys.stderr, flush=True)
-    for _ in range(2):
+    rng = np.random.default_rng(9)
+    sets = []
+    for i in range(4):
+        m = cfg["matrices"].copy()
+        if i:
+            m[..., 3::4] += rng.normal(0, 0.05, m[..., 3::4].shape).astype(np.float32)
+        sets.append(torch.from_numpy(np.ascontiguousarray(m)).cuda())
+    k = [0]
+
+    def step():
+        r.set_instance_transforms(sets[k[0] % 4])
         r.cast_rays(o, d, cfg["max_distance"])
+        k[0] += 1
+
+    for _ in range(2):
+        step()
     torch.cuda.synchronize()
     steps = max(3, args.steps // 4)
-    t = timed_steps(lambda: r.cast_rays(o, d, cfg["max_distance"]), steps, 1, torch)
+    t = timed_steps(step, steps, 1, torch)
     n = o.shape[0] * o.shape[1]
+    r.set_instance_transforms(sets[0])
     out = r.cast_rays(o, d, cfg["max_distance"])
     torch.cuda.synchronize()
     return dict(rays_per_s=n / t, ms=t * 1e3, rays=n, hit_frac=float(torch.isfinite(out["distance"]).float().mean()),
