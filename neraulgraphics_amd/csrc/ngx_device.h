@@ -199,6 +199,20 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
     v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
     return v;
 }
+// The sum of v over the wavefront (every lane active), by wave_prefix_sum's stages (lane 63's inclusive sum, read from
+// every lane): register to register, with no LDS round trip, in one fixed order.
+__device__ __forceinline__ float wave_total(float v) {
+#define NGX_DPP_ADD(ctrl, row_mask, bound) \
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, row_mask, 0xf, bound))
+    NGX_DPP_ADD(0x111, 0xf, true);   // row_shr:1
+    NGX_DPP_ADD(0x112, 0xf, true);   // row_shr:2
+    NGX_DPP_ADD(0x114, 0xf, true);   // row_shr:4
+    NGX_DPP_ADD(0x118, 0xf, true);   // row_shr:8
+    NGX_DPP_ADD(0x142, 0xa, false);  // row_bcast:15, rows 1 and 3
+    NGX_DPP_ADD(0x143, 0xc, false);  // row_bcast:31, rows 2 and 3
+#undef NGX_DPP_ADD
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
 // A vector register holding some value the compiler neither computes nor knows, for the operands of lanes whose results
 // are masked out (idle lanes of a per-lane walk): no instruction is spent on it. An uninitialized variable would not do:
 // its value is undefined to the compiler, which may then fold it through the masks (an undefined value ANDed with a mask

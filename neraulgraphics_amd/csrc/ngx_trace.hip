@@ -1540,11 +1540,6 @@ __global__ __launch_bounds__(256, BLOCKS) void ray_kernel(const RayArgs A) {
 // X = Rb y + pb (y the point in its body's frame), dL/dpb = a and dL/dq_unit = a^T (dR/dq_unit) y, where
 // a = dL/dt * n / n.d; the projection onto the raw quaternion is applied once per body afterwards.
 
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // The 7 gradient terms of one lane's hit: returns false when the hit does not depend on a body pose.
 __device__ __forceinline__ bool hit_pose_terms(const TraceScene &S, const int world, const int bodies_per_world,
                                                const int *body_of_instance, const float *local_rot_scale, const float *body_poses,
@@ -1595,10 +1590,11 @@ __device__ __forceinline__ void wave_accumulate(const bool has, const int body, 
         const int b0 = __builtin_amdgcn_readlane(body, first_lane(pending));
         const bool mine = has && body == b0;
         const uint64_t m = ballot(mine);
-        for (int k = 0; k < 7; ++k) {
-            const float s = wave_sum(mine ? g[k] : 0.0f);
-            if (lane == 0) acc[7 * b0 + k] += s;
-        }
+        // the seven sums side by side, then one lane adds them in
+        float s[7];
+        for (int k = 0; k < 7; ++k) s[k] = wave_total(mine ? g[k] : 0.0f);
+        if (lane == 0)
+            for (int k = 0; k < 7; ++k) acc[7 * b0 + k] += s[k];
         pending &= ~m;
     }
 }
