@@ -106,11 +106,7 @@ __device__ __forceinline__ int wave_max(int v) {
 __device__ void block_exclusive_scan(int *data, const int count, int *wave_tot) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int x = tid < count ? data[tid] : 0;
-    int incl = x;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    const int incl = wave_prefix_sum(x);
     if (lane == 63) wave_tot[wave] = incl;
     __syncthreads();
     int base = 0;
@@ -537,11 +533,7 @@ __global__ __launch_bounds__(1024) void big_sort_scan_kernel(const BigBuildScrat
         const int c = carry;
         const int idx = base + tid;
         const int x = idx < total ? S.tile_counts[idx] : 0;
-        int incl = x;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
+        int incl = wave_prefix_sum(x);
         if (lane == 63) wave_tot[wave] = incl;
         __syncthreads();
         for (int wv = 0; wv < wave; ++wv) incl += wave_tot[wv];
@@ -738,11 +730,7 @@ __global__ __launch_bounds__(1024) void ray_order_one_pass_kernel(const float *d
         const int s0 = tid * SPAN;
         int sum = 0;
         for (int k = 0; k < SPAN; ++k) sum += slot[s0 + k];
-        int incl = sum;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
+        const int incl = wave_prefix_sum(sum);
         if (lane == 63) wave_tot[wave] = incl;
         __syncthreads();
         int run = incl - sum;
@@ -817,11 +805,7 @@ __global__ __launch_bounds__(1024) void ray_order_kernel(const float *dirs, cons
             const int cin = carry;
             const int idx = b0 + tid;
             const int x = idx < total ? counts[idx] : 0;
-            int incl = x;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-            }
+            int incl = wave_prefix_sum(x);
             if (lane == 63) wave_tot[wave] = incl;
             __syncthreads();
             for (int wv = 0; wv < wave; ++wv) incl += wave_tot[wv];
