@@ -333,6 +333,10 @@ ngx_status ngx_debug_wave_prefix_sum(int32_t device_id, const int32_t *values, i
  * stable order of the rays' direction cells. */
 ngx_status ngx_debug_ray_order(int32_t device_id, const float *directions, int32_t rays_per_world, int32_t worlds, int32_t *order);
 
+/* No counterpart in the reference: how the last RGB picture of this context was drawn, 1 or 2 passes (see rgb_two_pass;
+ * 0 before any), for checking which kernels ran. */
+ngx_status ngx_debug_rgb_passes(const ngx_ctx *ctx, int32_t *passes);
+
 /* No counterpart in the reference: how many entries (1 to 128, the default) of the LDS stack on which the shared
  * per-lane bottom-level walk gives far subtrees to idle lanes this context's later calls use; the lanes keep what does
  * not fit on their own stacks. Results do not depend on it (a ray's ties aside); a test lowers it to run that path. */

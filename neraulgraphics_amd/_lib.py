@@ -118,6 +118,7 @@ _SIGNATURES = {
     "ngx_debug_check_reciprocal": (ctypes.c_int32, [ctypes.c_int32, _VP, _VP, _VP]),
     "ngx_debug_wave_prefix_sum": (ctypes.c_int32, [ctypes.c_int32, _VP, ctypes.c_int32, _VP]),
     "ngx_debug_ray_order": (ctypes.c_int32, [ctypes.c_int32, _VP, ctypes.c_int32, ctypes.c_int32, _VP]),
+    "ngx_debug_rgb_passes": (ctypes.c_int32, [_VP, ctypes.POINTER(ctypes.c_int32)]),
     "ngx_debug_set_share_capacity": (ctypes.c_int32, [_VP, ctypes.c_int32]),
     "ngx_build_skip_host": (ctypes.c_int32, [_VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, c_int32_p]),
 }

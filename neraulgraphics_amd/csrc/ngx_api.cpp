@@ -70,6 +70,17 @@ bool quaternion_usable(const float *q) {
            q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3] > 0.0f;
 }
 
+// Allocates count elements for ptr unless it holds some already; false (and nothing held, no error left pending) when
+// the device has no room.
+template <class T>
+bool try_device_alloc(T *&ptr, const size_t count) {
+    if (ptr) return true;
+    if (hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T)) == hipSuccess) return true;
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return false;
+}
+
 template <class T>
 void device_alloc(T *&ptr, const size_t count) {
     if (ptr) {
@@ -155,6 +166,7 @@ struct ngx_ctx {
     float4 *d_inst_mat = nullptr, *d_sphere_mat = nullptr, *d_lights = nullptr;
     uint8_t *d_rgb = nullptr;
     bool shade_no_bounce = false;       // every surface's reflectivity below the path's cut-off: paths end at their first hit
+    int last_rgb_passes = 0;            // how the last RGB picture was drawn: 1 or 2 passes (0 before any)
     int4 *d_rgb_hits = nullptr;         // the two-pass picture's hits, pixels
     float2 *d_rgb_uv = nullptr;         // and their barycentrics (meshes with vertex normals)
 
@@ -926,15 +938,20 @@ ngx_status ngx_render_batch(ngx_ctx *ctx, const float *cams12, ngx_memory memory
             g.spp = spp;
             g.frame = frame;
             g.blocks = ctx->rgb_blocks;
-            if (ctx->rgb_two_pass && spp == 1 && ctx->shade_no_bounce && ctx->persistent && ctx->compute_units > 0 && a.walk != 0) {
-                if (!ctx->d_rgb_hits) device_alloc(ctx->d_rgb_hits, pixels);
-                if (ctx->d_attrs && !ctx->d_rgb_uv) device_alloc(ctx->d_rgb_uv, pixels);
+            bool two_pass = ctx->rgb_two_pass && spp == 1 && ctx->shade_no_bounce && ctx->persistent && ctx->compute_units > 0 &&
+                            a.walk != 0;
+            // the hit records are an optimization: where the device cannot hold them, the one-pass kernel draws the
+            // same bytes
+            if (two_pass && !try_device_alloc(ctx->d_rgb_hits, pixels)) two_pass = false;
+            if (two_pass && ctx->d_attrs && !try_device_alloc(ctx->d_rgb_uv, pixels)) two_pass = false;
+            if (two_pass) {
                 g.hits = ctx->d_rgb_hits;
                 g.hit_uv = ctx->d_attrs ? ctx->d_rgb_uv : nullptr;
                 HIP_CHECK(ngx::launch_rgb_two_pass(g, ctx->render_blocks, s, ctx->compute_units));
             } else {
                 HIP_CHECK(ngx::launch_rgb(g, s, ctx->persistent ? ctx->compute_units : 0));
             }
+            ctx->last_rgb_passes = two_pass ? 2 : 1;
         }
         if (timed) HIP_CHECK(hipEventRecord(ctx->ev1, s));
         ctx->timed = timed;
@@ -1529,6 +1546,13 @@ ngx_status ngx_build_blas_host(const float *vertices, int32_t vertex_count, cons
                                       c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.0f};
             std::memcpy(o, packed, sizeof(packed));
         }
+    });
+}
+
+ngx_status ngx_debug_rgb_passes(const ngx_ctx *ctx, int32_t *passes) {
+    return guarded([&] {
+        require(ctx != nullptr && passes != nullptr, "context and passes must not be null");
+        *passes = ctx->last_rgb_passes;
     });
 }
 

@@ -906,7 +906,9 @@ def test_two_pass_picture_gives_the_one_pass_bytes(gpu):
     materials, two spheres, a soft light and a spotlight. The hit pass walks the trees in another order, so a ray
     through an edge two triangles share may keep the other triangle (the parity bar's one exception): all but a
     handful of pixels are the same bytes."""
+    import ctypes
     import neraulgraphics_amd as ng
+    from neraulgraphics_amd import _lib
     from .test_gpu_normals import _smooth_meshes
     _say("two-pass picture against the one-pass kernel")
     bw = workloads.make("body_worlds", worlds=24)
@@ -929,11 +931,23 @@ def test_two_pass_picture_gives_the_one_pass_bytes(gpu):
             for two in (True, False):
                 r.set_trace_config(**WALKS[walk], rgb_two_pass=two)
                 out[two] = r.render(cfg["cameras"], cfg["fov"], 1, rgb=True, frame=5)["rgb"].astype(np.int32)
+                passes = ctypes.c_int32()
+                _lib.check(r._lib.ngx_debug_rgb_passes(r._h, ctypes.byref(passes)))
+                assert passes.value == (2 if two else 1), (name, walk, two, passes.value)
             differ = np.any(out[True] != out[False], axis=-1)
             mean = float(np.abs(out[True] - out[False]).mean())
             _say(f"{name}, {walk} walk: {int(differ.sum())} of {differ.size} pixels differ; mean difference {mean:.6f}")
             assert int(differ.sum()) <= max(2, differ.size // 20000) and mean < 0.05, (name, walk)
             assert len(np.unique(out[True].reshape(-1, 3), axis=0)) > 200
+    # a surface that reflects (reflectivity 0.3) makes paths longer than one hit: one pass, whatever the setting
+    mirror = workloads.make("instanced_worlds", worlds=2)
+    mirror["instances"] = [ng.Instance(i.mesh, material=ng.Material((0.7, 0.7, 0.9), 0.5, 0.3, 0.3, 20.0)) if k == 0 else i
+                           for k, i in enumerate(mirror["instances"])]
+    r3 = workloads.make_renderer(mirror, lights=[ng.Light((2.0, 6.0, -4.0), intensity=1.2)])
+    r3.render(mirror["cameras"], mirror["fov"], 1, rgb=True)
+    passes = ctypes.c_int32()
+    _lib.check(r3._lib.ngx_debug_rgb_passes(r3._h, ctypes.byref(passes)))
+    assert passes.value == 1
 
 
 def test_rgb_known_answers_on_the_device(gpu, oracle):
