@@ -267,7 +267,8 @@ const char *ngx_last_error(void);
  *  rgb_two_pass  1 (the default): the RGB picture at one sample a pixel, where no surface is reflective enough to
  *              bounce a ray (every reflectivity below 1e-3), with the persistent launch and the packet walk, is drawn
  *              in two passes: the camera rays' hits (16 bytes a pixel kept on the device, 24 with vertex normals),
- *              then their shading and shadow rays; the same bytes as the one pass (0). */
+ *              then their shading and shadow rays; the same bytes as the one pass (0), which is also taken where the
+ *              device cannot hold the records. */
 typedef struct {
     int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles, sort_rays;
     /* Kernel variants, by the workgroups of 256 threads per compute unit each is built for (the registers a lane
