@@ -261,7 +261,7 @@ const char *ngx_last_error(void);
  *  persistent  1 (the default): a fixed grid of 8 workgroups per compute unit takes tiles from per-XCD
  *              counters; 0: one wavefront per tile, dealt by the dispatcher;
  *  lane_min_triangles  packet walk: the bottom level of meshes with at least this many triangles is walked
- *              lane by lane (each ray its own path) instead of as a packet (default 1000);
+ *              lane by lane (each ray its own path) instead of as a packet (default 300);
  *  sort_rays   ray queries of 1024 to 262,144 rays a world: 1 (the default) traces each world's rays in an order
  *              of their directions, so a wavefront's rays are neighbours; results are in the caller's order;
  *  rgb_two_pass  1 (the default): the RGB picture at one sample a pixel, where no surface is reflective enough to
@@ -273,8 +273,9 @@ typedef struct {
     int32_t tile_w, xcd_remap, packet, pair_limit, persistent, lane_min_triangles, sort_rays;
     /* Kernel variants, by the workgroups of 256 threads per compute unit each is built for (the registers a lane
      * may hold follow from it; 0 picks the walk's default): the depth / normal / ID kernel 5, 6 or 7 (default 7),
-     * the RGB kernel 4, 5, 6 or 7 (default 4 for the packet walk, 7 for the hybrid one), the ray-query kernel 5, 6
-     * or 7 (default 6), the depth backward 4, 5 or 6 (default 5). The per-lane walk has one variant of each. */
+     * the RGB kernel 4, 5, 6 or 7 (default 4 for the packet walk, 7 for the hybrid one; the two-pass picture's hit pass
+     * takes render_blocks and its shading pass rgb_blocks, default 4), the ray-query kernel 5, 6 or 7 (default 6), the
+     * depth backward 4, 5 or 6 (default 5). The per-lane walk has one variant of each. */
     int32_t render_blocks, rgb_blocks, ray_blocks, backward_blocks;
     int32_t rgb_two_pass;
 } ngx_trace_config;

@@ -234,7 +234,7 @@ class BatchRenderer:
         _lib.check(self._lib.ngx_synchronize(self._h))
 
     @_serialized
-    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=1000,
+    def set_trace_config(self, tile_w=8, xcd_remap=True, packet=True, pair_limit=64, persistent=True, lane_min_triangles=300,
                          sort_rays=True, render_blocks=0, rgb_blocks=0, ray_blocks=0, backward_blocks=0, rgb_two_pass=True):
         """Tile shape of a wavefront (tile_w x 64/tile_w pixels), XCD-contiguous tile order, packet (True, the
         default) or per-lane (False) walks, (ray, triangle) pairs batched before testing (1..64), the

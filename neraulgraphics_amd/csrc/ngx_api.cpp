@@ -195,7 +195,7 @@ struct ngx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     hipStream_t last_stream = nullptr;
-    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 1000, sort_rays = 1;
+    int tile_w = 8, xcd_remap = 1, packet = 1, pair_limit = 64, persistent = 1, lane_min_tris = 300, sort_rays = 1;
     int share_cap = ngx::SHARE_STACK;   // the two-child shared walk's LDS stack entries (ngx_debug_set_share_capacity)
     // kernel variants by workgroups per compute unit (0: each walk's default)
     int render_blocks = 0, rgb_blocks = 0, ray_blocks = 0, backward_blocks = 0;

@@ -1835,7 +1835,7 @@ static int resident_grid(const int compute_units, const long long tiles) {
 
 // The two-pass picture, persistent launches of the packet and hybrid walks only: the hit pass built like the depth
 // kernel (5, 6 or 7 workgroups per compute unit: hit_blocks, 0 for the default), the shading pass like the RGB
-// kernel (4 to 7: a.blocks).
+// kernel (4 to 7: a.blocks, 0 for 4).
 hipError_t launch_rgb_two_pass(const RgbArgs &a, const int hit_blocks, hipStream_t stream, const int compute_units) {
     if (a.r.total_tiles <= 0) return hipSuccess;
     if (a.spp != 1 || a.shade.light_count < 0 || a.shade.light_count > MAX_LIGHTS || a.hits == nullptr ||
@@ -1852,7 +1852,7 @@ hipError_t launch_rgb_two_pass(const RgbArgs &a, const int hit_blocks, hipStream
 #undef NGX_LAUNCH
     zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
 #define NGX_LAUNCH(W, B) rgb_shade_kernel<W, B><<<resident_grid<rgb_shade_kernel<W, B>>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
-    switch (a.blocks ? a.blocks : hybrid ? RGB_BLOCKS_DEFAULT_HYBRID : RGB_BLOCKS_DEFAULT_PACKET) {
+    switch (a.blocks ? a.blocks : SHADE_BLOCKS_DEFAULT) {
     case 4: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 4); else NGX_LAUNCH(WALK_PACKET, 4); break;
     case 5: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 5); else NGX_LAUNCH(WALK_PACKET, 5); break;
     case 6: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 6); else NGX_LAUNCH(WALK_PACKET, 6); break;

@@ -169,7 +169,7 @@ def test_trace_configurations_render_the_same_images(gpu, oracle, tile_w, pair_l
     r = workloads.make_renderer(cfg)
     r.set_trace_config(tile_w=tile_w, pair_limit=pair_limit, persistent=persistent)
     assert r.trace_config == dict(tile_w=tile_w, xcd_remap=1, packet=1, pair_limit=pair_limit, persistent=int(persistent),
-                                  lane_min_triangles=1000, sort_rays=1, render_blocks=0, rgb_blocks=0, ray_blocks=0,
+                                  lane_min_triangles=300, sort_rays=1, render_blocks=0, rgb_blocks=0, ray_blocks=0,
                                   backward_blocks=0, rgb_two_pass=1)
     out = r.render(cfg["cameras"], cfg["fov"], depth=True, normals=True, object_ids=True)
     ref = forward(cfg, oracle)
