@@ -274,8 +274,9 @@ typedef struct {
     /* Kernel variants, by the workgroups of 256 threads per compute unit each is built for (the registers a lane
      * may hold follow from it; 0 picks the walk's default): the depth / normal / ID kernel 5, 6 or 7 (default 7),
      * the RGB kernel 4, 5, 6 or 7 (default 4 for the packet walk, 7 for the hybrid one; the two-pass picture's hit pass
-     * takes render_blocks and its shading pass rgb_blocks, default 4), the ray-query kernel 5, 6 or 7 (default 6), the
-     * depth backward 4, 5 or 6 (default 5). The per-lane walk has one variant of each. */
+     * takes render_blocks and its shading pass rgb_blocks, default 5 for a scene of one light and 4 otherwise), the
+     * ray-query kernel 5, 6 or 7 (default 6), the depth backward 4, 5 or 6 (default 5). The per-lane walk has one
+     * variant of each. */
     int32_t render_blocks, rgb_blocks, ray_blocks, backward_blocks;
     int32_t rgb_two_pass;
 } ngx_trace_config;

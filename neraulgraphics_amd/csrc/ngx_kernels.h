@@ -209,8 +209,9 @@ struct PoseGradFinishArgs {
 constexpr int RENDER_BLOCKS[] = {5, 6, 7}, RENDER_BLOCKS_DEFAULT = 7;
 constexpr int RGB_BLOCKS[] = {4, 5, 6, 7}, RGB_BLOCKS_DEFAULT_PACKET = 4, RGB_BLOCKS_DEFAULT_HYBRID = 7;
 // The two-pass picture's shading pass by default: 4 workgroups per compute unit, the build without spills (the reference's
-// falling-bodies scene, hybrid walk: 4.97 ms a frame for 32 bodies at 128x128 against 5.84 at 5)
-constexpr int SHADE_BLOCKS_DEFAULT = 4;
+// falling-bodies scene, hybrid walk: 4.97 ms a frame for 32 bodies at 128x128 against 5.84 at 5), and 5 for the build
+// for one light, which holds less across its shadow walk (the same scene: 4.75 ms at 5, 5.01 at 4, 4.86 at 6)
+constexpr int SHADE_BLOCKS_DEFAULT = 4, SHADE_ONE_LIGHT_BLOCKS_DEFAULT = 5;
 constexpr int RAY_BLOCKS[] = {5, 6, 7}, RAY_BLOCKS_DEFAULT = 6;
 constexpr int BACKWARD_BLOCKS[] = {4, 5, 6}, BACKWARD_BLOCKS_DEFAULT = 5;
 

@@ -1257,13 +1257,88 @@ __device__ __forceinline__ TilePixel tile_pixel(const RenderArgs &A, const long 
 __device__ __forceinline__ uint32_t pixel_stream(const RenderArgs &A, const TilePixel &p, const int frame) {
     return pcg_hash((uint32_t)(p.y * A.width + p.x) ^ pcg_hash((uint32_t)frame + 1u));
 }
-// The pixel's bytes from the sum of its samples (render_core.h:276-283).
-__device__ __forceinline__ void write_rgb(const RgbArgs &G, const size_t index, const float3 sum) {
+// The pixel's bytes from the sum of its samples (render_core.h:276-283), red in the low byte.
+__device__ __forceinline__ uint32_t rgb_bytes(const RgbArgs &G, const float3 sum) {
     const float3 c = mul3(sum, 1.0f / (float)G.spp);
+    const uint32_t r = (uint32_t)(saturate(srgb_channel(c.x)) * 255.0f + 0.5f);
+    const uint32_t g = (uint32_t)(saturate(srgb_channel(c.y)) * 255.0f + 0.5f);
+    const uint32_t b = (uint32_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
+    return r | g << 8 | b << 16;
+}
+__device__ __forceinline__ void store_rgb(const RgbArgs &G, const size_t index, const uint32_t bytes) {
     uint8_t *out = G.rgb + 3 * index;
-    out[0] = (uint8_t)(saturate(srgb_channel(c.x)) * 255.0f + 0.5f);
-    out[1] = (uint8_t)(saturate(srgb_channel(c.y)) * 255.0f + 0.5f);
-    out[2] = (uint8_t)(saturate(srgb_channel(c.z)) * 255.0f + 0.5f);
+    out[0] = (uint8_t)bytes;
+    out[1] = (uint8_t)(bytes >> 8);
+    out[2] = (uint8_t)(bytes >> 16);
+}
+__device__ __forceinline__ void write_rgb(const RgbArgs &G, const size_t index, const float3 sum) {
+    store_rgb(G, index, rgb_bytes(G, sum));
+}
+
+// A hit's point, surface normal and material records (render_core.h:206-214): the instance's template material,
+// the floor's checker tint, or the sphere's material.
+__device__ __forceinline__ void hit_material(const RgbArgs &G, const int kind, const float3 o, const float3 d, const float best,
+                                             const LaneHit &h, float3 &point, float3 &normal, float4 &m0, float4 &m1) {
+    const RenderArgs &A = G.r;
+    const ShadeScene &E = G.shade;
+    point = add3(o, mul3(d, best));
+    int object;
+    surface(A.s, kind, o, d, best, h, normal, object);
+    if (kind == OBJECT_INSTANCE) {
+        const float4 *mp = E.instance_materials + 2 * (size_t)A.s.inst[h.inst].tmpl;
+        m0 = mp[0];
+        m1 = mp[1];
+    } else if (kind == OBJECT_FLOOR) {
+        m0 = E.floor_material[0];
+        m1 = E.floor_material[1];
+        const int cx = (int)floorf(point.x * E.floor_checker_scale), cz = (int)floorf(point.z * E.floor_checker_scale);
+        const float3 tint = ((cx + cz) & 1) ? mk3(1.0f, 1.0f, 1.0f) : mk3(1.0f, 0.7f, 0.3f);
+        m0.x = tint.x * 0.3f;
+        m0.y = tint.y * 0.3f;
+        m0.z = tint.z * 0.3f;
+    } else {
+        const float4 *mp = E.sphere_materials + 2 * (size_t)(kind - OBJECT_SPHERE);
+        m0 = mp[0];
+        m1 = mp[1];
+    }
+}
+
+// Light i at a lane's hit (render_core.h:215-233): its sample point drawn from rng, the direction and distance to
+// it, and the diffuse (cd) and specular (cs) light it adds if nothing blocks it. The light's terms come first:
+// where both are zero (a surface facing away from the light, and no highlight) adding them changes nothing whether
+// or not the light is blocked (the sums start at +0 and only grow), so the result says whether the lane needs a
+// shadow ray at all.
+__device__ __forceinline__ bool light_terms(const ShadeScene &E, const int i, const bool hit, const float3 point, const float3 normal,
+                                            const float3 d, const float4 m1, uint32_t &rng, float3 &cd, float3 &cs,
+                                            float3 &light_dir, float &light_distance) {
+    const float4 lp0 = E.lights[4 * i], lq = E.lights[4 * i + 1], ldir = E.lights[4 * i + 2], lcol = E.lights[4 * i + 3];
+    light_dir = mk3(0.0f, 1.0f, 0.0f);
+    light_distance = 0.0f;
+    float strength = 0.0f;
+    if (hit) {
+        const float3 lp = sample_sphere_light(xyz(lp0), lq.x, rng);
+        const float3 to_light = sub3(lp, point);
+        light_distance = sqrtf(dot3(to_light, to_light));
+        light_dir = mul3(to_light, rcp_rn(light_distance));
+        strength = lp0.w * light_factor(lq, ldir, light_dir, light_distance);
+    }
+    const bool test = hit && strength > 0.0f;
+    cd = mk3(0.0f, 0.0f, 0.0f);
+    cs = cd;
+    if (test) {
+        const float3 lit = mul3(xyz(lcol), strength);
+        cd = mul3(lit, fmaxf(0.0f, dot3(light_dir, normal)));
+        cs = mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z));
+    }
+    return test && (cd.x != 0.0f || cd.y != 0.0f || cd.z != 0.0f || cs.x != 0.0f || cs.y != 0.0f || cs.z != 0.0f);
+}
+
+// A hit's light from the summed unblocked terms and the ambient term (render_core.h:234-237), added to radiance.
+__device__ __forceinline__ void add_direct(const ShadeScene &E, const float4 m0, const float4 m1, const float3 diffuse,
+                                           const float3 specular, const float throughput, float3 &radiance) {
+    const float3 dc = xyz(m0);
+    const float3 direct = add3(mul3v(dc, mul3(diffuse, m0.w)), mul3(specular, m1.x));
+    radiance = add3(radiance, mul3(add3(direct, mul3(dc, E.ambient * m0.w)), throughput));
 }
 
 // What one traced segment of a path adds (render_core.h:206-240): the lanes in active traced o + t d, and kind
@@ -1283,51 +1358,13 @@ __device__ __forceinline__ void shade_hit(const RgbArgs &G, PairLds *L, const in
     }
     float3 normal = mk3(0.0f, 1.0f, 0.0f), point = o;
     float4 m0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m1 = m0;
-    if (hit) {
-        point = add3(o, mul3(d, best));
-        int object;
-        surface(A.s, kind, o, d, best, h, normal, object);
-        if (kind == OBJECT_INSTANCE) {
-            const float4 *mp = E.instance_materials + 2 * (size_t)A.s.inst[h.inst].tmpl;
-            m0 = mp[0];
-            m1 = mp[1];
-        } else if (kind == OBJECT_FLOOR) {
-            m0 = E.floor_material[0];
-            m1 = E.floor_material[1];
-            const int cx = (int)floorf(point.x * E.floor_checker_scale), cz = (int)floorf(point.z * E.floor_checker_scale);
-            const float3 tint = ((cx + cz) & 1) ? mk3(1.0f, 1.0f, 1.0f) : mk3(1.0f, 0.7f, 0.3f);
-            m0.x = tint.x * 0.3f;
-            m0.y = tint.y * 0.3f;
-            m0.z = tint.z * 0.3f;
-        } else {
-            const float4 *mp = E.sphere_materials + 2 * (size_t)(kind - OBJECT_SPHERE);
-            m0 = mp[0];
-            m1 = mp[1];
-        }
-    }
+    if (hit) hit_material(G, kind, o, d, best, h, point, normal, m0, m1);
     float3 diffuse = mk3(0.0f, 0.0f, 0.0f), specular = mk3(0.0f, 0.0f, 0.0f);
     for (int i = 0; i < E.light_count; ++i) {
-        const float4 lp0 = E.lights[4 * i], lq = E.lights[4 * i + 1], ldir = E.lights[4 * i + 2], lcol = E.lights[4 * i + 3];
-        float3 light_dir = mk3(0.0f, 1.0f, 0.0f);
-        float light_distance = 0.0f, strength = 0.0f;
-        if (hit) {
-            const float3 lp = sample_sphere_light(xyz(lp0), lq.x, rng);
-            const float3 to_light = sub3(lp, point);
-            light_distance = sqrtf(dot3(to_light, to_light));
-            light_dir = mul3(to_light, rcp_rn(light_distance));
-            strength = lp0.w * light_factor(lq, ldir, light_dir, light_distance);
-        }
-        // The light's terms come first: where both are zero (a surface facing away from the light, and no
-        // highlight) adding them changes nothing whether or not the light is blocked (the sums start at +0 and
-        // only grow), so that lane casts no shadow ray, and a wave with no such lane left skips the walk.
-        const bool test = hit && strength > 0.0f;
-        float3 cd = mk3(0.0f, 0.0f, 0.0f), cs = cd;
-        if (test) {
-            const float3 lit = mul3(xyz(lcol), strength);
-            cd = mul3(lit, fmaxf(0.0f, dot3(light_dir, normal)));
-            cs = mul3(lit, powf(fmaxf(0.0f, dot3(reflect3(light_dir, normal), d)), m1.z));
-        }
-        const bool need = test && (cd.x != 0.0f || cd.y != 0.0f || cd.z != 0.0f || cs.x != 0.0f || cs.y != 0.0f || cs.z != 0.0f);
+        float3 cd, cs, light_dir;
+        float light_distance;
+        const bool need = light_terms(E, i, hit, point, normal, d, m1, rng, cd, cs, light_dir, light_distance);
+        // a wave with no lane needing a shadow ray skips the walk
         const uint64_t shadow = ballot(need);
         if (!shadow) continue;
         const bool blocked = occluded<WALK, Shadow>(A.s, L, world, offset_origin(point, normal, light_dir), light_dir, shadow,
@@ -1338,9 +1375,7 @@ __device__ __forceinline__ void shade_hit(const RgbArgs &G, PairLds *L, const in
         }
     }
     if (hit) {
-        const float3 dc = xyz(m0);
-        const float3 direct = add3(mul3v(dc, mul3(diffuse, m0.w)), mul3(specular, m1.x));
-        radiance = add3(radiance, mul3(add3(direct, mul3(dc, E.ambient * m0.w)), throughput));
+        add_direct(E, m0, m1, diffuse, specular, throughput, radiance);
         throughput *= m1.y;
         if (throughput < MIN_THROUGHPUT) {
             active = false;
@@ -1372,7 +1407,7 @@ __device__ __forceinline__ void rgb_tile(const RgbArgs &G, PairLds *L, const lon
         float throughput = 1.0f;
         bool active = p.valid;
         for (int depth = 0; ballot(active); ++depth) {
-            float best;
+            float best = 0.0f;
             LaneHit h;
             ShadeRays stats;
             int kind = OBJECT_NONE;
@@ -1422,39 +1457,88 @@ __device__ __forceinline__ void rgb_hit_tile(const RgbArgs &G, PairLds *L, const
     if (G.hit_uv != nullptr) G.hit_uv[p.index] = kind == OBJECT_INSTANCE ? make_float2(h.u, h.v) : make_float2(0.0f, 0.0f);
 }
 
-template <int WALK>
-__device__ __forceinline__ void rgb_shade_tile(const RgbArgs &G, PairLds *L, const long long tile) {
-    const RenderArgs &A = G.r;
-    const TilePixel p = tile_pixel(A, tile);
-    const Cam cam = load_camera(A.cams + 12 * (size_t)p.view);
-    uint32_t rng = pixel_stream(A, p, G.frame);
-    const float jx = rand01(rng), jy = rand01(rng);
-    float3 o = cam.pos;
-    float3 d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)p.x + jx, (float)p.y + jy, A.width, A.height);
-    int kind = OBJECT_NONE;
-    float best = 0.0f;
+// The ray, its random numbers and its hit as the first pass traced them.
+struct ShadeStart {
+    TilePixel p;
+    uint32_t rng;
+    float3 o, d;
+    int kind;
+    float best;
     LaneHit h;
-    h.inst = -1;
-    h.tri = -1;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    if (p.valid) {
-        const int4 r = G.hits[p.index];
-        best = __int_as_float(r.x);
-        kind = r.y;
-        h.inst = r.z;
-        h.tri = r.w;
+};
+__device__ __forceinline__ ShadeStart shade_start(const RgbArgs &G, const long long tile) {
+    const RenderArgs &A = G.r;
+    ShadeStart q;
+    q.p = tile_pixel(A, tile);
+    const Cam cam = load_camera(A.cams + 12 * (size_t)q.p.view);
+    q.rng = pixel_stream(A, q.p, G.frame);
+    const float jx = rand01(q.rng), jy = rand01(q.rng);
+    q.o = cam.pos;
+    q.d = camera_ray_at(cam, A.fov_scale, A.aspect, (float)q.p.x + jx, (float)q.p.y + jy, A.width, A.height);
+    q.kind = OBJECT_NONE;
+    q.best = 0.0f;
+    q.h.inst = -1;
+    q.h.tri = -1;
+    q.h.u = 0.0f;
+    q.h.v = 0.0f;
+    if (q.p.valid) {
+        const int4 r = G.hits[q.p.index];
+        q.best = __int_as_float(r.x);
+        q.kind = r.y;
+        q.h.inst = r.z;
+        q.h.tri = r.w;
         if (G.hit_uv != nullptr) {
-            const float2 uv = G.hit_uv[p.index];
-            h.u = uv.x;
-            h.v = uv.y;
+            const float2 uv = G.hit_uv[q.p.index];
+            q.h.u = uv.x;
+            q.h.v = uv.y;
         }
     }
+    return q;
+}
+
+template <int WALK>
+__device__ __forceinline__ void rgb_shade_tile(const RgbArgs &G, PairLds *L, const long long tile) {
+    ShadeStart q = shade_start(G, tile);
     float3 radiance = mk3(0.0f, 0.0f, 0.0f);
     float throughput = 1.0f;
-    bool active = p.valid;
-    shade_hit<WALK, OcclusionTwoChild>(G, L, p.world, kind, best, h, active, o, d, rng, radiance, throughput);
-    if (p.valid) write_rgb(G, p.index, add3(mk3(0.0f, 0.0f, 0.0f), radiance));
+    bool active = q.p.valid;
+    shade_hit<WALK, OcclusionTwoChild>(G, L, q.p.world, q.kind, q.best, q.h, active, q.o, q.d, q.rng, radiance, throughput);
+    if (q.p.valid) write_rgb(G, q.p.index, add3(mk3(0.0f, 0.0f, 0.0f), radiance));
+}
+
+// The same pass for a scene of one light. With one light and one segment a pixel takes one of two colours: the
+// one with the light's terms added, if its shadow ray is not blocked, and the one without. Both are worked out
+// (as shade_hit sums them) and encoded before the shadow walk, so two words, not the shading state, are what a
+// lane holds across it, and the kernel fits in fewer registers.
+template <int WALK>
+__device__ __forceinline__ void rgb_shade_one_light_tile(const RgbArgs &G, PairLds *L, const long long tile) {
+    const ShadeScene &E = G.shade;
+    const ShadeStart q = shade_start(G, tile);
+    const float throughput = 1.0f;
+    const bool hit = q.p.valid && q.kind != OBJECT_NONE;
+    float3 radiance = mk3(0.0f, 0.0f, 0.0f);
+    if (q.p.valid && !hit) radiance = add3(radiance, mul3(sky(E, q.d), throughput));
+    float3 normal = mk3(0.0f, 1.0f, 0.0f), point = q.o;
+    float4 m0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m1 = m0;
+    if (hit) hit_material(G, q.kind, q.o, q.d, q.best, q.h, point, normal, m0, m1);
+    uint32_t rng = q.rng;
+    float3 cd, cs, light_dir;
+    float light_distance;
+    const bool need = light_terms(E, 0, hit, point, normal, q.d, m1, rng, cd, cs, light_dir, light_distance);
+    const float3 zero = mk3(0.0f, 0.0f, 0.0f);
+    float3 dark = radiance, lit = radiance;
+    if (hit) {
+        add_direct(E, m0, m1, zero, zero, throughput, dark);
+        add_direct(E, m0, m1, add3(zero, cd), add3(zero, cs), throughput, lit);
+    }
+    const uint32_t dark_bytes = rgb_bytes(G, add3(zero, dark));
+    const uint32_t lit_bytes = rgb_bytes(G, add3(zero, lit));
+    const uint64_t shadow = ballot(need);
+    bool blocked = false;
+    if (shadow)
+        blocked = occluded<WALK, OcclusionTwoChild>(G.r.s, L, q.p.world, offset_origin(point, normal, light_dir), light_dir, shadow,
+                                                    light_distance);
+    if (q.p.valid) store_rgb(G, q.p.index, need && !blocked ? lit_bytes : dark_bytes);
 }
 
 template <int WALK, int BLOCKS>
@@ -1463,10 +1547,14 @@ __global__ __launch_bounds__(256, BLOCKS) void rgb_hit_kernel(const RgbArgs G) {
     take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_hit_tile<WALK>(G, L, tile); });
 }
 
-template <int WALK, int BLOCKS>
+// ONE_LIGHT: the scene has exactly one light (the kernel built for it holds less across the shadow walk).
+template <int WALK, int BLOCKS, bool ONE_LIGHT>
 __global__ __launch_bounds__(256, BLOCKS) void rgb_shade_kernel(const RgbArgs G) {
     NGX_PAIR_LDS(L);
-    take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) { rgb_shade_tile<WALK>(G, L, tile); });
+    take_tiles(G.r.tile_counters, G.r.total_tiles, [&](const long long tile) {
+        if constexpr (ONE_LIGHT) rgb_shade_one_light_tile<WALK>(G, L, tile);
+        else rgb_shade_tile<WALK>(G, L, tile);
+    });
 }
 
 // Ray queries (render_core.h:331-349): 64 consecutive rays a wave (with xcd_remap, each XCD takes a
@@ -1836,6 +1924,17 @@ static int resident_grid(const int compute_units, const long long tiles) {
 // The two-pass picture, persistent launches of the packet and hybrid walks only: the hit pass built like the depth
 // kernel (5, 6 or 7 workgroups per compute unit: hit_blocks, 0 for the default), the shading pass like the RGB
 // kernel (4 to 7: a.blocks, 0 for 4).
+// The shading pass, built for one light where the scene has one.
+template <int WALK, int BLOCKS>
+static void launch_shade(const RgbArgs &a, hipStream_t stream, const int compute_units) {
+    if (a.shade.light_count == 1)
+        rgb_shade_kernel<WALK, BLOCKS, true><<<resident_grid<rgb_shade_kernel<WALK, BLOCKS, true>>(compute_units, a.r.total_tiles), 256, 0,
+                                               stream>>>(a);
+    else
+        rgb_shade_kernel<WALK, BLOCKS, false><<<resident_grid<rgb_shade_kernel<WALK, BLOCKS, false>>(compute_units, a.r.total_tiles), 256,
+                                                0, stream>>>(a);
+}
+
 hipError_t launch_rgb_two_pass(const RgbArgs &a, const int hit_blocks, hipStream_t stream, const int compute_units) {
     if (a.r.total_tiles <= 0) return hipSuccess;
     if (a.spp != 1 || a.shade.light_count < 0 || a.shade.light_count > MAX_LIGHTS || a.hits == nullptr ||
@@ -1851,8 +1950,8 @@ hipError_t launch_rgb_two_pass(const RgbArgs &a, const int hit_blocks, hipStream
     }
 #undef NGX_LAUNCH
     zero_counters_kernel<<<1, 8 * COUNTER_SLICES, 0, stream>>>(a.r.tile_counters);
-#define NGX_LAUNCH(W, B) rgb_shade_kernel<W, B><<<resident_grid<rgb_shade_kernel<W, B>>(compute_units, a.r.total_tiles), 256, 0, stream>>>(a)
-    switch (a.blocks ? a.blocks : SHADE_BLOCKS_DEFAULT) {
+#define NGX_LAUNCH(W, B) launch_shade<W, B>(a, stream, compute_units)
+    switch (a.blocks ? a.blocks : a.shade.light_count == 1 ? SHADE_ONE_LIGHT_BLOCKS_DEFAULT : SHADE_BLOCKS_DEFAULT) {
     case 4: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 4); else NGX_LAUNCH(WALK_PACKET, 4); break;
     case 5: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 5); else NGX_LAUNCH(WALK_PACKET, 5); break;
     case 6: if (hybrid) NGX_LAUNCH(WALK_HYBRID, 6); else NGX_LAUNCH(WALK_PACKET, 6); break;

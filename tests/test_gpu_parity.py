@@ -902,8 +902,9 @@ def test_rgb_depth_id_of_the_full_body_worlds_batch_on_a_sample_of_worlds(gpu, o
 def test_two_pass_picture_gives_the_one_pass_bytes(gpu):
     """The picture drawn in two passes (the camera rays' hits first, then their shading and shadow rays: one sample a
     pixel, no surface reflective enough to bounce a ray) against the one-pass kernel, for the packet and hybrid walks:
-    the body_worlds scene with its floor and soft light (24 worlds), and 8 worlds of meshes with vertex normals, glossy
-    materials, two spheres, a soft light and a spotlight. The hit pass walks the trees in another order, so a ray
+    the body_worlds scene with its floor and soft light (24 worlds), 8 worlds of meshes with vertex normals, glossy
+    materials, two spheres, a soft light and a spotlight, and the same meshes under the spotlight alone (scenes of one
+    light take a shading kernel of their own). The hit pass walks the trees in another order, so a ray
     through an edge two triangles share may keep the other triangle (the parity bar's one exception): all but a
     handful of pixels are the same bytes."""
     import ctypes
@@ -925,7 +926,14 @@ def test_two_pass_picture_gives_the_one_pass_bytes(gpu):
                                          ng.Light((-3.0, 4.0, -2.0), intensity=2.0, falloff=8.0, direction=(0.3, -1.0, -0.8),
                                                   spot_angle=0.6, color=(1.0, 0.8, 0.6))])
     r2.set_ambient(0.15)
-    for name, r, cfg in (("body_worlds", r1, bw), ("vertex normals and spheres", r2, iw)):
+    # one light again (the shading pass's one-light build), here a coloured spotlight with a distance falloff
+    r4 = workloads.make_renderer(iw, spheres=[ng.Sphere((0.5, -1.0, -9.0), 1.0)],
+                                 floor=ng.Plane(y=-4.0, half_extent_x=30.0, z_min=-60.0, z_max=5.0, checker_scale=0.5),
+                                 lights=[ng.Light((-3.0, 4.0, -2.0), intensity=2.0, falloff=8.0, direction=(0.3, -1.0, -0.8),
+                                                  spot_angle=0.6, color=(1.0, 0.8, 0.6))])
+    r4.set_ambient(0.15)
+    for name, r, cfg in (("body_worlds", r1, bw), ("vertex normals and spheres", r2, iw),
+                         ("vertex normals, one spotlight", r4, iw)):
         for walk in ("packet", "hybrid"):
             out = {}
             for two in (True, False):
