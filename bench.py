@@ -40,6 +40,13 @@ def render_bytes(views, width, height, instances, outputs_bytes_per_pixel=8):
     return outputs_bytes_per_pixel * views * width * height + 48 * views + 128 * instances
 
 
+def picture_bytes(views, width, height, instances):
+    """Algorithmic bytes of the RGB picture drawn in two passes: the hit pass writes a 16 B record a pixel and the
+    shading pass reads it back and writes 3 B of colour a pixel; each pass reads the cameras (48 B per view) and each
+    world's top-level tree and instance records once (64 + 64 B per instance)."""
+    return 35 * views * width * height + 2 * (48 * views + 128 * instances)
+
+
 def build_bytes(worlds, bodies, instances):
     """Algorithmic bytes of one top-level build launch: body poses in (28 B per body), per instance the
     placed matrix, the 64 B instance record, the reference-layout leaf and interior node (2 x 32 B) and the
@@ -300,7 +307,14 @@ def bench_rgb(args, torch):
 
     steps = max(5, args.steps // 2)
     sec = timed_steps(step, steps, max(2, args.warmup // 2), torch)
-    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3, trace_config=r.trace_config)
+    # the render call's kernels (depth + ID, then the picture's passes), timed by the library's events on their stream
+    kernel_ms = []
+    for _ in range(steps):
+        step()
+        torch.cuda.synchronize()
+        kernel_ms.append(r.last_kernel_ms)
+    return dict(images_per_s=cfg["worlds"] / sec, ms=sec * 1e3, trace_config=r.trace_config, kernel_ms=float(np.mean(kernel_ms)),
+                views=cfg["worlds"], instances=int(np.sum(cfg["counts"])))
 
 
 # The reference's own published batch-renderer timings on this scene (bench/results/run2/summary.txt:20-22, :41-43:
@@ -518,21 +532,25 @@ def _profile_summary(name):
         return json.load(f)
 
 
-# The trace-configuration fields a profiled kernel's traffic depends on, and its variant field with the variant a 0
-# stands for (the kernels' built-in defaults).
+# The trace-configuration fields a profiled kernel's traffic depends on, and its variant fields, each with the variant
+# a 0 stands for (the kernels' built-in defaults).
 _TRAFFIC_FIELDS = {
-    "render": (("tile_w", "xcd_remap", "packet", "pair_limit", "persistent", "lane_min_triangles"), "render_blocks", 7),
-    "backward": (("packet", "pair_limit", "lane_min_triangles"), "backward_blocks", 5),
+    "render": (("tile_w", "xcd_remap", "packet", "pair_limit", "persistent", "lane_min_triangles"), (("render_blocks", 7),)),
+    "backward": (("packet", "pair_limit", "lane_min_triangles"), (("backward_blocks", 5),)),
+    # the RGB + depth + ID call: the depth kernel and the hit pass take render_blocks, the shading pass rgb_blocks
+    # (5 for a scene of one light)
+    "rgb": (("tile_w", "xcd_remap", "packet", "pair_limit", "persistent", "lane_min_triangles", "rgb_two_pass"),
+            (("render_blocks", 7), ("rgb_blocks", 5))),
 }
 
 
 def _traffic(summary, kind, chosen):
     """HBM bytes per launch from the summary (or one of its `variants`) profiled under the configuration this run
     chose, compared on the fields the kernel depends on; else the summary's own figure. Returns (bytes, matched)."""
-    fields, variant, default = _TRAFFIC_FIELDS[kind]
+    fields, variants = _TRAFFIC_FIELDS[kind]
 
     def key(c):
-        return tuple(int(c.get(f, 0)) for f in fields) + (int(c.get(variant, 0)) or default,)
+        return tuple(int(c.get(f, 0)) for f in fields) + tuple(int(c.get(v, 0)) or d for v, d in variants)
 
     for entry in [summary] + list(summary.get("variants", [])):
         if entry.get("trace_config") is not None and key(entry["trace_config"]) == key(chosen):
@@ -686,6 +704,16 @@ def main():
     if crgb:
         line.update(rgb_depth_id_images_per_s=crgb["images_per_s"], rgb_depth_id_ms=crgb["ms"],
                     rgb_depth_id_trace_config=crgb["trace_config"])
+        rgbb = render_bytes(crgb["views"], 128, 128, crgb["instances"]) + picture_bytes(crgb["views"], 128, 128, crgb["instances"])
+        summary = _profile_summary("rgb_two_pass_body_worlds_traffic.json")
+        traffic, same = _traffic(summary, "rgb", crgb["trace_config"])
+        if traffic is not None:
+            traffic += summary["kernels"]["render_persistent_kernel"]["hbm_bytes_per_launch"]
+        achieved = rgbb / (crgb["kernel_ms"] * 1e-3) / 1e9
+        line["rgb_depth_id_roofline"] = {
+            "bound": "hbm", "kernel": "render_persistent_kernel + rgb_hit_kernel + rgb_shade_kernel (one render call)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_same_config": same, "algorithmic_bytes_per_launch": rgbb, "kernel_ms": crgb["kernel_ms"]}
     if cscene:
         line["falling_bodies_rgb"] = cscene
     line["cpu_baseline"] = cpu

@@ -55,3 +55,20 @@ def test_traffic_is_taken_from_the_profile_of_the_configuration_run(bench):
     got6, got5 = bench._traffic(bwd, "backward", six), bench._traffic(bwd, "backward", five)
     assert got6[1] and got5[1] and got6[0] != got5[0]
     assert bench._traffic(bwd, "backward", dict(six, backward_blocks=0)) == got5  # 0 is the 5-workgroup build
+    # the RGB + depth + ID call: the profiled configuration (its shading pass at 6) and not the default 5
+    rgb = bench._profile_summary("rgb_two_pass_body_worlds_traffic.json")
+    chosen = copy.deepcopy(rgb["trace_config"])
+    assert bench._traffic(rgb, "rgb", chosen) == (rgb["hbm_bytes_per_launch"], True)
+    assert bench._traffic(rgb, "rgb", dict(chosen, render_blocks=0)) == (rgb["hbm_bytes_per_launch"], True)
+    assert bench._traffic(rgb, "rgb", dict(chosen, rgb_blocks=0))[1] is False
+    assert bench._traffic(rgb, "rgb", dict(chosen, rgb_two_pass=0))[1] is False
+
+
+def test_picture_bytes(bench):
+    """The two-pass picture on body_worlds: the hit records written (16 B a pixel, 1,073.7 MB) and read back, 3 B a
+    pixel of colour, and each pass's cameras and top-level trees: the hit pass's 1,142.4 MB and the shading pass's
+    1,343.7 MB that the RGB profile quotes."""
+    hit = 16 * 4096 * 128 * 128 + 48 * 4096 + 128 * 534919
+    shade = (16 + 3) * 4096 * 128 * 128 + 48 * 4096 + 128 * 534919
+    assert (hit, shade) == (1142408064, 1343734656)
+    assert bench.picture_bytes(4096, 128, 128, 534919) == hit + shade
